@@ -1,0 +1,143 @@
+// Device-side building blocks shared by the HIP kernels: problem geometry
+// passed by value, ideal-gas physics (flux, Jacobian, Jacobian-vector
+// product) templated on the arithmetic type, and block reductions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace eg {
+constexpr int kMaxQ = 9;           // k <= 8
+constexpr int kBlock = 256;        // threads per block of the cell kernels
+constexpr int kReduceBlocks = 512; // fixed first-stage width of vector reductions
+
+// Geometry and 1-D tables of one problem. Node i of a cell is (ix, iy) =
+// (i % q, i / q), x fastest; cell e = iy*nx + ix; face 2e is the x-face with
+// e on its left (side 1 of e, side 0 of the +x neighbour), face 2e+1 the
+// y-face (side 3 of e, side 2 of the +y neighbour).
+struct Geo {
+  int nx, ny, n, q, p, m, t, nf;
+  double hx, hy, area, gamma, gm1, dt;
+  double D[kMaxQ * kMaxQ]; // D[a*q+i] = l_i'(x_a) (reference-interval derivative)
+  double e0[kMaxQ], e1[kMaxQ], w1[kMaxQ];
+};
+
+// Frozen HLL data of one face point, fixed for one implicit stage solve.
+struct Frozen {
+  double a, b, inv_sum, wl, wr, pen, pad0, pad1;
+};
+
+// ------------------------------------------------------------ physics
+template <class T> __device__ __forceinline__ T pressure(const T u[4], T gm1) {
+  return gm1 * (u[3] - T(.5) * (u[1] * u[1] + u[2] * u[2]) / u[0]);
+}
+template <class T> __device__ __forceinline__ void flux(const T u[4], int axis, T gm1, T f[4]) {
+  const T vn = u[axis + 1] / u[0], pr = pressure(u, gm1);
+  f[0] = u[axis + 1];
+  f[1] = u[1] * vn;
+  f[2] = u[2] * vn;
+  f[3] = (u[3] + pr) * vn;
+  f[axis + 1] += pr;
+}
+// Column-major 4x4 dF_axis/dU: j[c + 4*d] = dF_c/dU_d.
+template <class T> __device__ __forceinline__ void jacobian(const T u[4], int axis, T gm1, T j[16]) {
+  const T inv = T(1) / u[0];
+  const T vx = u[1] * inv, vy = u[2] * inv, vn = u[axis + 1] * inv;
+  const T h = u[3] + pressure(u, gm1);
+  const T dp[4] = {T(.5) * gm1 * (vx * vx + vy * vy), -gm1 * vx, -gm1 * vy, gm1};
+  T dv[4] = {-vn * inv, T(0), T(0), T(0)};
+  dv[axis + 1] = inv;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) { j[i] = T(0); }
+  j[0 + 4 * (axis + 1)] = T(1);
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) { j[(d + 1) + 4 * c] = u[d + 1] * dv[c] + (axis == d ? dp[c] : T(0)); }
+    j[(d + 1) + 4 * (d + 1)] += vn;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) { j[3 + 4 * c] = vn * dp[c] + h * dv[c]; }
+  j[15] += vn;
+}
+// y = dF_axis/dU (u) x without forming the matrix.
+template <class T>
+__device__ __forceinline__ void jvp(const T u[4], int axis, T gm1, const T x[4], T y[4]) {
+  const T inv = T(1) / u[0];
+  const T vx = u[1] * inv, vy = u[2] * inv, vn = u[axis + 1] * inv;
+  const T h = u[3] + pressure(u, gm1);
+  const T dpx = gm1 * (T(.5) * (vx * vx + vy * vy) * x[0] - vx * x[1] - vy * x[2] + x[3]);
+  const T dvx = (x[axis + 1] - vn * x[0]) * inv;
+  y[0] = x[axis + 1];
+  y[1] = u[1] * dvx + vn * x[1] + (axis == 0 ? dpx : T(0));
+  y[2] = u[2] * dvx + vn * x[2] + (axis == 1 ? dpx : T(0));
+  y[3] = vn * dpx + h * dvx + vn * x[3];
+}
+__device__ __forceinline__ bool positive(const double u[4], double gm1) {
+  for (int c = 0; c < 4; ++c) {
+    if (!isfinite(u[c])) { return false; }
+  }
+  return u[0] > 1e-13 && pressure(u, gm1) > 1e-13;
+}
+
+// Cell/face topology helpers.
+__device__ __forceinline__ int face_of(const Geo &g, int e, int side) {
+  const int ix = e % g.nx, iy = e / g.nx;
+  switch (side) {
+  case 0: return 2 * (iy * g.nx + (ix + g.nx - 1) % g.nx);
+  case 1: return 2 * e;
+  case 2: return 2 * (((iy + g.ny - 1) % g.ny) * g.nx + ix) + 1;
+  default: return 2 * e + 1;
+  }
+}
+__device__ __forceinline__ int neighbour_of(const Geo &g, int e, int side) {
+  const int ix = e % g.nx, iy = e / g.nx;
+  switch (side) {
+  case 0: return iy * g.nx + (ix + g.nx - 1) % g.nx;
+  case 1: return iy * g.nx + (ix + 1) % g.nx;
+  case 2: return ((iy + g.ny - 1) % g.ny) * g.nx + ix;
+  default: return ((iy + 1) % g.ny) * g.nx + ix;
+  }
+}
+// Node index of the a-th node along the normal line through face point qq of
+// a side, and the basis value of that node at the side.
+template <int Q> __device__ __forceinline__ int line_node(int side, int qq, int a) {
+  return side < 2 ? qq * Q + a : a * Q + qq;
+}
+template <int Q, class T>
+__device__ __forceinline__ T side_value(const Geo &g, int side, int a) {
+  return T((side & 1) ? g.e1[a] : g.e0[a]);
+}
+// Trace of a cell vector (4 components per node) at face point qq of a side.
+template <int Q, class T, class V>
+__device__ __forceinline__ void trace(const Geo &g, const V *cell, int side, int qq, T out[4]) {
+  out[0] = out[1] = out[2] = out[3] = T(0);
+#pragma unroll
+  for (int a = 0; a < Q; ++a) {
+    const T phi = side_value<Q, T>(g, side, a);
+    const V *v = cell + 4 * line_node<Q>(side, qq, a);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) { out[c] += phi * T(v[c]); }
+  }
+}
+
+// ------------------------------------------------------------ reductions
+template <class T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) { v += __shfl_down(v, off, 64); }
+  return v;
+}
+// Deterministic block sum (fixed tree); result valid in thread 0.
+template <class T> __device__ __forceinline__ T block_sum(T v, T *lds /* >= 16 */) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) { lds[wid] = v; }
+  __syncthreads();
+  T s = T(0);
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) { s += lds[i]; }
+  }
+  __syncthreads();
+  return s;
+}
+} // namespace eg

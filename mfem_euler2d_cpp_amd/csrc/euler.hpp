@@ -1,0 +1,209 @@
+// Host API of the MI355X implicit DG/HDG Newton update. It keeps the names,
+// argument meaning and error behaviour of the reference C++ API
+// (include/euler.hpp:29-258 of YueWu2002/mfem_euler2d_cpp): EulerGas,
+// Options/ParseOptions, Problem, LinearSolver, Run. Vectors named dev_* are
+// device pointers; all numerical work of the Newton update runs in the HIP
+// kernels of kernels_cell.hip / kernels_la.hip on the problem's stream.
+#pragma once
+#include <array>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace euler {
+inline constexpr double pi = 3.141592653589793238462643383279502884;
+using State = std::array<double, 4>;
+
+// include/euler.hpp:19-23
+inline void require(bool ok, const char *msg) {
+  if (!ok) { throw std::runtime_error(msg); }
+}
+inline void require(bool ok, const std::string &msg) {
+  if (!ok) { throw std::runtime_error(msg); }
+}
+void hip_check(hipError_t e, const char *what);
+double clock_s();
+
+// include/euler.hpp:29-43 (host physics: initial data, errors, known-answer tests)
+class EulerGas {
+  double gamma, gamma_minus_one;
+public:
+  explicit EulerGas(double value);
+  double Gamma() const { return gamma; }
+  double pressure(const State &u) const;
+  bool positive(const State &u) const;
+  State flux(const State &u, int axis) const;
+  void jacobian(const State &u, int axis, double j[16]) const;
+  std::array<double, 2> speeds(const State &l, const State &r, int axis, double safety) const;
+  State exact(double x, double y, double time, double alpha) const;
+};
+
+// include/euler.hpp:46-57, plus --precision fp64|mixed and --inner-tol for
+// the mixed-precision form.
+struct Options {
+  int nx = 100, ny = 100, k = 1, volume_order = -1, max_newton = 30;
+  int restart = 20, max_krylov = 1500, newton_steps = -1, steps = -1;
+  double alpha = 45, dt = .001, end = .2, cfl = -1, safety = 1.1, tol = 1e-12, linear_tol = 1e-12;
+  std::string assembly = "partial";
+  std::string form = "both", solver = "gmres", precond = "block-jacobi", integrator = "ssp-dirk3",
+              output = "code_test/results/run";
+  bool matrix_free = true, verify = false, save_solution = true, hdg_first = false;
+  bool check_update = false, timing_only = false;
+  std::string ilu_ordering = "none";
+  std::string precision = "fp64";
+  double inner_tol = 1e-4;
+};
+Options ParseOptions(int argc, char **argv);
+Options ParseOptions(const std::string &args);
+
+template <class T> class DeviceArray {
+public:
+  T *p = nullptr;
+  size_t n = 0;
+  DeviceArray() = default;
+  explicit DeviceArray(size_t count) { resize(count); }
+  DeviceArray(const DeviceArray &) = delete;
+  DeviceArray &operator=(const DeviceArray &) = delete;
+  ~DeviceArray() { release(); }
+  void resize(size_t count) {
+    if (count == n) { return; }
+    release();
+    if (count) { hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc"); }
+    n = count;
+  }
+  void release() {
+    if (p) { (void)hipFree(p); }
+    p = nullptr;
+    n = 0;
+  }
+  operator T *() const { return p; }
+};
+
+// Fixed BSR topology of a block ILU(0): pattern, update schedule and the
+// dependency levels of the forward and backward sweeps (src/block_ilu.cpp:10-59).
+struct BlockPattern {
+  int nb = 0, block = 0, nnzb = 0;
+  std::vector<int> row, col, diag, upd_ptr, upd;
+  std::vector<int> lower_level_ptr, lower_rows, upper_level_ptr, upper_rows;
+  DeviceArray<int> d_row, d_col, d_diag, d_upd_ptr, d_upd, d_lower_rows, d_upper_rows;
+  BlockPattern(const std::vector<std::vector<int>> &graph, int size, hipStream_t s);
+  int Find(int r, int c) const;
+};
+
+// include/euler.hpp:161-230: device-resident discretisation of one case.
+class Problem {
+public:
+  Options opt;
+  const EulerGas gas;
+  double alpha, lx, ly, hx, hy, area, dt = 0;
+  int n, p, m, t, nf, nu, nl, q;
+  std::vector<double> x1, w1, D1, end0, end1, origx, origy;
+  std::vector<double> cell_metric, trace_metric;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  eg::Geo geo{};
+  DeviceArray<eg::Frozen> frozen;
+  DeviceArray<double> old, d_cell_metric, d_trace_metric, partial, scalars;
+  DeviceArray<float> d_cell_metric32, d_trace_metric32;
+  DeviceArray<int> flags;
+  double *pinned = nullptr;
+  int *pinned_flags = nullptr;
+  int freeze_generation = 0;
+  double rate = 0;
+
+  Problem(const Options &o, const EulerGas &gas, hipStream_t s = nullptr);
+  ~Problem();
+  Problem(const Problem &) = delete;
+  std::vector<double> Initial() const;
+  void Freeze(const double *dev_u, double step, bool check_admissible = false);
+  double WaveRate() const { return rate; }
+  void SetStage(const double *dev_old, double step);
+  double CellNorm(const double *dev_r);
+  bool Admissible(const double *dev_u);
+  double Residual(const double *dev_u, double *dev_r);
+  double TrialResidual(const double *dev_u, const double *dev_du, double lambda, double *dev_trial,
+                       double *dev_r, bool &admissible);
+  std::vector<std::vector<int>> SolverGraph(bool hdg) const;
+  std::array<double, 4> Errors(const std::vector<double> &u, double time) const;
+  std::array<double, 4> Integral(const std::vector<double> &u) const;
+  std::array<double, 4> Differences(const std::vector<double> &u, const std::vector<double> &v) const;
+  void Save(const std::vector<double> &u, const std::string &prefix) const;
+  // scalar plumbing
+  void Sync();
+  int TakeFlags();
+  void CheckFlags(const char *context);
+  double ReduceScalar(const double *dev_partials, int count, int mode);
+};
+
+// Condensed HDG state of one Newton update (include/euler.hpp:138-160):
+// A_e^{-1} workspace, restriction blocks, [X | y], face panels of S_e, the
+// condensed right-hand side and the trace diagonal.
+class Hybrid {
+public:
+  Problem &p;
+  DeviceArray<double> A, C, Xy, panels, cy, D, rhs;
+  DeviceArray<float> panels32;
+  explicit Hybrid(Problem &problem, bool mixed);
+  void StreamCondense(const double *dev_u, const double *dev_r);
+  void SchurMult(const double *x, double *y);
+  void SchurMult(const float *x, float *y);
+  void Recover(const double *dev_dl, double *dev_du);
+  double RecoveredResidual(const double *dev_eps, double *dev_out);
+};
+
+// Result of one linear solve (include/euler.hpp:231-235).
+struct LinearResult {
+  int count = 0, outer = 0;
+  double residual = 0, assembly = 0, setup = 0, solve = 0, residual_norm = 0;
+};
+
+// Persistent device GMRES with block-Jacobi or block ILU(0) preconditioning
+// (include/euler.hpp:240-249, src/linear.cpp). form 0: DG body system (the
+// matrix-free Jacobian at the refreshed state); form 1: condensed Schur system.
+class LinearSolver {
+public:
+  Problem &p;
+  int form, N, block, restart;
+  bool mixed, ilu;
+  const double *state = nullptr; // DG Newton state (device)
+  Hybrid *hybrid = nullptr;
+  DeviceArray<double> V, w, z, comb, corr, res, x, act, partA, partB, blocks;
+  DeviceArray<float> V32, w32, z32, comb32, corr32, res32, x32, blocks32;
+  std::unique_ptr<BlockPattern> pattern;
+  DeviceArray<int> dslot, slot, contrib;
+  std::vector<double> h, cosine, sine, rotated, coefficients;
+
+  LinearSolver(Problem &p, int form, Hybrid *hybrid = nullptr);
+  void Refresh(const double *dev_u) { state = dev_u; }
+  void Setup();  // fill + factor the preconditioner at the current state
+  void Apply(const double *x, double *y);
+  void Apply(const float *x, float *y);
+  void Precondition(const double *b, double *y);
+  void Precondition(const float *b, float *y);
+  // x (member) is the solution; res holds rhs - A x on return.
+  void Solve(const double *dev_rhs, LinearResult &out, double tol, bool refresh);
+  double Norm(const double *v);
+private:
+  template <class T> double GMRES(const T *rhs, T *xv, double tol, int limit, int &iterations, bool zero_guess,
+                                  double *true_norm);
+  double MixedSolve(const double *rhs, double tol, int limit, int &iterations, int &outer, bool zero_guess);
+};
+
+// include/euler.hpp:250-256
+struct UpdateRecord {
+  int step, stage, newton, krylov;
+  double rms, damping, linear_residual, dg_relative, seconds;
+};
+struct RunResult {
+  std::vector<double> u;
+  std::array<double, 4> errors{};
+  double seconds = 0, drift = 0, newton_seconds = 0, final_time = 0;
+  int updates = 0, time_steps = 0;
+  std::vector<UpdateRecord> records;
+  std::vector<std::vector<double>> increments;
+};
+RunResult Run(Problem &p, const std::string &form, const std::string &prefix, bool keep_increments = false);
+} // namespace euler

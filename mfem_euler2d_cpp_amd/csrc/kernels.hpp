@@ -1,0 +1,87 @@
+// Host-side launcher declarations of the HIP kernels (gfx950).
+#pragma once
+#include "device.hpp"
+
+namespace eg {
+// Device error flags (OR-ed by kernels, read by the host after a sync).
+enum : int {
+  kErrInadmissible = 1,  // nonpositive / nonfinite node or face trace state
+  kErrFaceState = 2,     // Freeze: nonpositive old face state
+  kErrRoe = 4,           // Freeze: invalid Roe sound speed
+  kErrDegenerate = 8,    // Freeze: a + b <= 0
+  kErrSingular = 16,     // zero pivot in a dense block inverse
+  kErrNonfinite = 32,
+};
+// Largest dense block held in LDS by the Gauss-Jordan inverse (b*b doubles
+// plus 3b scratch within the 160 KB gfx950 workgroup limit).
+constexpr int kMaxLdsBlock = 140;
+
+int cell_blocks(const Geo &g);
+int freeze_blocks(const Geo &g);
+int recovered_blocks(const Geo &g);
+void launch_freeze(const Geo &g, const double *u, double safety, Frozen *fz, double *partial, int *flags,
+                   hipStream_t s);
+void launch_residual(const Geo &g, const Frozen *fz, const double *u, const double *du, double lambda,
+                     const double *old, double *r, double *trial, double *partial, int *flags, hipStream_t s);
+template <class T>
+void launch_dg_apply(const Geo &g, const Frozen *fz, const double *u, const T *x, T *y, hipStream_t s);
+void launch_dg_blocks(const Geo &g, const Frozen *fz, const double *u, double *diag, const int *dslot,
+                      double *bsr, const int *slot, hipStream_t s);
+void launch_hdg_build(const Geo &g, const Frozen *fz, const double *u, double *A, double *C, hipStream_t s);
+void launch_hdg_solve(const Geo &g, const Frozen *fz, const double *Ainv, const double *res, double *Xy,
+                      hipStream_t s);
+void launch_hdg_panels(const Geo &g, const double *Xy, const double *C, double *panels, double *cy,
+                       hipStream_t s);
+void launch_hdg_face(const Geo &g, const Frozen *fz, const double *cy, double *rhs, double *D, hipStream_t s);
+template <class T>
+void launch_schur_apply(const Geo &g, const T *panels, const double *D, const T *x, T *y, hipStream_t s);
+void launch_recover(const Geo &g, const double *Xy, const double *dl, double *du, hipStream_t s);
+void launch_recovered_residual(const Geo &g, const Frozen *fz, const double *D, const double *eps, double *out,
+                               double *partial, hipStream_t s);
+void launch_hdg_diag_blocks(const Geo &g, const double *panels, const double *D, double *blocks, hipStream_t s);
+void launch_hdg_fill(const Geo &g, const double *panels, const double *D, const int *contrib, const int *row_ptr,
+                     const int *diag_ptr, double *bsr, hipStream_t s);
+
+// Dense / block linear algebra.
+void la_init();
+void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s);
+template <class T>
+void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s);
+void launch_ilu_factor_level(double *bsr, int b, const int *rows, int nrows, const int *row_ptr,
+                             const int *diag_ptr, const int *col, const int *upd_ptr, const int *upd,
+                             int *flags, hipStream_t s);
+template <class T>
+void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
+                            const int *diag_ptr, const int *col, const T *rhs, T *w, hipStream_t s);
+template <class T>
+void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
+                            const int *diag_ptr, const int *col, T *w, hipStream_t s);
+
+// Vector kernels (Krylov). nb = kReduceBlocks partials per reduction.
+struct Coef {
+  double c[64];
+};
+template <class T>
+void launch_multidot(const T *V, long ldv, int nvec, const T *w, const T *metric, long n, double *partials,
+                     hipStream_t s);
+template <class T>
+void launch_cgs_update(const T *V, long ldv, int nvec, const double *partials_in, T *w, const T *metric, long n,
+                       double *proj_out, double *partials_out, int norm_mode, hipStream_t s);
+template <class T>
+void launch_normalize(const double *partials, T *w, T *vnext, long n, double *norm2_out, hipStream_t s);
+template <class T>
+void launch_lincomb(const T *V, long ldv, int count, const Coef &coef, T *out, long n, hipStream_t s);
+template <class T>
+void launch_dot_partials(const T *a, const T *b, const T *metric, long n, double *partials, hipStream_t s);
+void launch_sum_partials(const double *partials, int count, double *out, int mode, hipStream_t s);
+template <class A, class B>
+void launch_convert(const A *x, B *y, long n, hipStream_t s);
+template <class T>
+void launch_axpby(T *y, double a, const T *x, double b, long n, hipStream_t s); // y = a*x + b*y
+void launch_sub(const double *a, const double *b, double *out, long n, hipStream_t s); // out = a - b
+void launch_scale_copy(const double *x, double a, double *y, long n, hipStream_t s); // y = a*x
+void launch_dirk_combine(double *out, const double *base, const double *const *slopes, const double *coef,
+                         int count, long n, hipStream_t s);
+void launch_slope(const double *u, const double *base, double inv_dt, double *slope, long n, hipStream_t s);
+void launch_admissible(const Geo &g, const double *u, int *flags, hipStream_t s);
+} // namespace eg

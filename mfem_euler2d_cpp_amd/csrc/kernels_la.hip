@@ -1,0 +1,568 @@
+// Dense-block and Krylov kernels: batched partially pivoted Gauss-Jordan
+// inverses held in LDS, block GEMV for block-Jacobi, level-scheduled BSR
+// block ILU(0) factor and triangular sweeps, and the vector kernels of the
+// weighted restarted GMRES (fused CGS2 passes with deterministic two-stage
+// reductions).
+#include "kernels.hpp"
+
+namespace eg {
+namespace {
+constexpr int kLdsBytes = 163840;
+
+// Partially pivoted in-place Gauss-Jordan inverse of one b x b column-major
+// block per workgroup (pivot: first row of maximal |a|, as LAPACK idamax).
+// Mathematically the inverse of the block whose LU the reference uses
+// (src/linear.cpp:198-202, src/block_ilu.cpp:83-84).
+__global__ __launch_bounds__(256) void k_block_inverse(double *__restrict__ blocks, int b,
+                                                       const int *__restrict__ index, int *__restrict__ flags) {
+  extern __shared__ double sm[];
+  double *A = sm, *colk = sm + b * b, *rowk = colk + b;
+  int *piv = reinterpret_cast<int *>(rowk + b);
+  __shared__ int s_piv;
+  const int blk = index ? index[blockIdx.x] : blockIdx.x;
+  double *gA = blocks + size_t(blk) * b * b;
+  const int nt = blockDim.x, tid = threadIdx.x;
+  for (int i = tid; i < b * b; i += nt) { A[i] = gA[i]; }
+  __syncthreads();
+  bool singular = false;
+  for (int k = 0; k < b; ++k) {
+    if (tid < 64) {
+      double best = -1.;
+      int arg = b;
+      for (int r = k + tid; r < b; r += 64) {
+        const double v = fabs(A[r + k * b]);
+        if (v > best) { best = v; arg = r; }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_down(best, off, 64);
+        const int oa = __shfl_down(arg, off, 64);
+        if (ob > best || (ob == best && oa < arg)) { best = ob; arg = oa; }
+      }
+      if (tid == 0) { s_piv = arg; piv[k] = arg; }
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (p != k) {
+      for (int j = tid; j < b; j += nt) {
+        const double t = A[k + j * b];
+        A[k + j * b] = A[p + j * b];
+        A[p + j * b] = t;
+      }
+      __syncthreads();
+    }
+    for (int i = tid; i < b; i += nt) { colk[i] = A[i + k * b]; rowk[i] = A[k + i * b]; }
+    __syncthreads();
+    const double pivot = rowk[k];
+    if (pivot == 0.) { singular = true; }
+    const double inv = 1. / pivot;
+    for (int idx = tid; idx < b * b; idx += nt) {
+      const int i = idx % b, j = idx / b;
+      double v;
+      if (i == k) { v = j == k ? inv : rowk[j] * inv; }
+      else if (j == k) { v = -colk[i] * inv; }
+      else { v = A[idx] - colk[i] * (rowk[j] * inv); }
+      A[idx] = v;
+    }
+    __syncthreads();
+  }
+  for (int k = b - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k) {
+      for (int i = tid; i < b; i += nt) {
+        const double t = A[i + k * b];
+        A[i + k * b] = A[i + p * b];
+        A[i + p * b] = t;
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < b * b; i += nt) { gA[i] = A[i]; }
+  if (singular && tid == 0) { atomicOr(flags, kErrSingular); }
+}
+
+// y_blk = A_blk x_blk for count independent b x b blocks; one wave per block.
+template <class T>
+__global__ __launch_bounds__(256) void k_block_gemv(const T *__restrict__ blocks, int b, int count,
+                                                    const T *__restrict__ x, T *__restrict__ y) {
+  extern __shared__ unsigned char raw[];
+  T *xs = reinterpret_cast<T *>(raw);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int blk = blockIdx.x * 4 + wid;
+  const bool valid = blk < count;
+  T *xw = xs + wid * b;
+  if (valid) { for (int i = lane; i < b; i += 64) { xw[i] = x[size_t(blk) * b + i]; } }
+  __syncthreads();
+  if (valid) {
+    const T *a = blocks + size_t(blk) * b * b;
+    for (int r = lane; r < b; r += 64) {
+      T s = T(0);
+      for (int c = 0; c < b; ++c) { s += a[size_t(c) * b + r] * xw[c]; }
+      y[size_t(blk) * b + r] = s;
+    }
+  }
+}
+
+// One BSR row of ILU(0) (src/block_ilu.cpp:73-86) per workgroup, rows of one
+// dependency level in parallel. Diagonal blocks are replaced by the inverse
+// of their updated values, so L_ik = A_ik * inv(A_kk) is a block product.
+__global__ __launch_bounds__(256) void k_ilu_factor(double *__restrict__ bsr, int b, const int *__restrict__ rows,
+                                                    const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
+                                                    const int *__restrict__ col, const int *__restrict__ upd_ptr,
+                                                    const int *__restrict__ upd, int *__restrict__ flags) {
+  extern __shared__ double sm[];
+  const int bb = b * b;
+  double *Ls = sm, *As = sm + bb;
+  const int i = rows[blockIdx.x];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int ik = row_ptr[i]; ik < diag_ptr[i]; ++ik) {
+    const int k = col[ik];
+    double *aik = bsr + size_t(ik) * bb;
+    const double *dk = bsr + size_t(diag_ptr[k]) * bb;
+    for (int t = tid; t < bb; t += nt) { As[t] = aik[t]; }
+    __syncthreads();
+    for (int idx = tid; idx < bb; idx += nt) {
+      const int r = idx % b, c = idx / b;
+      double s = 0.;
+      for (int t = 0; t < b; ++t) { s += As[r + t * b] * dk[t + size_t(c) * b]; }
+      Ls[idx] = s;
+      aik[idx] = s;
+    }
+    __syncthreads();
+    for (int u = upd_ptr[ik]; u < upd_ptr[ik + 1]; ++u) {
+      double *tgt = bsr + size_t(upd[2 * u]) * bb;
+      const double *up = bsr + size_t(upd[2 * u + 1]) * bb;
+      for (int idx = tid; idx < bb; idx += nt) {
+        const int r = idx % b, c = idx / b;
+        double s = 0.;
+        for (int t = 0; t < b; ++t) { s += Ls[r + t * b] * up[t + size_t(c) * b]; }
+        tgt[idx] -= s;
+      }
+    }
+    __syncthreads();
+  }
+  // Invert the updated diagonal block in place (same algorithm as
+  // k_block_inverse, inlined: one row per workgroup).
+  double *gA = bsr + size_t(diag_ptr[i]) * bb;
+  double *A = sm, *colk = sm + bb, *rowk = colk + b;
+  int *piv = reinterpret_cast<int *>(rowk + b);
+  __shared__ int s_piv;
+  for (int t = tid; t < bb; t += nt) { A[t] = gA[t]; }
+  __syncthreads();
+  bool singular = false;
+  for (int k = 0; k < b; ++k) {
+    if (tid < 64) {
+      double best = -1.;
+      int arg = b;
+      for (int r = k + tid; r < b; r += 64) {
+        const double v = fabs(A[r + k * b]);
+        if (v > best) { best = v; arg = r; }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_down(best, off, 64);
+        const int oa = __shfl_down(arg, off, 64);
+        if (ob > best || (ob == best && oa < arg)) { best = ob; arg = oa; }
+      }
+      if (tid == 0) { s_piv = arg; piv[k] = arg; }
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (p != k) {
+      for (int j = tid; j < b; j += nt) {
+        const double t = A[k + j * b];
+        A[k + j * b] = A[p + j * b];
+        A[p + j * b] = t;
+      }
+      __syncthreads();
+    }
+    for (int r = tid; r < b; r += nt) { colk[r] = A[r + k * b]; rowk[r] = A[k + r * b]; }
+    __syncthreads();
+    const double pivot = rowk[k];
+    if (pivot == 0.) { singular = true; }
+    const double inv = 1. / pivot;
+    for (int idx = tid; idx < bb; idx += nt) {
+      const int r = idx % b, c = idx / b;
+      double v;
+      if (r == k) { v = c == k ? inv : rowk[c] * inv; }
+      else if (c == k) { v = -colk[r] * inv; }
+      else { v = A[idx] - colk[r] * (rowk[c] * inv); }
+      A[idx] = v;
+    }
+    __syncthreads();
+  }
+  for (int k = b - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k) {
+      for (int r = tid; r < b; r += nt) {
+        const double t = A[r + k * b];
+        A[r + k * b] = A[r + p * b];
+        A[r + p * b] = t;
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = tid; t < bb; t += nt) { gA[t] = A[t]; }
+  if (singular && tid == 0) { atomicOr(flags, kErrSingular); }
+}
+
+// Forward sweep of one level (src/block_ilu.cpp:87-99): w_i = b_i - sum_{j<i} L_ij w_j.
+template <class T>
+__global__ __launch_bounds__(256) void k_ilu_lower(const T *__restrict__ bsr, int b, const int *__restrict__ rows,
+                                                   const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
+                                                   const int *__restrict__ col, const T *__restrict__ rhs,
+                                                   T *__restrict__ w) {
+  const int i = rows[blockIdx.x];
+  const size_t bb = size_t(b) * b;
+  for (int r = threadIdx.x; r < b; r += blockDim.x) {
+    T s = rhs[size_t(i) * b + r];
+    for (int ij = row_ptr[i]; ij < diag_ptr[i]; ++ij) {
+      const T *a = bsr + ij * bb;
+      const T *wj = w + size_t(col[ij]) * b;
+      T acc = T(0);
+      for (int c = 0; c < b; ++c) { acc += a[size_t(c) * b + r] * wj[c]; }
+      s -= acc;
+    }
+    w[size_t(i) * b + r] = s;
+  }
+}
+// Backward sweep of one level (src/block_ilu.cpp:100-108):
+// w_i = inv(U_ii) (w_i - sum_{j>i} U_ij w_j).
+template <class T>
+__global__ __launch_bounds__(256) void k_ilu_upper(const T *__restrict__ bsr, int b, const int *__restrict__ rows,
+                                                   const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
+                                                   const int *__restrict__ col, T *__restrict__ w) {
+  extern __shared__ unsigned char raw[];
+  T *ts = reinterpret_cast<T *>(raw);
+  const int i = rows[blockIdx.x];
+  const size_t bb = size_t(b) * b;
+  for (int r = threadIdx.x; r < b; r += blockDim.x) {
+    T s = w[size_t(i) * b + r];
+    for (int ij = diag_ptr[i] + 1; ij < row_ptr[i + 1]; ++ij) {
+      const T *a = bsr + ij * bb;
+      const T *wj = w + size_t(col[ij]) * b;
+      T acc = T(0);
+      for (int c = 0; c < b; ++c) { acc += a[size_t(c) * b + r] * wj[c]; }
+      s -= acc;
+    }
+    ts[r] = s;
+  }
+  __syncthreads();
+  const T *d = bsr + diag_ptr[i] * bb;
+  for (int r = threadIdx.x; r < b; r += blockDim.x) {
+    T s = T(0);
+    for (int c = 0; c < b; ++c) { s += d[size_t(c) * b + r] * ts[c]; }
+    w[size_t(i) * b + r] = s;
+  }
+}
+
+// ------------------------------------------------------------ Krylov vectors
+// Partial sums over a fixed grid of kReduceBlocks blocks (grid-stride), so
+// every reduction has the same tree regardless of launch timing.
+constexpr int kMaxVec = 64;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ V, long ldv, int nvec,
+                                                  const T *__restrict__ w, const T *__restrict__ metric, long n,
+                                                  double *__restrict__ partials) {
+  __shared__ double red[16];
+  __shared__ double acc_s[kMaxVec];
+  for (int i0 = 0; i0 < nvec; i0 += 8) {
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+      const double wk = double(metric[k] * w[k]);
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        if (i0 + t < nvec) { acc[t] += double(V[(i0 + t) * ldv + k]) * wk; }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      if (i0 + t < nvec) {
+        const double s = block_sum(acc[t], red);
+        if (threadIdx.x == 0) { acc_s[i0 + t] = s; }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nvec; i += blockDim.x) { partials[size_t(i) * gridDim.x + blockIdx.x] = acc_s[i]; }
+}
+
+// Reduce partials[nvec][kReduceBlocks] -> proj (LDS) redundantly in every block.
+__device__ void reduce_partials(const double *__restrict__ partials, int nvec, double *proj) {
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  for (int i = wid; i < nvec; i += nw) {
+    double s = 0.;
+    for (int k = lane; k < kReduceBlocks; k += 64) { s += partials[size_t(i) * kReduceBlocks + k]; }
+    s = wave_sum(s);
+    if (lane == 0) { proj[i] = s; }
+  }
+  __syncthreads();
+}
+
+// w -= V proj (proj from the previous pass's partials), then either the next
+// pass's partial dots with V (norm_mode 0) or the partial metric norm of w (1).
+// Every thread re-reads only the entries it wrote itself.
+template <class T>
+__global__ __launch_bounds__(256) void k_cgs_update(const T *__restrict__ V, long ldv, int nvec,
+                                                    const double *__restrict__ partials_in, T *__restrict__ w,
+                                                    const T *__restrict__ metric, long n, double *__restrict__ proj_out,
+                                                    double *__restrict__ partials_out, int norm_mode) {
+  __shared__ double proj[kMaxVec];
+  __shared__ double red[16];
+  __shared__ double acc_s[kMaxVec];
+  reduce_partials(partials_in, nvec, proj);
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < nvec; i += blockDim.x) { proj_out[i] = proj[i]; }
+  }
+  const long first = blockIdx.x * long(blockDim.x) + threadIdx.x, stride = long(gridDim.x) * blockDim.x;
+  double nacc = 0.;
+  for (long k = first; k < n; k += stride) {
+    T s = T(0);
+    for (int i = 0; i < nvec; ++i) { s += V[i * ldv + k] * T(proj[i]); }
+    const T wk = w[k] - s;
+    w[k] = wk;
+    nacc += double(metric[k] * wk * wk);
+  }
+  int nout = 1;
+  if (norm_mode) {
+    const double sn = block_sum(nacc, red);
+    if (threadIdx.x == 0) { acc_s[0] = sn; }
+  } else {
+    nout = nvec;
+    for (int i0 = 0; i0 < nvec; i0 += 8) {
+      double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (long k = first; k < n; k += stride) {
+        const double mw = double(metric[k] * w[k]);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          if (i0 + t < nvec) { acc[t] += double(V[(i0 + t) * ldv + k]) * mw; }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        if (i0 + t < nvec) {
+          const double sd = block_sum(acc[t], red);
+          if (threadIdx.x == 0) { acc_s[i0 + t] = sd; }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nout; i += blockDim.x) { partials_out[size_t(i) * gridDim.x + blockIdx.x] = acc_s[i]; }
+}
+
+// next = sqrt(sum partials); vnext = w * (1/next) if next > 0 (src/linear.cpp:117-122).
+template <class T>
+__global__ __launch_bounds__(256) void k_normalize(const double *__restrict__ partials, T *__restrict__ w,
+                                                   T *__restrict__ vnext, long n, double *__restrict__ norm2_out) {
+  __shared__ double proj[1];
+  reduce_partials(partials, 1, proj);
+  const double norm2 = proj[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) { norm2_out[0] = norm2; }
+  const double next = sqrt(norm2);
+  if (next > 0) {
+    const T inv = T(1 / next);
+    for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+      vnext[k] = w[k] * inv;
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_lincomb(const T *__restrict__ V, long ldv, int count, Coef coef,
+                                                 T *__restrict__ out, long n) {
+  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+    T s = T(0);
+    for (int i = 0; i < count; ++i) { s += V[i * ldv + k] * T(coef.c[i]); }
+    out[k] = s;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_dot_partials(const T *__restrict__ a, const T *__restrict__ b,
+                                                      const T *__restrict__ metric, long n,
+                                                      double *__restrict__ partials) {
+  __shared__ double red[16];
+  double acc = 0.;
+  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+    acc += double(metric[k] * a[k] * b[k]);
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) { partials[blockIdx.x] = s; }
+}
+
+// mode 0: sum, mode 1: max.
+__global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partials, int count,
+                                                      double *__restrict__ out, int mode) {
+  __shared__ double red[16];
+  double acc = 0.;
+  for (int k = threadIdx.x; k < count; k += blockDim.x) {
+    acc = mode ? fmax(acc, partials[k]) : acc + partials[k];
+  }
+  if (mode) {
+    for (int off = 32; off > 0; off >>= 1) { acc = fmax(acc, __shfl_down(acc, off, 64)); }
+    if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6] = acc; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = 0.;
+      for (int i = 0; i < int(blockDim.x >> 6); ++i) { m = fmax(m, red[i]); }
+      out[0] = m;
+    }
+  } else {
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) { out[0] = s; }
+  }
+}
+
+template <class A, class B>
+__global__ void k_convert(const A *__restrict__ x, B *__restrict__ y, long n) {
+  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+    y[k] = B(x[k]);
+  }
+}
+template <class T>
+__global__ void k_axpby(T *__restrict__ y, double a, const T *__restrict__ x, double b, long n) {
+  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+    y[k] = b == 0. ? T(a) * x[k] : T(a) * x[k] + T(b) * y[k];
+  }
+}
+__global__ void k_sub(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ out, long n) {
+  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+    out[k] = a[k] - b[k];
+  }
+}
+__global__ void k_scale_copy(const double *__restrict__ x, double a, double *__restrict__ y, long n) {
+  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+    y[k] = x[k] * a;
+  }
+}
+struct SlopeSet {
+  const double *s[4];
+  double c[4];
+};
+// out = base + sum_j c_j slope_j, accumulated in j order (src/stepper.cpp:99-104,246-250).
+__global__ void k_dirk_combine(double *__restrict__ out, const double *__restrict__ base, SlopeSet set, int count,
+                               long n) {
+  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+    double v = base[k];
+    for (int j = 0; j < count; ++j) { v += set.c[j] * set.s[j][k]; }
+    out[k] = v;
+  }
+}
+// K_i = (U_i - base) * (1/(a_ii dt)) (src/stepper.cpp:242-244)
+__global__ void k_slope(const double *__restrict__ u, const double *__restrict__ base, double inv_dt,
+                        double *__restrict__ slope, long n) {
+  for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+    slope[k] = (u[k] - base[k]) * inv_dt;
+  }
+}
+int grid_for(long n) {
+  const long b = (n + 255) / 256;
+  return int(b < 2048 ? (b < 1 ? 1 : b) : 2048);
+}
+} // namespace
+
+void la_init() {
+  hipFuncSetAttribute(reinterpret_cast<const void *>(k_block_inverse), hipFuncAttributeMaxDynamicSharedMemorySize,
+                      kLdsBytes);
+  hipFuncSetAttribute(reinterpret_cast<const void *>(k_ilu_factor), hipFuncAttributeMaxDynamicSharedMemorySize,
+                      kLdsBytes);
+}
+void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
+  const size_t lds = size_t(b) * b * 8 + 2 * b * 8 + b * 4;
+  k_block_inverse<<<count, 256, lds, s>>>(blocks, b, index, flags);
+}
+template <class T>
+void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s) {
+  k_block_gemv<T><<<(count + 3) / 4, 256, 4 * b * sizeof(T), s>>>(blocks, b, count, x, y);
+}
+template void launch_block_gemv<double>(const double *, int, int, const double *, double *, hipStream_t);
+template void launch_block_gemv<float>(const float *, int, int, const float *, float *, hipStream_t);
+
+void launch_ilu_factor_level(double *bsr, int b, const int *rows, int nrows, const int *row_ptr,
+                             const int *diag_ptr, const int *col, const int *upd_ptr, const int *upd, int *flags,
+                             hipStream_t s) {
+  const size_t lds = 2 * size_t(b) * b * 8;
+  k_ilu_factor<<<nrows, 256, lds, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, upd_ptr, upd, flags);
+}
+template <class T>
+void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
+                            const int *diag_ptr, const int *col, const T *rhs, T *w, hipStream_t s) {
+  k_ilu_lower<T><<<nrows, b <= 64 ? 64 : (b <= 128 ? 128 : 256), 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w);
+}
+template <class T>
+void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
+                            const int *diag_ptr, const int *col, T *w, hipStream_t s) {
+  k_ilu_upper<T><<<nrows, b <= 64 ? 64 : (b <= 128 ? 128 : 256), b * sizeof(T), s>>>(bsr, b, rows, row_ptr,
+                                                                                     diag_ptr, col, w);
+}
+template void launch_ilu_lower_level<double>(const double *, int, const int *, int, const int *, const int *,
+                                             const int *, const double *, double *, hipStream_t);
+template void launch_ilu_lower_level<float>(const float *, int, const int *, int, const int *, const int *,
+                                            const int *, const float *, float *, hipStream_t);
+template void launch_ilu_upper_level<double>(const double *, int, const int *, int, const int *, const int *,
+                                             const int *, double *, hipStream_t);
+template void launch_ilu_upper_level<float>(const float *, int, const int *, int, const int *, const int *,
+                                            const int *, float *, hipStream_t);
+
+template <class T>
+void launch_multidot(const T *V, long ldv, int nvec, const T *w, const T *metric, long n, double *partials,
+                     hipStream_t s) {
+  k_multidot<T><<<kReduceBlocks, 256, 0, s>>>(V, ldv, nvec, w, metric, n, partials);
+}
+template <class T>
+void launch_cgs_update(const T *V, long ldv, int nvec, const double *partials_in, T *w, const T *metric, long n,
+                       double *proj_out, double *partials_out, int norm_mode, hipStream_t s) {
+  k_cgs_update<T><<<kReduceBlocks, 256, 0, s>>>(V, ldv, nvec, partials_in, w, metric, n, proj_out, partials_out,
+                                                norm_mode);
+}
+template <class T>
+void launch_normalize(const double *partials, T *w, T *vnext, long n, double *norm2_out, hipStream_t s) {
+  k_normalize<T><<<grid_for(n), 256, 0, s>>>(partials, w, vnext, n, norm2_out);
+}
+template <class T>
+void launch_lincomb(const T *V, long ldv, int count, const Coef &coef, T *out, long n, hipStream_t s) {
+  k_lincomb<T><<<grid_for(n), 256, 0, s>>>(V, ldv, count, coef, out, n);
+}
+template <class T>
+void launch_dot_partials(const T *a, const T *b, const T *metric, long n, double *partials, hipStream_t s) {
+  k_dot_partials<T><<<kReduceBlocks, 256, 0, s>>>(a, b, metric, n, partials);
+}
+void launch_sum_partials(const double *partials, int count, double *out, int mode, hipStream_t s) {
+  k_sum_partials<<<1, 256, 0, s>>>(partials, count, out, mode);
+}
+template <class A, class B> void launch_convert(const A *x, B *y, long n, hipStream_t s) {
+  k_convert<A, B><<<grid_for(n), 256, 0, s>>>(x, y, n);
+}
+template <class T> void launch_axpby(T *y, double a, const T *x, double b, long n, hipStream_t s) {
+  k_axpby<T><<<grid_for(n), 256, 0, s>>>(y, a, x, b, n);
+}
+void launch_sub(const double *a, const double *b, double *out, long n, hipStream_t s) {
+  k_sub<<<grid_for(n), 256, 0, s>>>(a, b, out, n);
+}
+void launch_scale_copy(const double *x, double a, double *y, long n, hipStream_t s) {
+  k_scale_copy<<<grid_for(n), 256, 0, s>>>(x, a, y, n);
+}
+void launch_dirk_combine(double *out, const double *base, const double *const *slopes, const double *coef, int count,
+                         long n, hipStream_t s) {
+  SlopeSet set{};
+  for (int j = 0; j < count; ++j) { set.s[j] = slopes[j]; set.c[j] = coef[j]; }
+  k_dirk_combine<<<grid_for(n), 256, 0, s>>>(out, base, set, count, n);
+}
+void launch_slope(const double *u, const double *base, double inv_dt, double *slope, long n, hipStream_t s) {
+  k_slope<<<grid_for(n), 256, 0, s>>>(u, base, inv_dt, slope, n);
+}
+
+#define EG_INST_T(T)                                                                                          \
+  template void launch_multidot<T>(const T *, long, int, const T *, const T *, long, double *, hipStream_t);   \
+  template void launch_cgs_update<T>(const T *, long, int, const double *, T *, const T *, long, double *,     \
+                                     double *, int, hipStream_t);                                             \
+  template void launch_normalize<T>(const double *, T *, T *, long, double *, hipStream_t);                   \
+  template void launch_lincomb<T>(const T *, long, int, const Coef &, T *, long, hipStream_t);                \
+  template void launch_dot_partials<T>(const T *, const T *, const T *, long, double *, hipStream_t);         \
+  template void launch_axpby<T>(T *, double, const T *, double, long, hipStream_t);
+EG_INST_T(double)
+EG_INST_T(float)
+template void launch_convert<double, float>(const double *, float *, long, hipStream_t);
+template void launch_convert<float, double>(const float *, double *, long, hipStream_t);
+template void launch_convert<double, double>(const double *, double *, long, hipStream_t);
+} // namespace eg
