@@ -1,0 +1,220 @@
+// extern "C" boundary (include/eg_abi.h) over the euler:: host API.
+#include <cstring>
+#include <memory>
+
+#include "../../include/eg_abi.h"
+#include "euler.hpp"
+
+using namespace euler;
+
+struct eg_ctx {
+  Options opt;
+  EulerGas gas;
+  std::unique_ptr<Problem> problem;
+  std::unique_ptr<Hybrid> hybrid;
+  std::unique_ptr<LinearSolver> solver[2];
+  const double *state = nullptr;
+  std::string error;
+  eg_ctx(const Options &o, double gamma) : opt(o), gas(gamma) {}
+  Hybrid &hyb() {
+    if (!hybrid) { hybrid = std::make_unique<Hybrid>(*problem, opt.precision == "mixed"); }
+    return *hybrid;
+  }
+  LinearSolver &lin(int form) {
+    require(form == 0 || form == 1, "form must be 0 (dg) or 1 (hdg)");
+    if (!solver[form]) {
+      solver[form] = std::make_unique<LinearSolver>(*problem, form, form ? &hyb() : nullptr);
+    }
+    if (form == 0) { solver[0]->Refresh(state); }
+    return *solver[form];
+  }
+};
+
+namespace {
+thread_local std::string create_error;
+template <class F> int guard(eg_ctx *ctx, F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception &e) {
+    if (ctx) { ctx->error = e.what(); }
+    else { create_error = e.what(); }
+    return 1;
+  }
+}
+} // namespace
+
+extern "C" {
+int eg_create(const char *args, double gamma, void *stream, eg_ctx **out) {
+  *out = nullptr;
+  return guard(nullptr, [&] {
+    auto ctx = std::make_unique<eg_ctx>(ParseOptions(std::string(args ? args : "")), gamma);
+    ctx->problem = std::make_unique<Problem>(ctx->opt, ctx->gas, static_cast<hipStream_t>(stream));
+    *out = ctx.release();
+  });
+}
+void eg_destroy(eg_ctx *ctx) { delete ctx; }
+const char *eg_last_error(const eg_ctx *ctx) { return ctx ? ctx->error.c_str() : create_error.c_str(); }
+int eg_sizes(const eg_ctx *ctx, int *s) {
+  const auto &p = *ctx->problem;
+  const int v[8] = {p.n, p.p, p.m, p.t, p.nf, p.nu, p.nl, p.q};
+  std::memcpy(s, v, sizeof v);
+  return 0;
+}
+int eg_geometry(const eg_ctx *ctx, double *g) {
+  const auto &p = *ctx->problem;
+  const double v[6] = {p.hx, p.hy, p.area, p.lx, p.ly, p.dt};
+  std::memcpy(g, v, sizeof v);
+  return 0;
+}
+int eg_tables(const eg_ctx *ctx, double *x, double *w, double *d, double *e0, double *e1) {
+  const auto &p = *ctx->problem;
+  for (int i = 0; i < p.q; ++i) { x[i] = p.x1[i]; w[i] = p.w1[i]; e0[i] = p.end0[i]; e1[i] = p.end1[i]; }
+  std::memcpy(d, p.D1.data(), sizeof(double) * p.q * p.q);
+  return 0;
+}
+int eg_sync(eg_ctx *ctx) { return guard(ctx, [&] { ctx->problem->Sync(); }); }
+int eg_initial(eg_ctx *ctx, double *host_u) {
+  return guard(ctx, [&] {
+    const auto u = ctx->problem->Initial();
+    std::memcpy(host_u, u.data(), sizeof(double) * u.size());
+  });
+}
+int eg_freeze(eg_ctx *ctx, const double *dev_u, double dt, double *rate, double *host_frozen) {
+  return guard(ctx, [&] {
+    auto &p = *ctx->problem;
+    p.Freeze(dev_u, dt, false);
+    if (rate) { *rate = p.WaveRate(); }
+    if (host_frozen) {
+      std::vector<eg::Frozen> fz(size_t(p.nf) * p.q);
+      hip_check(hipMemcpy(fz.data(), p.frozen, sizeof(eg::Frozen) * fz.size(), hipMemcpyDeviceToHost), "download");
+      for (size_t i = 0; i < fz.size(); ++i) {
+        const double v[6] = {fz[i].a, fz[i].b, fz[i].inv_sum, fz[i].wl, fz[i].wr, fz[i].pen};
+        std::memcpy(host_frozen + 6 * i, v, sizeof v);
+      }
+    }
+  });
+}
+int eg_set_stage(eg_ctx *ctx, const double *dev_old, double dt) {
+  return guard(ctx, [&] { ctx->problem->SetStage(dev_old, dt); });
+}
+int eg_residual(eg_ctx *ctx, const double *dev_u, double *dev_r, double *norm) {
+  return guard(ctx, [&] {
+    const double v = ctx->problem->Residual(dev_u, dev_r);
+    if (norm) { *norm = v; }
+  });
+}
+int eg_trial_residual(eg_ctx *ctx, const double *dev_u, const double *dev_du, double lambda, double *dev_trial,
+                      double *dev_r, double *norm, int *admissible) {
+  return guard(ctx, [&] {
+    bool adm = false;
+    const double v = ctx->problem->TrialResidual(dev_u, dev_du, lambda, dev_trial, dev_r, adm);
+    if (norm) { *norm = v; }
+    if (admissible) { *admissible = adm ? 1 : 0; }
+  });
+}
+int eg_cell_norm(eg_ctx *ctx, const double *dev_v, double *out) {
+  return guard(ctx, [&] { *out = ctx->problem->CellNorm(dev_v); });
+}
+int eg_admissible(eg_ctx *ctx, const double *dev_u, int *ok) {
+  return guard(ctx, [&] { *ok = ctx->problem->Admissible(dev_u) ? 1 : 0; });
+}
+int eg_dg_refresh(eg_ctx *ctx, const double *dev_u) {
+  return guard(ctx, [&] {
+    ctx->state = dev_u;
+    if (ctx->solver[0]) { ctx->solver[0]->Refresh(dev_u); }
+  });
+}
+int eg_dg_apply(eg_ctx *ctx, const double *dev_x, double *dev_y) {
+  return guard(ctx, [&] {
+    require(ctx->state != nullptr, "DG operator has no refreshed state");
+    auto &p = *ctx->problem;
+    eg::launch_dg_apply<double>(p.geo, p.frozen, ctx->state, dev_x, dev_y, p.stream);
+  });
+}
+int eg_dg_apply_f32(eg_ctx *ctx, const float *dev_x, float *dev_y) {
+  return guard(ctx, [&] {
+    require(ctx->state != nullptr, "DG operator has no refreshed state");
+    auto &p = *ctx->problem;
+    eg::launch_dg_apply<float>(p.geo, p.frozen, ctx->state, dev_x, dev_y, p.stream);
+  });
+}
+int eg_pc_setup(eg_ctx *ctx, int form) { return guard(ctx, [&] { ctx->lin(form).Setup(); }); }
+int eg_pc_apply(eg_ctx *ctx, int form, const double *dev_x, double *dev_y) {
+  return guard(ctx, [&] { ctx->lin(form).Precondition(dev_x, dev_y); });
+}
+int eg_hdg_condense(eg_ctx *ctx, const double *dev_u, const double *dev_r) {
+  return guard(ctx, [&] { ctx->hyb().StreamCondense(dev_u, dev_r); });
+}
+int eg_hdg_rhs(eg_ctx *ctx, double *dev_rhs, double *dev_D) {
+  return guard(ctx, [&] {
+    auto &h = ctx->hyb();
+    auto &p = *ctx->problem;
+    if (dev_rhs) { hip_check(hipMemcpyAsync(dev_rhs, h.rhs, sizeof(double) * p.nl, hipMemcpyDeviceToDevice, p.stream), "copy"); }
+    if (dev_D) { hip_check(hipMemcpyAsync(dev_D, h.D, sizeof(double) * p.nl, hipMemcpyDeviceToDevice, p.stream), "copy"); }
+  });
+}
+int eg_hdg_xy(eg_ctx *ctx, double *dev_xy) {
+  return guard(ctx, [&] {
+    auto &h = ctx->hyb();
+    hip_check(hipMemcpyAsync(dev_xy, h.Xy, sizeof(double) * h.Xy.n, hipMemcpyDeviceToDevice, ctx->problem->stream),
+              "copy");
+  });
+}
+int eg_schur_apply(eg_ctx *ctx, const double *dev_x, double *dev_y) {
+  return guard(ctx, [&] { ctx->hyb().SchurMult(dev_x, dev_y); });
+}
+int eg_recover(eg_ctx *ctx, const double *dev_dl, double *dev_du) {
+  return guard(ctx, [&] { ctx->hyb().Recover(dev_dl, dev_du); });
+}
+int eg_recovered_residual(eg_ctx *ctx, const double *dev_eps, double *dev_out, double *norm) {
+  return guard(ctx, [&] {
+    const double v = ctx->hyb().RecoveredResidual(dev_eps, dev_out);
+    if (norm) { *norm = v; }
+  });
+}
+int eg_linear_solve(eg_ctx *ctx, int form, const double *dev_rhs, double *dev_x, double *dev_res, double tol,
+                    int refresh, eg_linear_stats *stats) {
+  return guard(ctx, [&] {
+    auto &s = ctx->lin(form);
+    auto &p = *ctx->problem;
+    LinearResult r;
+    s.Solve(dev_rhs, r, tol, refresh != 0);
+    hip_check(hipMemcpyAsync(dev_x, s.x, sizeof(double) * s.N, hipMemcpyDeviceToDevice, p.stream), "copy");
+    if (dev_res) { hip_check(hipMemcpyAsync(dev_res, s.res, sizeof(double) * s.N, hipMemcpyDeviceToDevice, p.stream), "copy"); }
+    p.Sync();
+    if (stats) {
+      stats->iterations = r.count;
+      stats->outer = r.outer;
+      stats->relative_residual = r.residual;
+      stats->residual_norm = r.residual_norm;
+      stats->setup_s = r.setup;
+      stats->solve_s = r.solve;
+    }
+  });
+}
+int eg_run(eg_ctx *ctx, int form, const char *prefix, double *host_u_final, eg_run_stats *stats,
+           double *host_increments, int max_updates, int *krylov) {
+  return guard(ctx, [&] {
+    auto &p = *ctx->problem;
+    auto r = Run(p, form ? "hdg" : "dg", prefix ? prefix : "", host_increments != nullptr);
+    std::memcpy(host_u_final, r.u.data(), sizeof(double) * p.nu);
+    if (stats) {
+      stats->updates = r.updates;
+      stats->time_steps = r.time_steps;
+      stats->newton_seconds = r.newton_seconds;
+      stats->final_time = r.final_time;
+      stats->drift = r.drift;
+      stats->seconds = r.seconds;
+      for (int c = 0; c < 4; ++c) { stats->errors[c] = r.errors[c]; }
+    }
+    const int nrec = std::min<int>(r.updates, max_updates);
+    if (host_increments) {
+      for (int i = 0; i < nrec; ++i) {
+        std::memcpy(host_increments + size_t(i) * p.nu, r.increments[i].data(), sizeof(double) * p.nu);
+      }
+    }
+    if (krylov) { for (int i = 0; i < nrec; ++i) { krylov[i] = r.records[i].krylov; } }
+  });
+}
+}

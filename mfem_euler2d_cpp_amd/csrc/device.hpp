@@ -27,10 +27,10 @@ struct Frozen {
 };
 
 // ------------------------------------------------------------ physics
-template <class T> __device__ __forceinline__ T pressure(const T u[4], T gm1) {
+template <class T> __host__ __device__ __forceinline__ T pressure(const T u[4], T gm1) {
   return gm1 * (u[3] - T(.5) * (u[1] * u[1] + u[2] * u[2]) / u[0]);
 }
-template <class T> __device__ __forceinline__ void flux(const T u[4], int axis, T gm1, T f[4]) {
+template <class T> __host__ __device__ __forceinline__ void flux(const T u[4], int axis, T gm1, T f[4]) {
   const T vn = u[axis + 1] / u[0], pr = pressure(u, gm1);
   f[0] = u[axis + 1];
   f[1] = u[1] * vn;
@@ -39,7 +39,7 @@ template <class T> __device__ __forceinline__ void flux(const T u[4], int axis, 
   f[axis + 1] += pr;
 }
 // Column-major 4x4 dF_axis/dU: j[c + 4*d] = dF_c/dU_d.
-template <class T> __device__ __forceinline__ void jacobian(const T u[4], int axis, T gm1, T j[16]) {
+template <class T> __host__ __device__ __forceinline__ void jacobian(const T u[4], int axis, T gm1, T j[16]) {
   const T inv = T(1) / u[0];
   const T vx = u[1] * inv, vy = u[2] * inv, vn = u[axis + 1] * inv;
   const T h = u[3] + pressure(u, gm1);

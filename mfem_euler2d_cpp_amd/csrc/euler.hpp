@@ -113,6 +113,7 @@ public:
   int *pinned_flags = nullptr;
   int freeze_generation = 0;
   double rate = 0;
+  double last_state_norm = 0; // CellNorm of the state of the last (Trial)Residual
 
   Problem(const Options &o, const EulerGas &gas, hipStream_t s = nullptr);
   ~Problem();
@@ -171,7 +172,7 @@ public:
   const double *state = nullptr; // DG Newton state (device)
   Hybrid *hybrid = nullptr;
   DeviceArray<double> V, w, z, comb, corr, res, x, act, partA, partB, blocks;
-  DeviceArray<float> V32, w32, z32, comb32, corr32, res32, x32, blocks32;
+  DeviceArray<float> V32, w32, z32, comb32, corr32, res32, x32, b32, blocks32;
   std::unique_ptr<BlockPattern> pattern;
   DeviceArray<int> dslot, slot, contrib;
   std::vector<double> h, cosine, sine, rotated, coefficients;

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_residual(Geo g, const Frozen *__restric
     }
   }
   __syncthreads();
-  double contrib = 0.;
+  double contrib = 0., snorm = 0.;
   if (active) {
     double acc[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -169,6 +169,7 @@ __global__ __launch_bounds__(256) void k_residual(Geo g, const Frozen *__restric
       const double rc = (acc[c] * im + s[c]) - old[off + c];
       r[off + c] = rc;
       contrib += metric * rc * rc;
+      snorm += metric * s[c] * s[c];
     }
     if (trial) {
 #pragma unroll
@@ -177,7 +178,28 @@ __global__ __launch_bounds__(256) void k_residual(Geo g, const Frozen *__restric
   }
   if (!ok) { atomicOr(flags, kErrInadmissible); }
   const double sum = block_sum(contrib, red);
-  if (tid == 0 && partial) { partial[blockIdx.x] = sum; }
+  const double ssum = block_sum(snorm, red);
+  if (tid == 0 && partial) {
+    partial[blockIdx.x] = sum;
+    partial[gridDim.x + blockIdx.x] = ssum;
+  }
+}
+
+// src/problem.cpp:259-275: positivity at every node and face trace.
+template <int Q>
+__global__ __launch_bounds__(256) void k_admissible(Geo g, const double *__restrict__ u, int *__restrict__ flags) {
+  constexpr int P = Q * Q, PT = P + 4 * Q;
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= g.n * PT) { return; }
+  const int e = id / PT, k = id % PT;
+  double s[4];
+  if (k < P) {
+    for (int c = 0; c < 4; ++c) { s[c] = u[size_t(e) * g.m + 4 * k + c]; }
+  } else {
+    const int fp = k - P;
+    trace<Q, double>(g, u + size_t(e) * g.m, fp / Q, fp % Q, s);
+  }
+  if (!positive(s, g.gm1)) { atomicOr(flags, kErrInadmissible); }
 }
 
 // src/partial.cpp:31-135: y = x + M^{-1}[-D_x^T(dt w J_x x) - D_y^T(dt w J_y x)
@@ -655,6 +677,10 @@ void launch_freeze(const Geo &g, const double *u, double safety, Frozen *fz, dou
 }
 int freeze_blocks(const Geo &g) { return (g.nf * g.q + 255) / 256; }
 
+void launch_admissible(const Geo &g, const double *u, int *flags, hipStream_t s) {
+  const int nb = (g.n * (g.p + 4 * g.q) + 255) / 256;
+  EG_DISPATCH_Q(g.q, (k_admissible<Q><<<nb, 256, 0, s>>>(g, u, flags)));
+}
 void launch_residual(const Geo &g, const Frozen *fz, const double *u, const double *du, double lambda,
                      const double *old, double *r, double *trial, double *partial, int *flags, hipStream_t s) {
   const int nb = cell_blocks(g);

@@ -391,9 +391,12 @@ __global__ __launch_bounds__(256) void k_dot_partials(const T *__restrict__ a, c
 }
 
 // mode 0: sum, mode 1: max.
+// One block per row of count partials: out[row] = sum (mode 0) or max (mode 1).
 __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partials, int count,
                                                       double *__restrict__ out, int mode) {
   __shared__ double red[16];
+  partials += size_t(blockIdx.x) * count;
+  out += blockIdx.x;
   double acc = 0.;
   for (int k = threadIdx.x; k < count; k += blockDim.x) {
     acc = mode ? fmax(acc, partials[k]) : acc + partials[k];
@@ -462,9 +465,9 @@ int grid_for(long n) {
 } // namespace
 
 void la_init() {
-  hipFuncSetAttribute(reinterpret_cast<const void *>(k_block_inverse), hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_block_inverse), hipFuncAttributeMaxDynamicSharedMemorySize,
                       kLdsBytes);
-  hipFuncSetAttribute(reinterpret_cast<const void *>(k_ilu_factor), hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_ilu_factor), hipFuncAttributeMaxDynamicSharedMemorySize,
                       kLdsBytes);
 }
 void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
@@ -527,8 +530,8 @@ template <class T>
 void launch_dot_partials(const T *a, const T *b, const T *metric, long n, double *partials, hipStream_t s) {
   k_dot_partials<T><<<kReduceBlocks, 256, 0, s>>>(a, b, metric, n, partials);
 }
-void launch_sum_partials(const double *partials, int count, double *out, int mode, hipStream_t s) {
-  k_sum_partials<<<1, 256, 0, s>>>(partials, count, out, mode);
+void launch_sum_partials(const double *partials, int count, double *out, int mode, hipStream_t s, int rows) {
+  k_sum_partials<<<rows, 256, 0, s>>>(partials, count, out, mode);
 }
 template <class A, class B> void launch_convert(const A *x, B *y, long n, hipStream_t s) {
   k_convert<A, B><<<grid_for(n), 256, 0, s>>>(x, y, n);

@@ -1,0 +1,321 @@
+// Device-resident weighted restarted GMRES (src/linear.cpp:76-157) with
+// block-Jacobi or block ILU(0) preconditioning, for the DG body system and
+// the condensed Schur system. Krylov vectors, operator actions, the
+// preconditioner and all reductions run on the GPU; only the (j+1)-vectors
+// of CGS2 projections and the norms cross to the host, where the Hessenberg
+// Givens rotations are applied exactly as in the reference.
+//
+// precision "mixed": an FP64 refinement loop (true residual with the FP64
+// operator) around an inner GMRES whose Krylov basis, operator action and
+// preconditioner run in FP32.
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#include "euler.hpp"
+
+namespace euler {
+
+// src/linear.cpp:27-57,162-166
+LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
+    : p(prob), form(f), N(f == 0 ? prob.nu : prob.nl), block(f == 0 ? prob.m : prob.t),
+      restart(std::min(N, prob.opt.restart)), mixed(prob.opt.precision == "mixed"),
+      ilu(prob.opt.precond == "block-ilu0"), hybrid(hyb) {
+  require(p.opt.solver == "gmres", "Invalid solver");
+  require(restart + 1 <= 64, "GMRES restart above 63 is not supported by the device solver");
+  require(form == 0 || hybrid != nullptr, "Schur solver needs its condensed state");
+  if (form == 0) { require(p.m <= eg::kMaxLdsBlock, "DG block preconditioners on the GPU support k <= 4 in this build"); }
+  V.resize(size_t(N) * (restart + 1));
+  for (auto *v : {&w, &z, &comb, &corr, &res, &x, &act}) { v->resize(N); }
+  partA.resize(size_t(64) * eg::kReduceBlocks);
+  partB.resize(size_t(64) * eg::kReduceBlocks);
+  if (mixed) {
+    V32.resize(size_t(N) * (restart + 1));
+    for (auto *v : {&w32, &z32, &comb32, &corr32, &res32, &x32, &b32}) { v->resize(N); }
+  }
+  const int nblk = N / block;
+  if (ilu) {
+    pattern = std::make_unique<BlockPattern>(p.SolverGraph(form == 1), block, p.stream);
+    blocks.resize(size_t(pattern->nnzb) * block * block);
+    if (form == 0) {
+      std::vector<int> ds(p.n), sl(4 * p.n);
+      for (int e = 0; e < p.n; ++e) {
+        const int ix = e % p.opt.nx, iy = e / p.opt.nx, nx = p.opt.nx, ny = p.opt.ny;
+        const int nb[4] = {iy * nx + (ix + nx - 1) % nx, iy * nx + (ix + 1) % nx, ((iy + ny - 1) % ny) * nx + ix,
+                           ((iy + 1) % ny) * nx + ix};
+        ds[e] = pattern->Find(e, e);
+        for (int s = 0; s < 4; ++s) { sl[4 * e + s] = pattern->Find(e, nb[s]); }
+      }
+      dslot.resize(ds.size());
+      slot.resize(sl.size());
+      hip_check(hipMemcpyAsync(dslot, ds.data(), sizeof(int) * ds.size(), hipMemcpyHostToDevice, p.stream), "upload");
+      hip_check(hipMemcpyAsync(slot, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice, p.stream), "upload");
+    } else {
+      const int nx = p.opt.nx, ny = p.opt.ny;
+      auto face = [&](int e, int side) {
+        const int ix = e % nx, iy = e / nx;
+        switch (side) {
+        case 0: return 2 * (iy * nx + (ix + nx - 1) % nx);
+        case 1: return 2 * e;
+        case 2: return 2 * (((iy + ny - 1) % ny) * nx + ix) + 1;
+        default: return 2 * e + 1;
+        }
+      };
+      std::vector<int> ct(8 * p.nf);
+      for (int f = 0; f < p.nf; ++f) {
+        const int e = f / 2, ix = e % nx, iy = e / nx;
+        const int er = (f & 1) ? ((iy + 1) % ny) * nx + ix : iy * nx + (ix + 1) % nx;
+        const int first = std::min(e, er), second = std::max(e, er);
+        for (int o = 0; o < 4; ++o) {
+          ct[8 * f + o] = pattern->Find(f, face(first, o));
+          ct[8 * f + 4 + o] = pattern->Find(f, face(second, o));
+        }
+      }
+      contrib.resize(ct.size());
+      hip_check(hipMemcpyAsync(contrib, ct.data(), sizeof(int) * ct.size(), hipMemcpyHostToDevice, p.stream),
+                "upload");
+    }
+  } else {
+    blocks.resize(size_t(nblk) * block * block);
+  }
+  if (mixed) { blocks32.resize(blocks.n); }
+  h.assign(size_t(restart + 1) * restart, 0.);
+  cosine.assign(restart, 0.);
+  sine.assign(restart, 0.);
+  rotated.assign(restart + 1, 0.);
+  coefficients.assign(restart, 0.);
+  p.Sync();
+}
+
+// Fill and factor the preconditioner at the current Newton state
+// (src/linear.cpp:182-203; src/partial.cpp:140-173,265-298;
+// src/operators.cpp:48-74,269-303; src/block_ilu.cpp:73-86).
+void LinearSolver::Setup() {
+  const int b = block;
+  if (form == 0) {
+    require(state != nullptr, "DG solver has no refreshed state");
+    if (ilu) {
+      eg::launch_dg_blocks(p.geo, p.frozen, state, blocks, dslot, blocks, slot, p.stream);
+    } else {
+      eg::launch_dg_blocks(p.geo, p.frozen, state, blocks, nullptr, nullptr, nullptr, p.stream);
+    }
+  } else {
+    if (ilu) {
+      eg::launch_hdg_fill(p.geo, hybrid->panels, hybrid->D, contrib, pattern->d_row, pattern->d_diag, blocks,
+                          p.stream);
+    } else {
+      eg::launch_hdg_diag_blocks(p.geo, hybrid->panels, hybrid->D, blocks, p.stream);
+    }
+  }
+  if (ilu) {
+    const auto &pt = *pattern;
+    for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
+      const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
+      eg::launch_ilu_factor_level(blocks, b, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col,
+                                  pt.d_upd_ptr, pt.d_upd, p.flags, p.stream);
+    }
+  } else {
+    eg::launch_block_inverse(blocks, b, N / b, nullptr, p.flags, p.stream);
+  }
+  if (mixed) { eg::launch_convert<double, float>(blocks, blocks32, blocks.n, p.stream); }
+  p.CheckFlags(ilu ? "Singular BSR ILU diagonal" : "Singular block Jacobi factor");
+}
+
+void LinearSolver::Apply(const double *xv, double *yv) {
+  if (form == 0) { eg::launch_dg_apply<double>(p.geo, p.frozen, state, xv, yv, p.stream); }
+  else { hybrid->SchurMult(xv, yv); }
+}
+void LinearSolver::Apply(const float *xv, float *yv) {
+  if (form == 0) { eg::launch_dg_apply<float>(p.geo, p.frozen, state, xv, yv, p.stream); }
+  else { hybrid->SchurMult(xv, yv); }
+}
+template <class T> static void precondition(LinearSolver &s, const T *bsr, const T *b, T *y) {
+  if (!s.ilu) {
+    eg::launch_block_gemv<T>(bsr, s.block, s.N / s.block, b, y, s.p.stream);
+    return;
+  }
+  const auto &pt = *s.pattern;
+  for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
+    const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
+    eg::launch_ilu_lower_level<T>(bsr, s.block, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col, b,
+                                  y, s.p.stream);
+  }
+  for (size_t l = 0; l + 1 < pt.upper_level_ptr.size(); ++l) {
+    const int first = pt.upper_level_ptr[l], count = pt.upper_level_ptr[l + 1] - first;
+    eg::launch_ilu_upper_level<T>(bsr, s.block, pt.d_upper_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col, y,
+                                  s.p.stream);
+  }
+}
+void LinearSolver::Precondition(const double *b, double *y) { precondition<double>(*this, blocks, b, y); }
+void LinearSolver::Precondition(const float *b, float *y) { precondition<float>(*this, blocks32, b, y); }
+
+double LinearSolver::Norm(const double *v) {
+  const double *metric = form == 0 ? p.d_cell_metric.p : p.d_trace_metric.p;
+  eg::launch_dot_partials<double>(v, v, metric, N, p.partial, p.stream);
+  return std::sqrt(p.ReduceScalar(p.partial, eg::kReduceBlocks, 0));
+}
+// One restarted GMRES solve in arithmetic T on the buffers of that type.
+// On return x holds the solution and res = rhs - A x (its norm returned).
+template <class T>
+double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iterations, bool zero_guess,
+                           double *true_norm) {
+  const bool f64 = sizeof(T) == 8;
+  T *Vb = f64 ? reinterpret_cast<T *>(V.p) : reinterpret_cast<T *>(V32.p);
+  T *wb = f64 ? reinterpret_cast<T *>(w.p) : reinterpret_cast<T *>(w32.p);
+  T *zb = f64 ? reinterpret_cast<T *>(z.p) : reinterpret_cast<T *>(z32.p);
+  T *cb = f64 ? reinterpret_cast<T *>(comb.p) : reinterpret_cast<T *>(comb32.p);
+  T *kb = f64 ? reinterpret_cast<T *>(corr.p) : reinterpret_cast<T *>(corr32.p);
+  T *rb = f64 ? reinterpret_cast<T *>(res.p) : reinterpret_cast<T *>(res32.p);
+  T *ab = wb; // the action buffer doubles as w outside the Arnoldi step
+  const T *metric = f64 ? reinterpret_cast<const T *>(form == 0 ? p.d_cell_metric.p : p.d_trace_metric.p)
+                        : reinterpret_cast<const T *>(form == 0 ? p.d_cell_metric32.p : p.d_trace_metric32.p);
+  hipStream_t s = p.stream;
+  double *scal = p.scalars.p;
+  auto norm = [&](const T *v) {
+    eg::launch_dot_partials<T>(v, v, metric, N, p.partial, s);
+    return std::sqrt(p.ReduceScalar(p.partial, eg::kReduceBlocks, 0));
+  };
+  auto H = [&](int i, int j) -> double & { return h[i + size_t(j) * (restart + 1)]; };
+  if (zero_guess) {
+    hip_check(hipMemsetAsync(xv, 0, sizeof(T) * N, s), "memset");
+    hip_check(hipMemcpyAsync(rb, rhs, sizeof(T) * N, hipMemcpyDeviceToDevice, s), "copy");
+  } else {
+    Apply(xv, ab);
+    hip_check(hipMemcpyAsync(rb, rhs, sizeof(T) * N, hipMemcpyDeviceToDevice, s), "copy");
+    eg::launch_axpby<T>(rb, -1., ab, 1., N, s);
+  }
+  const double rhs_norm = norm(rhs);
+  iterations = 0;
+  if (rhs_norm == 0.) {
+    if (true_norm) { *true_norm = 0.; }
+    return 0.;
+  }
+  double residual_norm = zero_guess ? rhs_norm : norm(rb);
+  while (iterations < limit) {
+    const double beta = residual_norm;
+    if (beta <= tol * rhs_norm) { break; }
+    eg::launch_axpby<T>(Vb, 1 / beta, rb, 0., N, s);
+    std::fill(h.begin(), h.end(), 0.);
+    std::fill(rotated.begin(), rotated.end(), 0.);
+    rotated[0] = beta;
+    int count = 0;
+    for (int j = 0; j < restart && iterations < limit; ++j) {
+      T *vj = Vb + size_t(j) * N;
+      Precondition(vj, zb);
+      Apply(zb, wb);
+      // CGS2: two fused projection passes, then the norm and normalisation.
+      eg::launch_multidot<T>(Vb, N, j + 1, wb, metric, N, partA, s);
+      eg::launch_cgs_update<T>(Vb, N, j + 1, partA, wb, metric, N, scal, partB, 0, s);
+      eg::launch_cgs_update<T>(Vb, N, j + 1, partB, wb, metric, N, scal + 64, partA, 1, s);
+      eg::launch_normalize<T>(partA, wb, Vb + size_t(j + 1) * N, N, scal + 128, s);
+      hip_check(hipMemcpyAsync(p.pinned, scal, sizeof(double) * 129, hipMemcpyDeviceToHost, s), "readback");
+      p.Sync();
+      for (int i = 0; i <= j; ++i) { H(i, j) = (0. + p.pinned[i]) + p.pinned[64 + i]; }
+      const double next = std::sqrt(p.pinned[128]);
+      H(j + 1, j) = next;
+      for (int i = 0; i < j; ++i) {
+        const double u = H(i, j), vr = H(i + 1, j);
+        H(i, j) = cosine[i] * u + sine[i] * vr;
+        H(i + 1, j) = -sine[i] * u + cosine[i] * vr;
+      }
+      const double d = std::hypot(H(j, j), H(j + 1, j));
+      require(std::isfinite(d) && d > 0, "GMRES Arnoldi breakdown");
+      cosine[j] = H(j, j) / d;
+      sine[j] = H(j + 1, j) / d;
+      H(j, j) = d;
+      H(j + 1, j) = 0.;
+      rotated[j + 1] = -sine[j] * rotated[j];
+      rotated[j] *= cosine[j];
+      ++iterations;
+      count = j + 1;
+      if (std::abs(rotated[j + 1]) <= tol * rhs_norm || next <= std::numeric_limits<double>::min()) { break; }
+    }
+    eg::Coef coef{};
+    for (int i = count - 1; i >= 0; --i) {
+      double value = rotated[i];
+      for (int j = i + 1; j < count; ++j) { value -= H(i, j) * coefficients[j]; }
+      coefficients[i] = value / H(i, i);
+      coef.c[i] = coefficients[i];
+    }
+    eg::launch_lincomb<T>(Vb, N, count, coef, cb, N, s);
+    Precondition(cb, kb);
+    eg::launch_axpby<T>(xv, 1., kb, 1., N, s);
+    Apply(xv, ab);
+    hip_check(hipMemcpyAsync(rb, rhs, sizeof(T) * N, hipMemcpyDeviceToDevice, s), "copy");
+    eg::launch_axpby<T>(rb, -1., ab, 1., N, s);
+    residual_norm = norm(rb);
+    require(std::isfinite(residual_norm), "Nonfinite GMRES residual");
+  }
+  if (true_norm) { *true_norm = residual_norm; }
+  return residual_norm / rhs_norm;
+}
+
+// FP64 refinement around the FP32 inner GMRES (precision "mixed").
+double LinearSolver::MixedSolve(const double *rhs, double tol, int limit, int &iterations, int &outer,
+                                bool zero_guess) {
+  hipStream_t s = p.stream;
+  if (zero_guess) {
+    hip_check(hipMemsetAsync(x, 0, sizeof(double) * N, s), "memset");
+    hip_check(hipMemcpyAsync(res, rhs, sizeof(double) * N, hipMemcpyDeviceToDevice, s), "copy");
+  } else {
+    Apply(x.p, act.p);
+    eg::launch_sub(rhs, act, res, N, s);
+  }
+  const double rhs_norm = Norm(rhs);
+  iterations = 0;
+  outer = 0;
+  if (rhs_norm == 0.) { return 0.; }
+  double residual_norm = zero_guess ? rhs_norm : Norm(res);
+  while (residual_norm > tol * rhs_norm && iterations < limit) {
+    eg::launch_convert<double, float>(res, b32, N, s);
+    int inner = 0;
+    double inner_norm = 0.;
+    const double inner_tol = std::max(p.opt.inner_tol, 0.5 * tol * rhs_norm / residual_norm);
+    GMRES<float>(b32.p, x32.p, inner_tol, limit - iterations, inner, true, &inner_norm);
+    iterations += inner;
+    ++outer;
+    eg::launch_convert<float, double>(x32, corr, N, s);
+    eg::launch_axpby<double>(x, 1., corr, 1., N, s);
+    Apply(x.p, act.p);
+    eg::launch_sub(rhs, act, res, N, s);
+    residual_norm = Norm(res);
+    require(std::isfinite(residual_norm), "Nonfinite GMRES residual");
+    if (inner == 0) { break; }
+  }
+  return residual_norm;
+}
+
+// src/linear.cpp:167-249 (gmres, matrix-free). tol is the relative GMRES
+// tolerance (o.linear_tol, tightened by the caller for HDG refinement).
+void LinearSolver::Solve(const double *dev_rhs, LinearResult &out, double tol, bool refresh) {
+  out = LinearResult{};
+  double start = clock_s();
+  if (refresh) { Setup(); }
+  out.setup = clock_s() - start;
+  start = clock_s();
+  int its = 0, outer = 0;
+  double true_norm = 0.;
+  const double rhs_norm = Norm(dev_rhs);
+  if (rhs_norm == 0.) {
+    hip_check(hipMemsetAsync(x, 0, sizeof(double) * N, p.stream), "memset");
+    hip_check(hipMemsetAsync(res, 0, sizeof(double) * N, p.stream), "memset");
+    out.solve = clock_s() - start;
+    return;
+  }
+  if (mixed) {
+    true_norm = MixedSolve(dev_rhs, tol, p.opt.max_krylov, its, outer, refresh);
+  } else {
+    GMRES<double>(dev_rhs, x.p, tol, p.opt.max_krylov, its, refresh, &true_norm);
+  }
+  out.solve = clock_s() - start;
+  out.count = its;
+  out.outer = outer;
+  out.residual_norm = true_norm;
+  out.residual = true_norm / rhs_norm;
+  require(true_norm <= tol * rhs_norm, "GMRES iteration limit reached");
+  require(std::isfinite(out.residual) && out.residual <= std::max(10 * tol, 1e-12),
+          "True inverse-mass weighted linear residual too large");
+}
+template double LinearSolver::GMRES<double>(const double *, double *, double, int, int &, bool, double *);
+template double LinearSolver::GMRES<float>(const float *, float *, double, int, int &, bool, double *);
+} // namespace euler

@@ -1,0 +1,264 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerances: kernels that evaluate the same formulas in a different summation
+order agree to ~1e-13 relative; dense inverses (Gauss-Jordan on the GPU vs the
+oracle's pivoted LU solves) to 1e-10; Newton increments and final states of
+whole runs at GMRES tolerance 1e-12 to 1e-9 relative (the north-star bar is
+1e-8 per update).
+"""
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SMALL = [f"--nx 3 --ny 2 --k {k}" for k in range(5)] + ["--nx 4 --ny 5 --k 2 --alpha-deg 30"]
+CFG1 = "--nx 8 --ny 8 --k 2 --alpha-deg 45 --steps 1 --dt 1e-4 --newton-steps 1 --precond block-jacobi"
+CFG2 = "--nx 16 --ny 16 --k 3 --cfl 1 --steps 3 --newton-steps 3 --precond block-ilu0"
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def perturbed(o):
+    u = o.initial()
+    i = np.arange(o.nu)
+    return u * (1 + 0.02 * np.sin(0.37 * (i + 1)))
+
+
+def make(gpu, oracle_lib, args, dt=1e-3):
+    import torch
+    print(f"case: {args}", flush=True)
+    o = oracle_lib.Oracle(args)
+    c = gpu.Context(args)
+    u = perturbed(o)
+    du = torch.tensor(u, device="cuda")
+    return o, c, u, du
+
+
+@pytest.mark.parametrize("args", SMALL + [CFG1, CFG2])
+def test_tables_initial_freeze(gpu, oracle_lib, args):
+    o, c, u, du = make(gpu, oracle_lib, args)
+    for a, b in zip(c.tables(), o.tables()):
+        assert np.allclose(a, b, rtol=0, atol=1e-14), (a, b)
+    assert rel(c.initial(), o.initial()) < 1e-13
+    rg, fg = c.freeze(du, 1e-3, frozen=True)
+    ro, fo = o.freeze(u, 1e-3, frozen=True)
+    assert abs(rg - ro) <= 1e-14 * ro
+    assert rel(fg, fo) < 1e-14
+
+
+@pytest.mark.parametrize("args", SMALL + [CFG1, CFG2])
+def test_residual_and_trial(gpu, oracle_lib, args):
+    import torch
+    o, c, u, du = make(gpu, oracle_lib, args)
+    c.freeze(du, 1e-3)
+    o.freeze(u, 1e-3)
+    old = 0.999 * u
+    o.set_stage(old, 2e-3)
+    c.set_stage(torch.tensor(old, device="cuda"), 2e-3)
+    r = torch.zeros_like(du)
+    ng = c.residual(du, r)
+    ro = o.residual(u)
+    assert rel(r.cpu().numpy(), ro) < 1e-12
+    assert abs(ng - o.cell_norm(ro)) <= 1e-12 * ng
+    step = torch.tensor(np.cos(0.11 * np.arange(o.nu)) * 1e-3 * np.abs(u), device="cuda")
+    trial = torch.zeros_like(du)
+    nt, adm = c.trial_residual(du, step, 0.5, trial, r)
+    ut = u + 0.5 * step.cpu().numpy()
+    assert adm and o.admissible(ut)
+    assert rel(trial.cpu().numpy(), ut) < 1e-15
+    assert rel(r.cpu().numpy(), o.residual(ut)) < 1e-12
+    # a large negative step makes the density negative: inadmissible
+    _, adm = c.trial_residual(du, du, -2.0, trial, r)
+    assert not adm
+    assert c.admissible(du) and not c.admissible(-du)
+
+
+@pytest.mark.parametrize("args", SMALL + [CFG2])
+def test_dg_apply(gpu, oracle_lib, args):
+    import torch
+    o, c, u, du = make(gpu, oracle_lib, args)
+    c.freeze(du, 1e-3)
+    o.freeze(u, 1e-3)
+    c.set_stage(du, 0.05)
+    o.set_stage(u, 0.05)
+    o.dg_refresh(u)
+    c.call("eg_dg_refresh", gpu._ptr(du))
+    x = np.sin(0.37 * (np.arange(o.nu) + 1))
+    y = torch.zeros_like(du)
+    c.call("eg_dg_apply", gpu._ptr(torch.tensor(x, device="cuda")), gpu._ptr(y))
+    assert rel(y.cpu().numpy(), o.dg_apply(x)) < 1e-12
+    y32 = torch.zeros(o.nu, dtype=torch.float32, device="cuda")
+    c.call("eg_dg_apply_f32", gpu._ptr(torch.tensor(x, dtype=torch.float32, device="cuda")), gpu._ptr(y32))
+    assert rel(y32.cpu().numpy(), o.dg_apply(x)) < 1e-5
+
+
+@pytest.mark.parametrize("precond", ["block-jacobi", "block-ilu0"])
+@pytest.mark.parametrize("args", SMALL + [CFG2.replace(" --precond block-ilu0", "")])
+def test_dg_preconditioner(gpu, oracle_lib, args, precond):
+    import torch
+    args = f"{args} --precond {precond}"
+    o, c, u, du = make(gpu, oracle_lib, args)
+    c.freeze(du, 1e-3)
+    o.freeze(u, 1e-3)
+    c.set_stage(du, 0.05)
+    o.set_stage(u, 0.05)
+    o.dg_refresh(u)
+    c.call("eg_dg_refresh", gpu._ptr(du))
+    c.call("eg_pc_setup", 0)
+    b = np.sin(0.17 * (np.arange(o.nu) + 1))
+    y = torch.zeros_like(du)
+    c.call("eg_pc_apply", 0, gpu._ptr(torch.tensor(b, device="cuda")), gpu._ptr(y))
+    ref = o.precondition(0, 1 if precond == "block-ilu0" else 0, b)
+    assert rel(y.cpu().numpy(), ref) < 1e-10
+
+
+def hdg_case(gpu, oracle_lib, args):
+    import torch
+    o, c, u, du = make(gpu, oracle_lib, args)
+    c.freeze(du, 1e-3)
+    o.freeze(u, 1e-3)
+    c.set_stage(du, 0.05)
+    o.set_stage(u, 0.05)
+    r = 1e-3 * np.sin(0.29 * (np.arange(o.nu) + 1))
+    dr = torch.tensor(r, device="cuda")
+    c.call("eg_hdg_condense", gpu._ptr(du), gpu._ptr(dr))
+    X, y, rhs, D = o.condense(u, r)
+    return o, c, u, du, X, y, rhs, D
+
+
+@pytest.mark.parametrize("args", SMALL + [CFG2])
+def test_hdg_condense_schur_recover(gpu, oracle_lib, args):
+    import torch
+    o, c, u, du, X, y, rhs, D = hdg_case(gpu, oracle_lib, args)
+    n, m, t = o.n, o.m, o.t
+    grhs, gD = torch.zeros(o.nl, dtype=torch.float64, device="cuda"), torch.zeros(o.nl, dtype=torch.float64,
+                                                                                  device="cuda")
+    c.call("eg_hdg_rhs", gpu._ptr(grhs), gpu._ptr(gD))
+    assert rel(gD.cpu().numpy(), D) < 1e-14
+    assert rel(grhs.cpu().numpy(), rhs) < 1e-10
+    xy = torch.zeros(n * m * (4 * t + 1), dtype=torch.float64, device="cuda")
+    c.call("eg_hdg_xy", gpu._ptr(xy))
+    xy = xy.cpu().numpy().reshape(n, 4 * t + 1, m)
+    assert rel(xy[:, :4 * t, :].reshape(-1), X.reshape(n, 4 * t, m).reshape(-1)) < 1e-10
+    assert rel(xy[:, 4 * t, :].reshape(-1), y) < 1e-10
+    x = np.sin(0.27 * np.arange(o.nl))
+    dx = torch.tensor(x, device="cuda")
+    out = torch.zeros(o.nl, dtype=torch.float64, device="cuda")
+    c.call("eg_schur_apply", gpu._ptr(dx), gpu._ptr(out))
+    assert rel(out.cpu().numpy(), o.schur_apply(x)) < 1e-10
+    dun = torch.zeros_like(du)
+    c.call("eg_recover", gpu._ptr(dx), gpu._ptr(dun))
+    assert rel(dun.cpu().numpy(), o.recover(x)) < 1e-10
+    er = torch.zeros_like(du)
+    import ctypes
+    nrm = ctypes.c_double()
+    c.call("eg_recovered_residual", gpu._ptr(dx), gpu._ptr(er), ctypes.byref(nrm))
+    ref = o.recovered_residual(x)
+    assert rel(er.cpu().numpy(), ref) < 1e-13
+    assert abs(nrm.value - o.cell_norm(ref)) <= 1e-12 * nrm.value
+
+
+@pytest.mark.parametrize("precond", ["block-jacobi", "block-ilu0"])
+@pytest.mark.parametrize("args", SMALL + [CFG2.replace(" --precond block-ilu0", "")])
+def test_hdg_preconditioner(gpu, oracle_lib, args, precond):
+    import torch
+    o, c, u, du, *_ = hdg_case(gpu, oracle_lib, f"{args} --precond {precond}")
+    c.call("eg_pc_setup", 1)
+    b = np.sin(0.17 * (np.arange(o.nl) + 1))
+    y = torch.zeros(o.nl, dtype=torch.float64, device="cuda")
+    c.call("eg_pc_apply", 1, gpu._ptr(torch.tensor(b, device="cuda")), gpu._ptr(y))
+    ref = o.precondition(1, 1 if precond == "block-ilu0" else 0, b)
+    assert rel(y.cpu().numpy(), ref) < 1e-10
+
+
+@pytest.mark.parametrize("precond", ["block-jacobi", "block-ilu0"])
+@pytest.mark.parametrize("args", ["--nx 3 --ny 2 --k 2", "--nx 8 --ny 8 --k 2", "--nx 16 --ny 16 --k 3"])
+def test_linear_solves(gpu, oracle_lib, args, precond):
+    import torch
+    args = f"{args} --precond {precond}"
+    o, c, u, du, *_ = hdg_case(gpu, oracle_lib, args)
+    # DG system
+    o.dg_refresh(u)
+    c.call("eg_dg_refresh", gpu._ptr(du))
+    b = -1e-3 * np.sin(0.31 * (np.arange(o.nu) + 1))
+    xo, eqo, cnt, relres = o.linear_solve(0, b)
+    xg = torch.zeros_like(du)
+    st = c.linear_solve(0, torch.tensor(b, device="cuda"), xg, tol=1e-12)
+    assert rel(xg.cpu().numpy(), xo) < 1e-9
+    assert abs(st.iterations - cnt) <= 1 and st.relative_residual <= 1e-11
+    # condensed Schur system
+    X, y, rhs, D = o.condense(u, 1e-3 * np.sin(0.29 * (np.arange(o.nu) + 1)), want_x=False)
+    xo, eqo, cnt, relres = o.linear_solve(1, rhs)
+    xg = torch.zeros(o.nl, dtype=torch.float64, device="cuda")
+    st = c.linear_solve(1, torch.tensor(rhs, device="cuda"), xg, tol=1e-12)
+    assert rel(xg.cpu().numpy(), xo) < 1e-9
+    assert abs(st.iterations - cnt) <= 1
+
+
+@pytest.mark.parametrize("form", ["dg", "hdg"])
+@pytest.mark.parametrize("args", [CFG1, CFG2.replace("--steps 3", "--steps 1"),
+                                  "--nx 6 --ny 5 --k 1 --cfl .5 --steps 1 --newton-steps 2 --precond block-ilu0",
+                                  "--nx 4 --ny 4 --k 4 --cfl 1 --steps 1 --newton-steps 2"])
+def test_run_parity(gpu, oracle_lib, args, form):
+    o = oracle_lib.Oracle(args)
+    c = gpu.Context(args)
+    print(f"case: {args} form={form}", flush=True)
+    ugo, so, inco, kro = o.run(form, increments=64)
+    ug, sg, incg, krg = c.run(form, increments=64)
+    nupd = int(so[0])
+    assert sg.updates == nupd and sg.time_steps == int(so[1])
+    for i in range(nupd):
+        # per-update increment error in the cell metric (src/stepper.cpp:194-199)
+        e = np.linalg.norm(incg[i] - inco[i]) / np.linalg.norm(inco[i])
+        assert e < 1e-9, (i, e)
+    assert rel(ug, ugo) < 1e-10
+    assert abs(sg.drift) < 1e-12
+    assert np.allclose(list(sg.errors), so[5:9], rtol=1e-8)
+
+
+def test_mixed_precision_run(gpu, oracle_lib):
+    args = "--nx 16 --ny 16 --k 3 --alpha-deg 30 --cfl 1 --steps 1 --newton-steps 3 --restart 30 --precond block-ilu0"
+    for form in ("dg", "hdg"):
+        print(f"case: mixed {form}", flush=True)
+        c64 = gpu.Context(args)
+        c32 = gpu.Context(args + " --precision mixed")
+        u64, s64, i64, _ = c64.run(form, increments=16)
+        u32, s32, i32, _ = c32.run(form, increments=16)
+        for i in range(s64.updates):
+            e = np.linalg.norm(i32[i] - i64[i]) / np.linalg.norm(i64[i])
+            assert e < 1e-8, (form, i, e)
+        assert rel(u32, u64) < 1e-9
+
+
+def test_cli_reference_smoke(gpu, tmp_path):
+    """code_test/test_solver.py:47-56 thresholds on the GPU euler2d CLI."""
+    import csv
+    out = tmp_path / "smoke"
+    subprocess.run([str(gpu.CLI), "--output", str(out), "--newton-tol", "1e-12", "--linear-tol", "1e-12",
+                    "--nx", "8", "--ny", "8", "--k", "1", "--cfl", ".1", "--steps", "1", "--form", "both",
+                    "--no-save-solution"], check=True, timeout=300)
+    for form in ("dg", "hdg"):
+        row = next(csv.DictReader(open(f"{out}_{form}_summary.csv")))
+        assert row["integrator"] == "ssp-dirk3" and int(row["time_steps"]) == 1 and int(row["newton_updates"]) > 0
+    comp = next(csv.DictReader(open(f"{out}_comparison.csv")))
+    assert float(comp["max_nodal_difference"]) < 1e-8
+
+
+def test_cli_backward_euler_update_check(gpu, tmp_path):
+    """code_test/test_solver.py:58-74: converged BE, --check-update."""
+    import csv
+    out = tmp_path / "be"
+    subprocess.run([str(gpu.CLI), "--output", str(out), "--newton-tol", "1e-12", "--linear-tol", "1e-12",
+                    "--nx", "8", "--ny", "8", "--k", "2", "--cfl", ".1", "--steps", "1", "--integrator", "be",
+                    "--form", "both", "--check-update", "--no-save-solution"], check=True, timeout=300)
+    for r in csv.DictReader(open(f"{out}_hdg_iterations.csv")):
+        v = float(r["update_check_relative"])
+        assert np.isfinite(v) and v < 1e-8
+    comp = next(csv.DictReader(open(f"{out}_comparison.csv")))
+    assert float(comp["max_nodal_difference"]) < 1e-8 and float(comp["solution_L2_difference"]) < 1e-8
