@@ -115,6 +115,16 @@ int eg_linear_solve(eg_ctx *ctx, int form, const double *dev_rhs, double *dev_x,
 int eg_run(eg_ctx *ctx, int form, const char *prefix, double *host_u_final, eg_run_stats *stats,
            double *host_increments, int max_updates, int *krylov);
 
+/* Measurement hook (no counterpart in the reference, which only times whole
+ * updates with steady_clock, include/euler.hpp:24-28): sets up the context's
+ * problem at its initial state and times `reps` back-to-back launches of one
+ * kernel with HIP events on the context's stream. kernel: "residual",
+ * "dg_apply", "dg_apply_f32", "schur_apply", "pc_dg", "pc_hdg",
+ * "hdg_condense", "pc_setup_dg", "pc_setup_hdg". avg_ms: mean duration per
+ * launch; bytes: algorithmic HBM bytes of one launch (each array counted
+ * once). */
+int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, double *bytes);
+
 #ifdef __cplusplus
 }
 #endif

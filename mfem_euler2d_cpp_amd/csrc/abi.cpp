@@ -1,5 +1,6 @@
 // extern "C" boundary (include/eg_abi.h) over the euler:: host API.
 #include <cstring>
+#include <functional>
 #include <memory>
 
 #include "../../include/eg_abi.h"
@@ -191,6 +192,80 @@ int eg_linear_solve(eg_ctx *ctx, int form, const double *dev_rhs, double *dev_x,
       stats->setup_s = r.setup;
       stats->solve_s = r.solve;
     }
+  });
+}
+int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, double *bytes) {
+  return guard(ctx, [&] {
+    auto &p = *ctx->problem;
+    const std::string k(kernel);
+    require(reps > 0, "reps must be positive");
+    DeviceArray<double> u(p.nu), r(p.nu), x(p.nu), y(p.nu), tl(p.nl), tl2(p.nl);
+    DeviceArray<float> x32(p.nu), y32(p.nu);
+    const auto init = p.Initial();
+    hip_check(hipMemcpyAsync(u, init.data(), sizeof(double) * p.nu, hipMemcpyHostToDevice, p.stream), "upload");
+    p.Freeze(u, ctx->opt.dt, false);
+    const double dt = (ctx->opt.cfl > 0 ? ctx->opt.cfl / p.WaveRate() : ctx->opt.dt) * 0.24169426078821;
+    p.SetStage(u, dt);
+    p.Residual(u, r);
+    eg::launch_scale_copy(r, -1., x, p.nu, p.stream);
+    eg::launch_convert<double, float>(x, x32, p.nu, p.stream);
+    ctx->state = u;
+    const double nu = p.nu, nl = p.nl, nfq = double(p.nf) * p.q;
+    std::function<void()> fn;
+    double b = 0.;
+    if (k == "residual") {
+      fn = [&] { eg::launch_residual(p.geo, p.frozen, u, nullptr, 0., p.old, y, nullptr, p.partial, p.flags, p.stream); };
+      b = 3 * nu * 8 + nfq * 5 * 8;
+    } else if (k == "dg_apply" || k == "dg_apply_f32") {
+      const bool f32 = k == "dg_apply_f32";
+      fn = [&, f32] {
+        if (f32) { eg::launch_dg_apply<float>(p.geo, p.frozen, u, x32, y32, p.stream); }
+        else { eg::launch_dg_apply<double>(p.geo, p.frozen, u, x, y, p.stream); }
+      };
+      b = 2 * nu * (f32 ? 4 : 8) + nu * 8 + nfq * 5 * 8;
+    } else if (k == "hdg_condense") {
+      fn = [&] { ctx->hyb().StreamCondense(u, r); };
+      b = double(p.n) * p.m * (4 * p.t + 1) * 8 + 2 * nu * 8 + 2 * nl * 8 + double(p.nf) * 8 * p.t * p.t * 8;
+    } else if (k == "schur_apply") {
+      ctx->hyb().StreamCondense(u, r);
+      hip_check(hipMemcpyAsync(tl, ctx->hyb().rhs, sizeof(double) * p.nl, hipMemcpyDeviceToDevice, p.stream), "copy");
+      fn = [&] { ctx->hyb().SchurMult(tl.p, tl2.p); };
+      b = double(p.nf) * 8 * p.t * p.t * 8 + 3 * nl * 8;
+    } else if (k == "pc_dg" || k == "pc_setup_dg") {
+      auto &s = ctx->lin(0);
+      s.Setup();
+      if (k == "pc_dg") { fn = [&] { s.Precondition(x.p, y.p); }; }
+      else { fn = [&] { s.Setup(); }; }
+      b = double(s.blocks.n) * 8 + 2 * nu * 8;
+    } else if (k == "pc_hdg" || k == "pc_setup_hdg") {
+      ctx->hyb().StreamCondense(u, r);
+      hip_check(hipMemcpyAsync(tl, ctx->hyb().rhs, sizeof(double) * p.nl, hipMemcpyDeviceToDevice, p.stream), "copy");
+      auto &s = ctx->lin(1);
+      s.Setup();
+      if (k == "pc_hdg") { fn = [&] { s.Precondition(tl.p, tl2.p); }; }
+      else { fn = [&] { s.Setup(); }; }
+      b = double(s.blocks.n) * 8 + 2 * nl * 8;
+    } else {
+      throw std::runtime_error("Unknown kernel name: " + k);
+    }
+    fn(); // warm
+    p.Sync();
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, p.stream), "event");
+    for (int i = 0; i < reps; ++i) { fn(); }
+    hip_check(hipEventRecord(e1, p.stream), "event");
+    hip_check(hipEventSynchronize(e1), "event");
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p.TakeFlags();
+    ctx->state = nullptr;
+    if (ctx->solver[0]) { ctx->solver[0]->Refresh(nullptr); }
+    *avg_ms = double(ms) / reps;
+    *bytes = b;
   });
 }
 int eg_run(eg_ctx *ctx, int form, const char *prefix, double *host_u_final, eg_run_stats *stats,

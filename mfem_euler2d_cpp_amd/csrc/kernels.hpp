@@ -43,7 +43,7 @@ void launch_hdg_fill(const Geo &g, const double *panels, const double *D, const 
                      const int *diag_ptr, double *bsr, hipStream_t s);
 
 // Dense / block linear algebra.
-void la_init();
+int la_init(); // nonzero if the large-LDS attribute could not be set
 void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s);
 template <class T>
 void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s);

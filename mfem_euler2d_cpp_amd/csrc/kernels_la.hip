@@ -464,11 +464,20 @@ int grid_for(long n) {
 }
 } // namespace
 
-void la_init() {
-  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_block_inverse), hipFuncAttributeMaxDynamicSharedMemorySize,
-                      kLdsBytes);
-  (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_ilu_factor), hipFuncAttributeMaxDynamicSharedMemorySize,
-                      kLdsBytes);
+// Allow the LDS-resident dense kernels the whole 160 KB workgroup budget
+// (dynamic part = budget - static part). Errors are cleared so that they do
+// not surface later as a sticky error on an unrelated call.
+int la_init() {
+  const void *fns[2] = {reinterpret_cast<const void *>(k_block_inverse), reinterpret_cast<const void *>(k_ilu_factor)};
+  int bad = 0;
+  for (const void *fn : fns) {
+    hipFuncAttributes attr{};
+    if (hipFuncGetAttributes(&attr, fn) != hipSuccess) { bad = 1; continue; }
+    const int dyn = kLdsBytes - int(attr.sharedSizeBytes);
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess) { bad = 1; }
+  }
+  (void)hipGetLastError();
+  return bad;
 }
 void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
   const size_t lds = size_t(b) * b * 8 + 2 * b * 8 + b * 4;

@@ -201,7 +201,9 @@ Problem::Problem(const Options &o, const EulerGas &g, hipStream_t s)
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     own_stream = true;
   }
-  eg::la_init();
+  // Dense blocks above 64 KB need the extended LDS attribute.
+  require(eg::la_init() == 0 || 2 * size_t(std::max(m, t)) * std::max(m, t) * 8 <= 65536,
+          "Cannot enable 160 KB LDS for the dense block kernels");
   frozen.resize(size_t(nf) * q);
   old.resize(nu);
   d_cell_metric.resize(nu);

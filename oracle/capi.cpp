@@ -68,11 +68,13 @@ int or_geometry(void *hv, double *g) {
   std::memcpy(g, v, sizeof v);
   return 0;
 }
-// GL nodes x (q), weights (q), D1 (q*q, column-major), l_i(0), l_i(1)
+// GL nodes x (q), weights (q), d[a*q+i] = l_i'(x_a), l_i(0), l_i(1)
 int or_tables(void *hv, double *x, double *w, double *d, double *e0, double *e1) {
   auto &p = static_cast<Handle *>(hv)->p;
   for (int i = 0; i < p.q; ++i) { x[i] = p.x1[i]; w[i] = p.w1[i]; e0[i] = p.end0[i]; e1[i] = p.end1[i]; }
-  std::memcpy(d, p.D1.data(), sizeof(double) * p.q * p.q);
+  for (int a = 0; a < p.q; ++a) {
+    for (int i = 0; i < p.q; ++i) { d[a * p.q + i] = p.D1(a, i); }
+  }
   return 0;
 }
 int or_metrics(void *hv, double *cell, double *trace) {
@@ -293,6 +295,143 @@ int or_run(void *hv, int form, const char *prefix, double *u_final, double *stat
     if (krylov) { for (int i = 0; i < nrec; ++i) { krylov[i] = r.records[i].krylov; } }
   });
 }
+// Dense operator with block-diagonal / BSR views, for the weighted-GMRES
+// known-answer case of src/verification.cpp:174-254.
+namespace {
+class DenseOperator : public Operator {
+public:
+  Dense a;
+  explicit DenseOperator(const Dense &m) : a(m) { height = m.h; }
+  void Mult(const double *x, double *y) const override {
+    for (int i = 0; i < a.h; ++i) {
+      double s = 0.;
+      for (int j = 0; j < a.w; ++j) { s += a(i, j) * x[j]; }
+      y[i] = s;
+    }
+  }
+  void DiagonalBlocks(int size, std::vector<Dense> &blocks) const override {
+    blocks.resize(height / size);
+    for (int b = 0; b < height / size; ++b) {
+      blocks[b].resize(size, size);
+      for (int j = 0; j < size; ++j) {
+        for (int i = 0; i < size; ++i) { blocks[b](i, j) = a(b * size + i, b * size + j); }
+      }
+    }
+  }
+  std::vector<std::vector<int>> BlockGraph(int size) const override {
+    std::vector<std::vector<int>> g(height / size);
+    for (int r = 0; r < height / size; ++r) {
+      for (int c = 0; c < height / size; ++c) {
+        bool nz = false;
+        for (int j = 0; j < size; ++j) {
+          for (int i = 0; i < size; ++i) { nz = nz || a(r * size + i, c * size + j) != 0.; }
+        }
+        if (nz || r == c) { g[r].push_back(c); }
+      }
+    }
+    return g;
+  }
+  void FillBlocks(FixedBlockILU &t) const override {
+    t.Zero();
+    const int b = t.block;
+    for (int r = 0; r < height / b; ++r) {
+      for (int k = t.row[r]; k < t.row[r + 1]; ++k) {
+        const int c = t.col[k];
+        for (int j = 0; j < b; ++j) {
+          for (int i = 0; i < b; ++i) { t.data[k](i, j) = a(r * b + i, c * b + j); }
+        }
+      }
+    }
+  }
+};
+} // namespace
+
+// src/verification.cpp:174-254: GMRES in the diagonal mass metric and in
+// the Cholesky-whitened dense basis must give the same physical correction
+// and the same iteration counts; a zero RHS takes 0 iterations.
+// out (per preconditioner bj, ilu): error, original count, transformed count,
+// zero-rhs count, zero-rhs |x|.
+int or_verify_weighted(double *out) {
+  return guard([&] {
+    constexpr int n = 6;
+    Dense k(n, n), t(n, n), kt(n, n), ht(n, n), l(n, n), li(n, n), normalized(n, n);
+    const double mass[n] = {.2, .7, 1.1, .3, .8, 1.4};
+    for (int i = 0; i < n; ++i) {
+      k(i, i) = 4 + .3 * i;
+      t(i, i) = .7 + .2 * i;
+      if (i + 1 < n) { k(i, i + 1) = .3; k(i + 1, i) = -.2; }
+    }
+    t(0, 1) = .25; t(1, 2) = -.15; t(3, 5) = .2;
+    for (int i = 0; i < n; ++i) {
+      for (int j = 0; j < n; ++j) {
+        for (int a = 0; a < n; ++a) {
+          ht(i, j) += t(a, i) * mass[a] * t(a, j);
+          for (int b = 0; b < n; ++b) { kt(i, j) += t(a, i) * k(a, b) * t(b, j); }
+        }
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      for (int j = 0; j <= i; ++j) {
+        double v = ht(i, j);
+        for (int a = 0; a < j; ++a) { v -= l(i, a) * l(j, a); }
+        if (i == j) { require(v > 0, "Test Gram must be SPD"); l(i, j) = std::sqrt(v); }
+        else { l(i, j) = v / l(j, j); }
+      }
+    }
+    for (int j = 0; j < n; ++j) {
+      for (int i = 0; i < n; ++i) {
+        double v = i == j ? 1. : 0.;
+        for (int a = 0; a < i; ++a) { v -= l(i, a) * li(a, j); }
+        li(i, j) = v / l(i, i);
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      for (int j = 0; j < n; ++j) {
+        for (int a = 0; a < n; ++a) {
+          for (int b = 0; b < n; ++b) { normalized(i, j) += li(i, a) * kt(a, b) * li(j, b); }
+        }
+      }
+    }
+    std::vector<double> rhs(n), trhs(n, 0.), metric(n), ones(n, 1.);
+    for (int i = 0; i < n; ++i) { rhs[i] = std::sin(i + 1.); metric[i] = 1 / mass[i]; }
+    for (int i = 0; i < n; ++i) {
+      for (int a = 0; a < n; ++a) {
+        for (int b = 0; b < n; ++b) { trhs[i] += li(i, a) * t(b, a) * rhs[b]; }
+      }
+    }
+    DenseOperator A(k), B(normalized);
+    int slot = 0;
+    for (const char *pre : {"block-jacobi", "block-ilu0"}) {
+      Options o;
+      o.precond = pre;
+      o.restart = 2;
+      o.linear_tol = 1e-10;
+      LinearSolver s1(n, 3, o, metric), s2(n, 3, o, ones);
+      LinearResult r1, r2, r3;
+      s1.Solve(A, rhs.data(), o, r1);
+      s2.Solve(B, trhs.data(), o, r2);
+      std::vector<double> phys(n, 0.);
+      for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n; ++j) {
+          for (int h = 0; h < n; ++h) { phys[i] += t(i, j) * li(h, j) * r2.x[h]; }
+        }
+      }
+      double num = 0., den = 0.;
+      for (int i = 0; i < n; ++i) { num += (phys[i] - r1.x[i]) * (phys[i] - r1.x[i]); den += r1.x[i] * r1.x[i]; }
+      std::vector<double> zero(n, 0.);
+      LinearSolver s3(n, 3, o, metric);
+      s3.Solve(A, zero.data(), o, r3);
+      double zx = 0.;
+      for (double v : r3.x) { zx += v * v; }
+      out[slot++] = std::sqrt(num / den);
+      out[slot++] = r1.count;
+      out[slot++] = r2.count;
+      out[slot++] = r3.count;
+      out[slot++] = std::sqrt(zx);
+    }
+  });
+}
+
 // Physics points for known-answer tests.
 int or_gas(double gamma, const double *u, int axis, double safety, const double *r, double *out) {
   return guard([&] {

@@ -201,10 +201,23 @@ def test_linear_solves(gpu, oracle_lib, args, precond):
     assert abs(st.iterations - cnt) <= 1
 
 
+def increment_ok(o, inc_gpu, inc_ref, u_scale, bound=1e-8):
+    """The reference's DG/HDG one-update equivalence criterion
+    (src/stepper.cpp:197-198): ||du - du_ref||_M <= max(bound ||du_ref||_M,
+    4*allowance), allowance = 128 eps (||u||_M + ||old||_M + ||r||_M). Near
+    Newton convergence the residual itself is rounding noise of size
+    ~eps ||u||, so the increments only agree to the allowance there."""
+    metric, _ = o.metrics()
+    nrm = lambda v: float(np.sqrt(np.sum(metric * v * v)))
+    allowance = 128 * np.finfo(float).eps * 2 * u_scale
+    d, r = nrm(inc_gpu - inc_ref), nrm(inc_ref)
+    return d <= max(bound * r, 4 * allowance), d / max(r, 1e-300)
+
+
 @pytest.mark.parametrize("form", ["dg", "hdg"])
 @pytest.mark.parametrize("args", [CFG1, CFG2.replace("--steps 3", "--steps 1"),
-                                  "--nx 6 --ny 5 --k 1 --cfl .5 --steps 1 --newton-steps 2 --precond block-ilu0",
-                                  "--nx 4 --ny 4 --k 4 --cfl 1 --steps 1 --newton-steps 2"])
+                                  "--nx 12 --ny 12 --k 1 --cfl .5 --steps 1 --newton-steps 2 --precond block-ilu0",
+                                  "--nx 6 --ny 6 --k 4 --cfl .5 --steps 1 --newton-steps 2"])
 def test_run_parity(gpu, oracle_lib, args, form):
     o = oracle_lib.Oracle(args)
     c = gpu.Context(args)
@@ -213,10 +226,14 @@ def test_run_parity(gpu, oracle_lib, args, form):
     ug, sg, incg, krg = c.run(form, increments=64)
     nupd = int(so[0])
     assert sg.updates == nupd and sg.time_steps == int(so[1])
+    metric, _ = o.metrics()
+    u_scale = float(np.sqrt(np.sum(metric * ugo * ugo)))
+    worst = 0.
     for i in range(nupd):
-        # per-update increment error in the cell metric (src/stepper.cpp:194-199)
-        e = np.linalg.norm(incg[i] - inco[i]) / np.linalg.norm(inco[i])
-        assert e < 1e-9, (i, e)
+        ok, e = increment_ok(o, incg[i], inco[i], u_scale, bound=1e-9)
+        worst = max(worst, e)
+        assert ok, (i, e)
+    print(f"max per-update relative increment error {worst:.2e}")
     assert rel(ug, ugo) < 1e-10
     assert abs(sg.drift) < 1e-12
     assert np.allclose(list(sg.errors), so[5:9], rtol=1e-8)
@@ -230,9 +247,12 @@ def test_mixed_precision_run(gpu, oracle_lib):
         c32 = gpu.Context(args + " --precision mixed")
         u64, s64, i64, _ = c64.run(form, increments=16)
         u32, s32, i32, _ = c32.run(form, increments=16)
+        o = oracle_lib.Oracle(args)
+        metric, _ = o.metrics()
+        u_scale = float(np.sqrt(np.sum(metric * u64 * u64)))
         for i in range(s64.updates):
-            e = np.linalg.norm(i32[i] - i64[i]) / np.linalg.norm(i64[i])
-            assert e < 1e-8, (form, i, e)
+            ok, e = increment_ok(o, i32[i], i64[i], u_scale, bound=1e-8)
+            assert ok, (form, i, e)
         assert rel(u32, u64) < 1e-9
 
 
