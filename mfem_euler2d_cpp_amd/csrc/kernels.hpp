@@ -57,6 +57,20 @@ template <class T>
 void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
                             const int *diag_ptr, const int *col, T *w, hipStream_t s);
 
+// Register-resident / split-dot variants (kernels_ilu.hip) used by the
+// launchers above when the block size allows.
+bool fast_dense_ok(int b);
+int ilu_factor_lds_init();
+void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s);
+void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                            const int *col, const int *upd_ptr, const int *upd, int *flags, hipStream_t s);
+template <class T>
+void launch_ilu_lower_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                           const int *col, const T *rhs, T *w, hipStream_t s);
+template <class T>
+void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                           const int *col, T *w, hipStream_t s);
+
 // Vector kernels (Krylov). nb = kReduceBlocks partials per reduction.
 struct Coef {
   double c[64];

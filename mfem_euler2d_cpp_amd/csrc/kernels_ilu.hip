@@ -1,0 +1,352 @@
+// Dense block kernels tuned for the ILU(0)/block-Jacobi hot spots:
+//  * Gauss-Jordan inverse with the matrix resident in registers and implicit
+//    partial pivoting (no row swaps; two workgroup barriers per column),
+//  * one-row-per-workgroup ILU(0) factor with LDS-staged, 4x4 register-tiled
+//    block products,
+//  * level sweeps whose block GEMVs split every row's dot product over
+//    several threads (independent loads in flight, LDS reduction).
+#include "kernels.hpp"
+
+namespace eg {
+namespace {
+constexpr int kThreads = 256;
+
+// In-place inverse of the b x b column-major block A (global memory) by one
+// workgroup (b <= blockDim.x). Thread (tc, tr) = (tid % b, tid / b) owns
+// column tc at rows tr, tr + R, ... (R = blockDim.x / b) in registers.
+// Column k is eliminated with pivot row p = first unused row of maximal
+// |S(p,k)|; the stored matrix S then satisfies A^{-1}(pinv[i], piv[j]) = S(i,j).
+// lds: 4b doubles + 3b ints. Returns true if a zero pivot occurred.
+template <int NE>
+__device__ bool gj_inverse_regs(double *__restrict__ A, int b, double *lds) {
+  double *colk = lds, *rowp = lds + 2 * b;
+  int *piv = reinterpret_cast<int *>(lds + 4 * b), *pinv = piv + b, *used = pinv + b;
+  const int tid = threadIdx.x, R = blockDim.x / b;
+  const int tc = tid % b, tr = tid / b, lane = tid & 63;
+  const bool active = tr < R;
+  double v[NE];
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int i = tr + e * R;
+    v[e] = (active && i < b) ? A[i + size_t(tc) * b] : 0.;
+  }
+  for (int i = tid; i < b; i += blockDim.x) { used[i] = 0; }
+  __syncthreads();
+  bool singular = false;
+  for (int k = 0; k < b; ++k) {
+    double *ck = colk + (k & 1) * b, *rp = rowp + (k & 1) * b;
+    if (active && tc == k) {
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        const int i = tr + e * R;
+        if (i < b) { ck[i] = v[e]; }
+      }
+    }
+    __syncthreads();
+    double best = -1.;
+    int arg = b;
+    for (int i = lane; i < b; i += 64) {
+      if (!used[i]) {
+        const double a = fabs(ck[i]);
+        if (a > best) { best = a; arg = i; }
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ob = __shfl_xor(best, off, 64);
+      const int oa = __shfl_xor(arg, off, 64);
+      if (ob > best || (ob == best && oa < arg)) { best = ob; arg = oa; }
+    }
+    const int p = arg;
+    const double ap = ck[p];
+    if (ap == 0.) { singular = true; }
+    const double inv = 1. / ap;
+    if (active && tr == p % R) {
+      const int ep = p / R;
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        if (e == ep) { rp[tc] = v[e]; }
+      }
+    }
+    __syncthreads();
+    if (tid == 0) { used[p] = 1; piv[k] = p; pinv[p] = k; }
+    if (active) {
+      const double rpj = rp[tc] * inv;
+#pragma unroll
+      for (int e = 0; e < NE; ++e) {
+        const int i = tr + e * R;
+        if (i < b) {
+          if (i == p) { v[e] = tc == k ? inv : v[e] * inv; }
+          else if (tc == k) { v[e] = -ck[i] * inv; }
+          else { v[e] = v[e] - ck[i] * rpj; }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int i = tr + e * R;
+      if (i < b) { A[pinv[i] + size_t(piv[tc]) * b] = v[e]; }
+    }
+  }
+  __syncthreads();
+  return singular;
+}
+
+template <int NE>
+__global__ __launch_bounds__(kThreads) void k_block_inverse_regs(double *__restrict__ blocks, int b,
+                                                                 const int *__restrict__ index, int *__restrict__ flags) {
+  __shared__ double lds[4 * 128 + 3 * 64];
+  const int blk = index ? index[blockIdx.x] : blockIdx.x;
+  if (gj_inverse_regs<NE>(blocks + size_t(blk) * b * b, b, lds) && threadIdx.x == 0) {
+    atomicOr(flags, kErrSingular);
+  }
+}
+
+// C (-)= X Y for b x b blocks: X and Yt (Y transposed) in LDS, C in global.
+// Each thread computes 4x4 tiles; mode 0 stores C = X Y (and into Xout in
+// LDS when given), mode 1 subtracts from C.
+__device__ void tile_product(const double *__restrict__ X, const double *__restrict__ Yt, double *__restrict__ C,
+                             int b, int mode, double *__restrict__ Xout) {
+  const int nt4 = b / 4, tiles = nt4 * nt4;
+  for (int tile = threadIdx.x; tile < tiles; tile += blockDim.x) {
+    const int r0 = (tile % nt4) * 4, c0 = (tile / nt4) * 4;
+    double acc[4][4] = {};
+    for (int kk = 0; kk < b; ++kk) {
+      double a[4], y[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) { a[i] = X[r0 + i + kk * b]; y[i] = Yt[c0 + i + kk * b]; }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { acc[j][i] += a[i] * y[j]; }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const size_t idx = (r0 + i) + size_t(c0 + j) * b;
+        if (mode == 0) {
+          C[idx] = acc[j][i];
+          if (Xout) { Xout[idx] = acc[j][i]; }
+        } else {
+          C[idx] -= acc[j][i];
+        }
+      }
+    }
+  }
+}
+// Stage a global column-major block into LDS, optionally transposed.
+__device__ void stage(const double *__restrict__ src, double *__restrict__ dst, int b, bool transpose) {
+  for (int t = threadIdx.x; t < b * b; t += blockDim.x) {
+    const int r = t % b, c = t / b;
+    if (transpose) { dst[c + r * b] = src[t]; }
+    else { dst[t] = src[t]; }
+  }
+}
+
+// One BSR row of ILU(0) per workgroup (src/block_ilu.cpp:73-86), rows of one
+// dependency level in parallel. L_ik = A_ik inv(A_kk) and A_ij -= L_ik A_kj
+// as LDS-staged tile products; the updated diagonal block is replaced by its
+// inverse. LDS: X (L_ik) and Y (second operand, transposed): 2 b^2 doubles.
+template <int NE>
+__global__ __launch_bounds__(kThreads) void k_ilu_factor_rows(double *__restrict__ bsr, int b,
+                                                              const int *__restrict__ rows,
+                                                              const int *__restrict__ row_ptr,
+                                                              const int *__restrict__ diag_ptr,
+                                                              const int *__restrict__ col,
+                                                              const int *__restrict__ upd_ptr,
+                                                              const int *__restrict__ upd, int *__restrict__ flags) {
+  extern __shared__ double sm[];
+  const size_t bb = size_t(b) * b;
+  double *X = sm, *Y = sm + bb;
+  const int i = rows[blockIdx.x];
+  for (int ik = row_ptr[i]; ik < diag_ptr[i]; ++ik) {
+    const int k = col[ik];
+    double *aik = bsr + ik * bb;
+    stage(aik, Y, b, false);                         // Y <- A_ik (untransposed: used as X operand)
+    stage(bsr + diag_ptr[k] * bb, X, b, true);       // X <- inv(A_kk)^T
+    __syncthreads();
+    // L = A_ik * inv(A_kk): operands (Y, X^T) -> write to A_ik; keep L in Y.
+    {
+      const int nt4 = b / 4, tiles = nt4 * nt4;
+      double *L = aik;
+      for (int tile = threadIdx.x; tile < tiles; tile += blockDim.x) {
+        const int r0 = (tile % nt4) * 4, c0 = (tile / nt4) * 4;
+        double acc[4][4] = {};
+        for (int kk = 0; kk < b; ++kk) {
+          double a[4], y[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) { a[t] = Y[r0 + t + kk * b]; y[t] = X[c0 + t + kk * b]; }
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) { acc[j][t] += a[t] * y[j]; }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) { L[(r0 + t) + size_t(c0 + j) * b] = acc[j][t]; }
+        }
+      }
+    }
+    __syncthreads();
+    stage(aik, X, b, false);  // X <- L_ik
+    for (int u = upd_ptr[ik]; u < upd_ptr[ik + 1]; ++u) {
+      __syncthreads();
+      stage(bsr + size_t(upd[2 * u + 1]) * bb, Y, b, true);  // Y <- A_kj^T
+      __syncthreads();
+      tile_product(X, Y, bsr + size_t(upd[2 * u]) * bb, b, 1, nullptr);
+    }
+    __syncthreads();
+  }
+  __threadfence_block();
+  if (gj_inverse_regs<NE>(bsr + diag_ptr[i] * bb, b, sm) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
+}
+
+// Level sweeps: every row's dot products are split over P = blockDim/b
+// threads (columns c = part, part + P, ...) and reduced through LDS in a
+// fixed order.
+template <class T>
+__device__ __forceinline__ T row_dot(const T *__restrict__ a, const T *__restrict__ x, int b, int r, int part,
+                                     int P) {
+  T acc0 = T(0), acc1 = T(0);
+  int c = part;
+  for (; c + P < b; c += 2 * P) {
+    acc0 += a[size_t(c) * b + r] * x[c];
+    acc1 += a[size_t(c + P) * b + r] * x[c + P];
+  }
+  if (c < b) { acc0 += a[size_t(c) * b + r] * x[c]; }
+  return acc0 + acc1;
+}
+// Forward sweep of one level (src/block_ilu.cpp:87-99): w_i = b_i - sum_{j<i} L_ij w_j.
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_ilu_lower_split(const T *__restrict__ bsr, int b,
+                                                              const int *__restrict__ rows,
+                                                              const int *__restrict__ row_ptr,
+                                                              const int *__restrict__ diag_ptr,
+                                                              const int *__restrict__ col, const T *__restrict__ rhs,
+                                                              T *__restrict__ w) {
+  __shared__ T red[kThreads];
+  const int i = rows[blockIdx.x];
+  const size_t bb = size_t(b) * b;
+  const int P = blockDim.x / b, r = threadIdx.x % b, part = threadIdx.x / b;
+  T acc = T(0);
+  if (part < P) {
+    for (int ij = row_ptr[i]; ij < diag_ptr[i]; ++ij) {
+      acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P);
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (part == 0) {
+    T s = T(0);
+    for (int q = 0; q < P; ++q) { s += red[q * b + r]; }
+    w[size_t(i) * b + r] = rhs[size_t(i) * b + r] - s;
+  }
+}
+// Backward sweep of one level (src/block_ilu.cpp:100-108):
+// w_i = inv(U_ii) (w_i - sum_{j>i} U_ij w_j).
+template <class T>
+__global__ __launch_bounds__(kThreads) void k_ilu_upper_split(const T *__restrict__ bsr, int b,
+                                                              const int *__restrict__ rows,
+                                                              const int *__restrict__ row_ptr,
+                                                              const int *__restrict__ diag_ptr,
+                                                              const int *__restrict__ col, T *__restrict__ w) {
+  __shared__ T red[kThreads];
+  __shared__ T ts[kThreads];
+  const int i = rows[blockIdx.x];
+  const size_t bb = size_t(b) * b;
+  const int P = blockDim.x / b, r = threadIdx.x % b, part = threadIdx.x / b;
+  T acc = T(0);
+  if (part < P) {
+    for (int ij = diag_ptr[i] + 1; ij < row_ptr[i + 1]; ++ij) {
+      acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P);
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (part == 0) {
+    T s = T(0);
+    for (int q = 0; q < P; ++q) { s += red[q * b + r]; }
+    ts[r] = w[size_t(i) * b + r] - s;
+  }
+  __syncthreads();
+  acc = part < P ? row_dot(bsr + diag_ptr[i] * bb, ts, b, r, part, P) : T(0);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (part == 0) {
+    T s = T(0);
+    for (int q = 0; q < P; ++q) { s += red[q * b + r]; }
+    w[size_t(i) * b + r] = s;
+  }
+}
+int ne_for(int b) {
+  const int R = kThreads / b, need = (b + R - 1) / R;
+  int ne = 1;
+  while (ne < need) { ne *= 2; }
+  return ne;
+}
+} // namespace
+
+#define EG_DISPATCH_NE(ne, CALL)                                                                   \
+  switch (ne) {                                                                                    \
+  case 1: { constexpr int NE = 1; CALL; } break;                                                   \
+  case 2: { constexpr int NE = 2; CALL; } break;                                                   \
+  case 4: { constexpr int NE = 4; CALL; } break;                                                   \
+  case 8: { constexpr int NE = 8; CALL; } break;                                                   \
+  case 16: { constexpr int NE = 16; CALL; } break;                                                 \
+  case 32: { constexpr int NE = 32; CALL; } break;                                                 \
+  default: { constexpr int NE = 64; CALL; } break;                                                 \
+  }
+
+bool fast_dense_ok(int b) { return b <= 128 && b % 4 == 0; }
+void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
+  EG_DISPATCH_NE(ne_for(b), (k_block_inverse_regs<NE><<<count, kThreads, 0, s>>>(blocks, b, index, flags)));
+}
+int ilu_factor_lds_init() {
+  int bad = 0;
+#define EG_SET(NE)                                                                                  \
+  {                                                                                                \
+    hipFuncAttributes attr{};                                                                      \
+    const void *fn = reinterpret_cast<const void *>(k_ilu_factor_rows<NE>);                        \
+    if (hipFuncGetAttributes(&attr, fn) != hipSuccess ||                                           \
+        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,                        \
+                            163840 - int(attr.sharedSizeBytes)) != hipSuccess) {                   \
+      bad = 1;                                                                                     \
+    }                                                                                              \
+  }
+  EG_SET(1) EG_SET(2) EG_SET(4) EG_SET(8) EG_SET(16) EG_SET(32) EG_SET(64)
+#undef EG_SET
+  (void)hipGetLastError();
+  return bad;
+}
+void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                            const int *col, const int *upd_ptr, const int *upd, int *flags, hipStream_t s) {
+  const size_t lds = std::max<size_t>(2 * size_t(b) * b * 8, (4 * b + 2 * b) * 8);
+  EG_DISPATCH_NE(ne_for(b), (k_ilu_factor_rows<NE><<<nrows, kThreads, lds, s>>>(bsr, b, rows, row_ptr, diag_ptr, col,
+                                                                                 upd_ptr, upd, flags)));
+}
+template <class T>
+void launch_ilu_lower_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                           const int *col, const T *rhs, T *w, hipStream_t s) {
+  k_ilu_lower_split<T><<<nrows, kThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w);
+}
+template <class T>
+void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                           const int *col, T *w, hipStream_t s) {
+  k_ilu_upper_split<T><<<nrows, kThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w);
+}
+template void launch_ilu_lower_fast<double>(const double *, int, const int *, int, const int *, const int *,
+                                            const int *, const double *, double *, hipStream_t);
+template void launch_ilu_lower_fast<float>(const float *, int, const int *, int, const int *, const int *,
+                                           const int *, const float *, float *, hipStream_t);
+template void launch_ilu_upper_fast<double>(const double *, int, const int *, int, const int *, const int *,
+                                            const int *, double *, hipStream_t);
+template void launch_ilu_upper_fast<float>(const float *, int, const int *, int, const int *, const int *,
+                                           const int *, float *, hipStream_t);
+} // namespace eg

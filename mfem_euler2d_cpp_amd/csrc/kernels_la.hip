@@ -477,9 +477,10 @@ int la_init() {
     if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess) { bad = 1; }
   }
   (void)hipGetLastError();
-  return bad;
+  return bad | ilu_factor_lds_init();
 }
 void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
+  if (fast_dense_ok(b)) { launch_block_inverse_fast(blocks, b, count, index, flags, s); return; }
   const size_t lds = size_t(b) * b * 8 + 2 * b * 8 + b * 4;
   k_block_inverse<<<count, 256, lds, s>>>(blocks, b, index, flags);
 }
@@ -493,17 +494,23 @@ template void launch_block_gemv<float>(const float *, int, int, const float *, f
 void launch_ilu_factor_level(double *bsr, int b, const int *rows, int nrows, const int *row_ptr,
                              const int *diag_ptr, const int *col, const int *upd_ptr, const int *upd, int *flags,
                              hipStream_t s) {
+  if (fast_dense_ok(b) && b <= 100) {
+    launch_ilu_factor_fast(bsr, b, rows, nrows, row_ptr, diag_ptr, col, upd_ptr, upd, flags, s);
+    return;
+  }
   const size_t lds = 2 * size_t(b) * b * 8;
   k_ilu_factor<<<nrows, 256, lds, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, upd_ptr, upd, flags);
 }
 template <class T>
 void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
                             const int *diag_ptr, const int *col, const T *rhs, T *w, hipStream_t s) {
+  if (b <= 256) { launch_ilu_lower_fast<T>(bsr, b, rows, nrows, row_ptr, diag_ptr, col, rhs, w, s); return; }
   k_ilu_lower<T><<<nrows, b <= 64 ? 64 : (b <= 128 ? 128 : 256), 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w);
 }
 template <class T>
 void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
                             const int *diag_ptr, const int *col, T *w, hipStream_t s) {
+  if (b <= 256) { launch_ilu_upper_fast<T>(bsr, b, rows, nrows, row_ptr, diag_ptr, col, w, s); return; }
   k_ilu_upper<T><<<nrows, b <= 64 ? 64 : (b <= 128 ? 128 : 256), b * sizeof(T), s>>>(bsr, b, rows, row_ptr,
                                                                                      diag_ptr, col, w);
 }
