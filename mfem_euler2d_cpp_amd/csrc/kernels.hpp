@@ -11,6 +11,7 @@ enum : int {
   kErrDegenerate = 8,    // Freeze: a + b <= 0
   kErrSingular = 16,     // zero pivot in a dense block inverse
   kErrNonfinite = 32,
+  kErrSpin = 64,         // a dataflow dependency wait exceeded its spin bound
 };
 // Largest dense block held in LDS by the Gauss-Jordan inverse (b*b doubles
 // plus 3b scratch within the 160 KB gfx950 workgroup limit).
@@ -70,6 +71,15 @@ void launch_ilu_lower_fast(const T *bsr, int b, const int *rows, int nrows, cons
 template <class T>
 void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                            const int *col, T *w, hipStream_t s);
+
+// Single-launch dataflow forms (cooperative grid, per-row completion flags);
+// return false when a co-resident launch is not possible (caller falls back
+// to the per-level launches).
+template <class T>
+bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr,
+                         const int *col, const T *rhs, T *w, int *done, int epoch, int *flags, hipStream_t s);
+bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
+                          const int *upd_ptr, const int *upd, int *done, int epoch, int *flags, hipStream_t s);
 
 // Vector kernels (Krylov). nb = kReduceBlocks partials per reduction.
 struct Coef {

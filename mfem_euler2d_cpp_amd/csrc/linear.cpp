@@ -10,6 +10,7 @@
 // preconditioner run in FP32.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 
 #include "euler.hpp"
@@ -37,6 +38,12 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
   if (ilu) {
     pattern = std::make_unique<BlockPattern>(p.SolverGraph(form == 1), block, p.stream);
     blocks.resize(size_t(pattern->nnzb) * block * block);
+    const char *lv = std::getenv("EG_ILU_LEVELS");
+    dataflow = !(lv && lv[0] == '1');
+    done_sweep.resize(pattern->nb);
+    done_factor.resize(pattern->nb);
+    hip_check(hipMemsetAsync(done_sweep, 0, sizeof(int) * pattern->nb, p.stream), "memset");
+    hip_check(hipMemsetAsync(done_factor, 0, sizeof(int) * pattern->nb, p.stream), "memset");
     if (form == 0) {
       std::vector<int> ds(p.n), sl(4 * p.n);
       for (int e = 0; e < p.n; ++e) {
@@ -109,10 +116,14 @@ void LinearSolver::Setup() {
   }
   if (ilu) {
     const auto &pt = *pattern;
-    for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
-      const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
-      eg::launch_ilu_factor_level(blocks, b, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col,
-                                  pt.d_upd_ptr, pt.d_upd, p.flags, p.stream);
+    const bool df = dataflow && eg::launch_ilu_factor_df(blocks, b, pt.nb, pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr,
+                                                         pt.d_upd, done_factor, ++epoch_factor, p.flags, p.stream);
+    if (!df) {
+      for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
+        const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
+        eg::launch_ilu_factor_level(blocks, b, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col,
+                                    pt.d_upd_ptr, pt.d_upd, p.flags, p.stream);
+      }
     }
   } else {
     eg::launch_block_inverse(blocks, b, N / b, nullptr, p.flags, p.stream);
@@ -135,6 +146,19 @@ template <class T> static void precondition(LinearSolver &s, const T *bsr, const
     return;
   }
   const auto &pt = *s.pattern;
+  if (s.dataflow) {
+    const int e0 = ++s.epoch_sweep;
+    if (eg::launch_ilu_sweep_df<T>(true, bsr, s.block, pt.nb, pt.d_row, pt.d_diag, pt.d_col, b, y, s.done_sweep, e0,
+                                   s.p.flags, s.p.stream)) {
+      const int e1 = ++s.epoch_sweep;
+      if (!eg::launch_ilu_sweep_df<T>(false, bsr, s.block, pt.nb, pt.d_row, pt.d_diag, pt.d_col, b, y, s.done_sweep,
+                                      e1, s.p.flags, s.p.stream)) {
+        throw std::runtime_error("Co-resident launch of the upper ILU sweep failed");
+      }
+      return;
+    }
+    s.dataflow = false; // not co-resident on this device: per-level launches from now on
+  }
   for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
     const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
     eg::launch_ilu_lower_level<T>(bsr, s.block, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col, b,
@@ -244,6 +268,7 @@ double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iter
     hip_check(hipMemcpyAsync(rb, rhs, sizeof(T) * N, hipMemcpyDeviceToDevice, s), "copy");
     eg::launch_axpby<T>(rb, -1., ab, 1., N, s);
     residual_norm = norm(rb);
+    require(!(*p.pinned_flags & eg::kErrSpin), "ILU dataflow dependency wait exceeded its bound");
     require(std::isfinite(residual_norm), "Nonfinite GMRES residual");
   }
   if (true_norm) { *true_norm = residual_norm; }

@@ -256,6 +256,7 @@ void Problem::CheckFlags(const char *context) {
   hip_check(hipMemcpyAsync(pinned_flags, flags, sizeof(int), hipMemcpyDeviceToHost, stream), "readback");
   Sync();
   const int f = TakeFlags();
+  if (f & eg::kErrSpin) { throw std::runtime_error("ILU dataflow dependency wait exceeded its bound"); }
   if (f & eg::kErrSingular) { throw std::runtime_error(context); }
   if (f & eg::kErrFaceState) { throw std::runtime_error("Nonpositive old face state; refine mesh"); }
   if (f & eg::kErrRoe) { throw std::runtime_error("Invalid Roe sound speed"); }

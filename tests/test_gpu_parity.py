@@ -289,3 +289,27 @@ def test_cli_backward_euler_update_check(gpu, tmp_path):
         assert np.isfinite(v) and v < 1e-8
     comp = next(csv.DictReader(open(f"{out}_comparison.csv")))
     assert float(comp["max_nodal_difference"]) < 1e-8 and float(comp["solution_L2_difference"]) < 1e-8
+
+
+@pytest.mark.parametrize("args", ["--nx 16 --ny 16 --k 3", "--nx 5 --ny 3 --k 2", "--nx 4 --ny 6 --k 0"])
+def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
+    """The single-launch dataflow ILU(0) factor and sweeps (per-row completion
+    flags) give the preconditioner of the one-launch-per-level schedule."""
+    import torch
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("EG_ILU_LEVELS", mode)
+        o, c, u, du, *_ = hdg_case(gpu, oracle_lib, args + " --precond block-ilu0")
+        res = []
+        for form, size in ((1, o.nl), (0, o.nu)):
+            if form == 0:
+                c.call("eg_dg_refresh", gpu._ptr(du))
+            c.call("eg_pc_setup", form)
+            b = np.sin(0.13 * (np.arange(size) + 1))
+            y = torch.zeros(size, dtype=torch.float64, device="cuda")
+            c.call("eg_pc_apply", form, gpu._ptr(torch.tensor(b, device="cuda")), gpu._ptr(y))
+            res.append(y.cpu().numpy())
+            assert rel(res[-1], o.precondition(form, 1, b)) < 1e-10
+        out[mode] = res
+    for a, b in zip(out["0"], out["1"]):
+        assert rel(a, b) < 1e-13
