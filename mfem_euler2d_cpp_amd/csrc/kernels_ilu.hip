@@ -11,6 +11,20 @@ namespace eg {
 namespace {
 constexpr int kThreads = 256;
 
+// Relaxed agent-scope atomic accesses: coherent across XCDs (sc1) without
+// cache write-back/invalidate; used for data that another workgroup of the
+// same launch writes or reads.
+template <class T> __device__ __forceinline__ T ld_coherent(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ void st_coherent(T *p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <bool COH> __device__ __forceinline__ double ld(const double *p) { return COH ? ld_coherent(p) : *p; }
+template <bool COH> __device__ __forceinline__ void st(double *p, double v) {
+  if (COH) { st_coherent(p, v); } else { *p = v; }
+}
+
 // In-place inverse of the b x b column-major block A (global memory) by one
 // workgroup (b <= blockDim.x). Thread (tc, tr) = (tid % b, tid / b) owns
 // column tc at rows tr, tr + R, ... (R = blockDim.x / b) in registers.
@@ -21,7 +35,7 @@ constexpr int kThreads = 256;
 // pivot costs R LDS reads instead of a column scan and wave reduction.
 // lds: 4b + 2*64 doubles, then 2*64 + 2b ints. Returns true on a zero pivot.
 constexpr int kGjLds = 4 * 128 + 2 * 64 + (2 * 64 + 2 * 128 + 1) / 2 + 1;
-template <int NE>
+template <int NE, bool COH = false>
 __device__ bool gj_inverse_regs(double *__restrict__ A, int b, double *lds) {
   const int R = blockDim.x / b;
   double *colk = lds, *rowp = lds + 2 * b, *cbest = lds + 4 * b;
@@ -34,7 +48,7 @@ __device__ bool gj_inverse_regs(double *__restrict__ A, int b, double *lds) {
 #pragma unroll
   for (int e = 0; e < NE; ++e) {
     const int i = tr + e * R;
-    v[e] = (active && i < b) ? A[i + size_t(tc) * b] : 0.;
+    v[e] = (active && i < b) ? ld<COH>(A + i + size_t(tc) * b) : 0.;
   }
   __syncthreads();
   bool singular = false;
@@ -95,7 +109,7 @@ __device__ bool gj_inverse_regs(double *__restrict__ A, int b, double *lds) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       const int i = tr + e * R;
-      if (i < b) { A[pinv[i] + size_t(piv[tc]) * b] = v[e]; }
+      if (i < b) { st<COH>(A + pinv[i] + size_t(piv[tc]) * b, v[e]); }
     }
   }
   __syncthreads();
@@ -115,6 +129,7 @@ __global__ __launch_bounds__(kThreads) void k_block_inverse_regs(double *__restr
 // C (-)= X Y for b x b blocks: X and Yt (Y transposed) in LDS, C in global.
 // Each thread computes 4x4 tiles; mode 0 stores C = X Y (and into Xout in
 // LDS when given), mode 1 subtracts from C.
+template <bool COH = false>
 __device__ void tile_product(const double *__restrict__ X, const double *__restrict__ Yt, double *__restrict__ C,
                              int b, int mode, double *__restrict__ Xout) {
   const int nt4 = b / 4, tiles = nt4 * nt4;
@@ -137,21 +152,23 @@ __device__ void tile_product(const double *__restrict__ X, const double *__restr
       for (int i = 0; i < 4; ++i) {
         const size_t idx = (r0 + i) + size_t(c0 + j) * b;
         if (mode == 0) {
-          C[idx] = acc[j][i];
+          st<COH>(C + idx, acc[j][i]);
           if (Xout) { Xout[idx] = acc[j][i]; }
         } else {
-          C[idx] -= acc[j][i];
+          st<COH>(C + idx, ld<COH>(C + idx) - acc[j][i]);
         }
       }
     }
   }
 }
 // Stage a global column-major block into LDS, optionally transposed.
+template <bool COH = false>
 __device__ void stage(const double *__restrict__ src, double *__restrict__ dst, int b, bool transpose) {
   for (int t = threadIdx.x; t < b * b; t += blockDim.x) {
     const int r = t % b, c = t / b;
-    if (transpose) { dst[c + r * b] = src[t]; }
-    else { dst[t] = src[t]; }
+    const double v = ld<COH>(src + t);
+    if (transpose) { dst[c + r * b] = v; }
+    else { dst[t] = v; }
   }
 }
 
@@ -302,26 +319,49 @@ __global__ __launch_bounds__(kThreads) void k_ilu_upper_split(const T *__restric
 // precede its own row in that order and all workgroups are resident, every
 // wait ends; a wait is nevertheless bounded by spin_limit and then raises
 // kErrSpin instead of hanging.
+//
+// Cross-workgroup data (the sweep vector and the flags) is accessed only with
+// relaxed agent-scope atomics, which the gfx950 memory model makes coherent
+// across XCDs without L2 write-back/invalidate; the writer completes its
+// stores (s_waitcnt) before the workgroup barrier that precedes the flag.
+__device__ __forceinline__ void complete_memory_ops() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 __device__ __forceinline__ void wait_rows(const int *__restrict__ col, int j0, int j1, const int *done, int epoch,
                                           int spin_limit, int *flags) {
   for (int ij = j0 + int(threadIdx.x); ij < j1; ij += blockDim.x) {
     const int j = col[ij];
     int spins = 0;
-    while (__hip_atomic_load(done + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+    while (ld_coherent(done + j) < epoch) {
       if (++spins > spin_limit) {
         atomicOr(flags, kErrSpin);
         break;
       }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
   }
-  __threadfence();
+  complete_memory_ops();
   __syncthreads();
 }
 __device__ __forceinline__ void publish_row(int *done, int i, int epoch) {
-  __threadfence();
+  complete_memory_ops();
   __syncthreads();
-  if (threadIdx.x == 0) { __hip_atomic_store(done + i, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+  if (threadIdx.x == 0) { st_coherent(done + i, epoch); }
+}
+// row_dot with the vector entries read coherently (written by other
+// workgroups of the same launch).
+template <class T>
+__device__ __forceinline__ T row_dot_coherent(const T *__restrict__ a, const T *x, int b, int r, int part, int P) {
+  T acc0 = T(0), acc1 = T(0);
+  int c = part;
+  for (; c + P < b; c += 2 * P) {
+    acc0 += a[size_t(c) * b + r] * ld_coherent(x + c);
+    acc1 += a[size_t(c + P) * b + r] * ld_coherent(x + c + P);
+  }
+  if (c < b) { acc0 += a[size_t(c) * b + r] * ld_coherent(x + c); }
+  return acc0 + acc1;
 }
 
 template <class T, bool LOWER>
@@ -340,14 +380,16 @@ __global__ __launch_bounds__(kThreads) void k_ilu_sweep_df(const T *__restrict__
     wait_rows(col, j0, j1, done, epoch, spin_limit, flags);
     T acc = T(0);
     if (part < P) {
-      for (int ij = j0; ij < j1; ++ij) { acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P); }
+      for (int ij = j0; ij < j1; ++ij) {
+        acc += row_dot_coherent(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P);
+      }
     }
     red[threadIdx.x] = acc;
     __syncthreads();
     if (part == 0) {
       T s = T(0);
       for (int q = 0; q < P; ++q) { s += red[q * b + r]; }
-      if (LOWER) { w[size_t(i) * b + r] = rhs[size_t(i) * b + r] - s; }
+      if (LOWER) { st_coherent(w + size_t(i) * b + r, rhs[size_t(i) * b + r] - s); }
       else { ts[r] = w[size_t(i) * b + r] - s; }
     }
     if (!LOWER) {
@@ -358,7 +400,7 @@ __global__ __launch_bounds__(kThreads) void k_ilu_sweep_df(const T *__restrict__
       if (part == 0) {
         T s = T(0);
         for (int q = 0; q < P; ++q) { s += red[q * b + r]; }
-        w[size_t(i) * b + r] = s;
+        st_coherent(w + size_t(i) * b + r, s);
       }
     }
     publish_row(done, i, epoch);
@@ -381,8 +423,8 @@ __global__ __launch_bounds__(kThreads) void k_ilu_factor_df(double *bsr, int b, 
     for (int ik = row_ptr[i]; ik < diag_ptr[i]; ++ik) {
       const int k = col[ik];
       double *aik = bsr + ik * bb;
-      stage(aik, Y, b, false);
-      stage(bsr + diag_ptr[k] * bb, X, b, true);
+      stage<true>(aik, Y, b, false);
+      stage<true>(bsr + diag_ptr[k] * bb, X, b, true);
       __syncthreads();
       {
         const int nt4 = b / 4, tiles = nt4 * nt4;
@@ -402,21 +444,21 @@ __global__ __launch_bounds__(kThreads) void k_ilu_factor_df(double *bsr, int b, 
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) { aik[(r0 + t) + size_t(c0 + j) * b] = acc[j][t]; }
+            for (int t = 0; t < 4; ++t) { st_coherent(aik + (r0 + t) + size_t(c0 + j) * b, acc[j][t]); }
           }
         }
       }
       __syncthreads();
-      stage(aik, X, b, false);
+      stage<true>(aik, X, b, false);
       for (int u = upd_ptr[ik]; u < upd_ptr[ik + 1]; ++u) {
         __syncthreads();
-        stage(bsr + size_t(upd[2 * u + 1]) * bb, Y, b, true);
+        stage<true>(bsr + size_t(upd[2 * u + 1]) * bb, Y, b, true);
         __syncthreads();
-        tile_product(X, Y, bsr + size_t(upd[2 * u]) * bb, b, 1, nullptr);
+        tile_product<true>(X, Y, bsr + size_t(upd[2 * u]) * bb, b, 1, nullptr);
       }
       __syncthreads();
     }
-    if (gj_inverse_regs<NE>(bsr + diag_ptr[i] * bb, b, sm) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
+    if (gj_inverse_regs<NE, true>(bsr + diag_ptr[i] * bb, b, sm) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
     publish_row(done, i, epoch);
   }
 }

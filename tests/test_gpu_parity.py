@@ -304,6 +304,7 @@ def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
         for form, size in ((1, o.nl), (0, o.nu)):
             if form == 0:
                 c.call("eg_dg_refresh", gpu._ptr(du))
+                o.dg_refresh(u)
             c.call("eg_pc_setup", form)
             b = np.sin(0.13 * (np.arange(size) + 1))
             y = torch.zeros(size, dtype=torch.float64, device="cuda")
