@@ -86,6 +86,7 @@ public:
 // dependency levels of the forward and backward sweeps (src/block_ilu.cpp:10-59).
 struct BlockPattern {
   int nb = 0, block = 0, nnzb = 0;
+  int max_deps = 0; // most off-diagonal blocks in one triangle of a row
   std::vector<int> row, col, diag, upd_ptr, upd;
   std::vector<int> lower_level_ptr, lower_rows, upper_level_ptr, upper_rows;
   DeviceArray<int> d_row, d_col, d_diag, d_upd_ptr, d_upd, d_lower_rows, d_upper_rows;

@@ -76,8 +76,9 @@ void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, cons
 // return false when a co-resident launch is not possible (caller falls back
 // to the per-level launches).
 template <class T>
-bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr,
-                         const int *col, const T *rhs, T *w, int *done, int epoch, int *flags, hipStream_t s);
+bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
+                         const int *diag_ptr, const int *col, const T *rhs, T *w, int *done, int epoch, int *flags,
+                         hipStream_t s);
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
                           const int *upd_ptr, const int *upd, int *done, int epoch, int *flags, hipStream_t s);
 

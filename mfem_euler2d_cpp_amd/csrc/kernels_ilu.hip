@@ -1,16 +1,18 @@
-// Dense block kernels tuned for the ILU(0)/block-Jacobi hot spots:
-//  * Gauss-Jordan inverse with the matrix resident in registers and implicit
-//    partial pivoting (no row swaps; two workgroup barriers per column),
-//  * one-row-per-workgroup ILU(0) factor with LDS-staged, 4x4 register-tiled
-//    block products,
-//  * level sweeps whose block GEMVs split every row's dot product over
-//    several threads (independent loads in flight, LDS reduction).
+// Dense block kernels for the block-Jacobi / ILU(0) hot spots, specialised on
+// the block size B (B = 4q for face blocks, 4q^2 <= 100 for cell blocks):
+//  * Gauss-Jordan inverse with the block resident in registers, implicit
+//    partial pivoting (no row swaps) and a branch-free update; 1024-thread
+//    workgroups for the 64 and 100 blocks so each SIMD has 4 waves to hide
+//    the FP64 latency of the elimination chain,
+//  * ILU(0) factor rows with LDS-staged, 4x4 register-tiled block products,
+//  * ILU(0) sweeps whose block GEMVs split each row's dot product across
+//    threads, per dependency level or as one dataflow launch.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace eg {
 namespace {
-constexpr int kThreads = 256;
-
 // Relaxed agent-scope atomic accesses: coherent across XCDs (sc1) without
 // cache write-back/invalidate; used for data that another workgroup of the
 // same launch writes or reads.
@@ -24,46 +26,60 @@ template <bool COH> __device__ __forceinline__ double ld(const double *p) { retu
 template <bool COH> __device__ __forceinline__ void st(double *p, double v) {
   if (COH) { st_coherent(p, v); } else { *p = v; }
 }
+// Complete this thread's outstanding memory operations (ordering before a
+// workgroup barrier that precedes a completion flag).
+__device__ __forceinline__ void complete_memory_ops() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
-// In-place inverse of the b x b column-major block A (global memory) by one
-// workgroup (b <= blockDim.x). Thread (tc, tr) = (tid % b, tid / b) owns
-// column tc at rows tr, tr + R, ... (R = blockDim.x / b) in registers.
-// Column k is eliminated with pivot row p = first unused row of maximal
-// |S(p,k)|; the stored matrix S then satisfies A^{-1}(pinv[i], piv[j]) = S(i,j).
-// The owners of column k publish it together with one pivot candidate each
-// (their largest unused |entry|, computed in registers), so choosing the
-// pivot costs R LDS reads instead of a column scan and wave reduction.
-// lds: 4b + 2*64 doubles, then 2*64 + 2b ints. Returns true on a zero pivot.
-constexpr int kGjLds = 4 * 128 + 2 * 64 + (2 * 64 + 2 * 128 + 1) / 2 + 1;
-template <int NE, bool COH = false>
-__device__ bool gj_inverse_regs(double *__restrict__ A, int b, double *lds) {
-  const int R = blockDim.x / b;
-  double *colk = lds, *rowp = lds + 2 * b, *cbest = lds + 4 * b;
-  int *carg = reinterpret_cast<int *>(cbest + 2 * 64), *piv = carg + 2 * 64, *pinv = piv + b;
-  const int tid = threadIdx.x;
-  const int tc = tid % b, tr = tid / b;
+template <int B> struct Shape {
+  static constexpr int R = std::min((B >= 64 ? 1024 : 256) / B, B); // row groups
+  static constexpr int NT = (B * R + 63) / 64 * 64;                  // threads per workgroup
+  static constexpr int NE = (B + R - 1) / R;                         // register entries per thread
+};
+// LDS scratch of the Gauss-Jordan (doubles): column and pivot row (double
+// buffered), R candidates (double buffered), then ints.
+template <int B> constexpr int gj_lds() { return 4 * B + 2 * Shape<B>::R + (2 * Shape<B>::R + 2 * B + 1) / 2 + 1; }
+
+// In-place inverse of the B x B column-major block A by one workgroup of
+// Shape<B>::NT threads. Thread (tc, tr) = (tid % B, tid / B) owns column tc
+// at rows tr, tr + R, ... in registers. Column k is eliminated with pivot
+// row p = the first unused row of maximal |S(p,k)| (NaN counts as 0 so the
+// index stays in range); the stored matrix S then satisfies
+// A^{-1}(pinv[i], piv[j]) = S(i,j). The owners of column k publish it with
+// one pivot candidate per row group, so the choice costs R LDS reads.
+// Mathematically the inverse of the block whose LU the reference uses
+// (src/linear.cpp:198-202, src/block_ilu.cpp:83-84, src/condensation.cpp:137-140).
+template <int B, bool COH>
+__device__ __forceinline__ bool gj_inverse(double *__restrict__ A, double *lds) {
+  constexpr int R = Shape<B>::R, NE = Shape<B>::NE;
+  double *colk = lds, *rowp = lds + 2 * B, *cbest = lds + 4 * B;
+  int *carg = reinterpret_cast<int *>(cbest + 2 * R), *piv = carg + 2 * R, *pinv = piv + B;
+  const int tid = threadIdx.x, tc = tid % B, tr = tid / B;
   const bool active = tr < R;
   double v[NE];
-  unsigned long long used = 0ull; // bit e: row tr + e*R already a pivot row
+  unsigned long long used = 0ull;
 #pragma unroll
   for (int e = 0; e < NE; ++e) {
     const int i = tr + e * R;
-    v[e] = (active && i < b) ? ld<COH>(A + i + size_t(tc) * b) : 0.;
+    v[e] = (active && i < B) ? ld<COH>(A + i + tc * B) : 0.;
   }
   __syncthreads();
   bool singular = false;
-  for (int k = 0; k < b; ++k) {
-    double *ck = colk + (k & 1) * b, *rp = rowp + (k & 1) * b, *cb = cbest + (k & 1) * 64;
-    int *ca = carg + (k & 1) * 64;
+  for (int k = 0; k < B; ++k) {
+    double *ck = colk + (k & 1) * B, *rp = rowp + (k & 1) * B, *cb = cbest + (k & 1) * R;
+    int *ca = carg + (k & 1) * R;
     if (active && tc == k) {
       double best = -1.;
-      int arg = b;
+      int arg = B;
 #pragma unroll
       for (int e = 0; e < NE; ++e) {
         const int i = tr + e * R;
-        if (i < b) {
+        if (i < B) {
           ck[i] = v[e];
-          const double a = fabs(v[e]);
+          const double a = isnan(v[e]) ? 0. : fabs(v[e]);
           if (!((used >> e) & 1ull) && a > best) { best = a; arg = i; }
         }
       }
@@ -72,35 +88,38 @@ __device__ bool gj_inverse_regs(double *__restrict__ A, int b, double *lds) {
     }
     __syncthreads();
     double best = -1.;
-    int p = b;
+    int p = B;
+#pragma unroll 4
     for (int g = 0; g < R; ++g) {
       const double ob = cb[g];
       const int oa = ca[g];
       if (ob > best || (ob == best && oa < p)) { best = ob; p = oa; }
     }
     const double ap = ck[p];
-    if (ap == 0.) { singular = true; }
+    if (!(fabs(ap) > 0.) || !isfinite(ap)) { singular = true; }
     const double inv = 1. / ap;
-    if (active && tr == p % R) {
-      const int ep = p / R;
+    const int ep = p / R;
+    if (active && tr == p - ep * R) {
       used |= 1ull << ep;
+      double sel = 0.;
 #pragma unroll
-      for (int e = 0; e < NE; ++e) {
-        if (e == ep) { rp[tc] = v[e]; }
-      }
+      for (int e = 0; e < NE; ++e) { sel = e == ep ? v[e] : sel; }
+      rp[tc] = sel;
     }
     __syncthreads();
     if (tid == 0) { piv[k] = p; pinv[p] = k; }
     if (active) {
       const double rpj = rp[tc] * inv;
+      const bool is_col = tc == k;
 #pragma unroll
       for (int e = 0; e < NE; ++e) {
         const int i = tr + e * R;
-        if (i < b) {
-          if (i == p) { v[e] = tc == k ? inv : v[e] * inv; }
-          else if (tc == k) { v[e] = -ck[i] * inv; }
-          else { v[e] = v[e] - ck[i] * rpj; }
-        }
+        const double c = ck[i < B ? i : 0];
+        const bool is_row = i == p;
+        double nv = v[e] - c * rpj;
+        nv = is_row ? v[e] * inv : nv;
+        nv = is_col ? (is_row ? inv : -c * inv) : nv;
+        v[e] = nv;
       }
     }
   }
@@ -109,37 +128,44 @@ __device__ bool gj_inverse_regs(double *__restrict__ A, int b, double *lds) {
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       const int i = tr + e * R;
-      if (i < b) { st<COH>(A + pinv[i] + size_t(piv[tc]) * b, v[e]); }
+      if (i < B) { st<COH>(A + pinv[i] + piv[tc] * B, v[e]); }
     }
   }
   __syncthreads();
   return singular;
 }
 
-template <int NE>
-__global__ __launch_bounds__(kThreads) void k_block_inverse_regs(double *__restrict__ blocks, int b,
-                                                                 const int *__restrict__ index, int *__restrict__ flags) {
-  __shared__ double lds[kGjLds];
+template <int B>
+__global__ __launch_bounds__(Shape<B>::NT) void k_block_inverse_b(double *__restrict__ blocks,
+                                                                  const int *__restrict__ index,
+                                                                  int *__restrict__ flags) {
+  __shared__ double lds[gj_lds<B>()];
   const int blk = index ? index[blockIdx.x] : blockIdx.x;
-  if (gj_inverse_regs<NE>(blocks + size_t(blk) * b * b, b, lds) && threadIdx.x == 0) {
-    atomicOr(flags, kErrSingular);
-  }
+  if (gj_inverse<B, false>(blocks + size_t(blk) * B * B, lds) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
 }
 
-// C (-)= X Y for b x b blocks: X and Yt (Y transposed) in LDS, C in global.
-// Each thread computes 4x4 tiles; mode 0 stores C = X Y (and into Xout in
-// LDS when given), mode 1 subtracts from C.
-template <bool COH = false>
-__device__ void tile_product(const double *__restrict__ X, const double *__restrict__ Yt, double *__restrict__ C,
-                             int b, int mode, double *__restrict__ Xout) {
-  const int nt4 = b / 4, tiles = nt4 * nt4;
+// Stage a global column-major block into LDS, optionally transposed.
+template <int B, bool COH> __device__ __forceinline__ void stage(const double *__restrict__ src, double *dst, bool tr) {
+  for (int t = threadIdx.x; t < B * B; t += blockDim.x) {
+    const double v = ld<COH>(src + t);
+    if (tr) { dst[(t / B) + (t % B) * B] = v; }
+    else { dst[t] = v; }
+  }
+}
+// 4x4-tiled product of X (LDS, column-major) and Y (LDS, transposed):
+// mode 0: C = X Y, mode 1: C -= X Y, C in global memory.
+template <int B, bool COH>
+__device__ __forceinline__ void tile_product(const double *__restrict__ X, const double *__restrict__ Yt,
+                                             double *__restrict__ C, int mode) {
+  constexpr int nt4 = B / 4, tiles = nt4 * nt4;
   for (int tile = threadIdx.x; tile < tiles; tile += blockDim.x) {
     const int r0 = (tile % nt4) * 4, c0 = (tile / nt4) * 4;
     double acc[4][4] = {};
-    for (int kk = 0; kk < b; ++kk) {
+#pragma unroll 4
+    for (int kk = 0; kk < B; ++kk) {
       double a[4], y[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) { a[i] = X[r0 + i + kk * b]; y[i] = Yt[c0 + i + kk * b]; }
+      for (int i = 0; i < 4; ++i) { a[i] = X[r0 + i + kk * B]; y[i] = Yt[c0 + i + kk * B]; }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
 #pragma unroll
@@ -150,94 +176,61 @@ __device__ void tile_product(const double *__restrict__ X, const double *__restr
     for (int j = 0; j < 4; ++j) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const size_t idx = (r0 + i) + size_t(c0 + j) * b;
-        if (mode == 0) {
-          st<COH>(C + idx, acc[j][i]);
-          if (Xout) { Xout[idx] = acc[j][i]; }
-        } else {
-          st<COH>(C + idx, ld<COH>(C + idx) - acc[j][i]);
-        }
+        double *cp = C + (r0 + i) + (c0 + j) * B;
+        st<COH>(cp, mode == 0 ? acc[j][i] : ld<COH>(cp) - acc[j][i]);
       }
     }
   }
 }
-// Stage a global column-major block into LDS, optionally transposed.
-template <bool COH = false>
-__device__ void stage(const double *__restrict__ src, double *__restrict__ dst, int b, bool transpose) {
-  for (int t = threadIdx.x; t < b * b; t += blockDim.x) {
-    const int r = t % b, c = t / b;
-    const double v = ld<COH>(src + t);
-    if (transpose) { dst[c + r * b] = v; }
-    else { dst[t] = v; }
-  }
-}
-
-// One BSR row of ILU(0) per workgroup (src/block_ilu.cpp:73-86), rows of one
-// dependency level in parallel. L_ik = A_ik inv(A_kk) and A_ij -= L_ik A_kj
-// as LDS-staged tile products; the updated diagonal block is replaced by its
-// inverse. LDS: X (L_ik) and Y (second operand, transposed): 2 b^2 doubles.
-template <int NE>
-__global__ __launch_bounds__(kThreads) void k_ilu_factor_rows(double *__restrict__ bsr, int b,
-                                                              const int *__restrict__ rows,
-                                                              const int *__restrict__ row_ptr,
-                                                              const int *__restrict__ diag_ptr,
-                                                              const int *__restrict__ col,
-                                                              const int *__restrict__ upd_ptr,
-                                                              const int *__restrict__ upd, int *__restrict__ flags) {
-  extern __shared__ double sm[];
-  const size_t bb = size_t(b) * b;
+// ILU(0) row i (src/block_ilu.cpp:73-86): for each lower block, L_ik = A_ik
+// inv(A_kk) and A_ij -= L_ik A_kj over the cached update pairs; then the
+// updated diagonal block is replaced by its inverse. LDS: 2 B^2 doubles.
+template <int B, bool COH>
+__device__ __forceinline__ bool factor_row(double *bsr, int i, const int *__restrict__ row_ptr,
+                                           const int *__restrict__ diag_ptr, const int *__restrict__ col,
+                                           const int *__restrict__ upd_ptr, const int *__restrict__ upd, double *sm) {
+  constexpr size_t bb = size_t(B) * B;
   double *X = sm, *Y = sm + bb;
-  const int i = rows[blockIdx.x];
   for (int ik = row_ptr[i]; ik < diag_ptr[i]; ++ik) {
-    const int k = col[ik];
     double *aik = bsr + ik * bb;
-    stage(aik, Y, b, false);                         // Y <- A_ik (untransposed: used as X operand)
-    stage(bsr + diag_ptr[k] * bb, X, b, true);       // X <- inv(A_kk)^T
+    stage<B, COH>(aik, X, false);                        // X <- A_ik
+    stage<B, COH>(bsr + diag_ptr[col[ik]] * bb, Y, true); // Y <- inv(A_kk)^T
     __syncthreads();
-    // L = A_ik * inv(A_kk): operands (Y, X^T) -> write to A_ik; keep L in Y.
-    {
-      const int nt4 = b / 4, tiles = nt4 * nt4;
-      double *L = aik;
-      for (int tile = threadIdx.x; tile < tiles; tile += blockDim.x) {
-        const int r0 = (tile % nt4) * 4, c0 = (tile / nt4) * 4;
-        double acc[4][4] = {};
-        for (int kk = 0; kk < b; ++kk) {
-          double a[4], y[4];
-#pragma unroll
-          for (int t = 0; t < 4; ++t) { a[t] = Y[r0 + t + kk * b]; y[t] = X[c0 + t + kk * b]; }
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) { acc[j][t] += a[t] * y[j]; }
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-#pragma unroll
-          for (int t = 0; t < 4; ++t) { L[(r0 + t) + size_t(c0 + j) * b] = acc[j][t]; }
-        }
-      }
-    }
+    tile_product<B, COH>(X, Y, aik, 0);                   // A_ik <- L_ik
     __syncthreads();
-    stage(aik, X, b, false);  // X <- L_ik
+    stage<B, COH>(aik, X, false);                        // X <- L_ik
     for (int u = upd_ptr[ik]; u < upd_ptr[ik + 1]; ++u) {
       __syncthreads();
-      stage(bsr + size_t(upd[2 * u + 1]) * bb, Y, b, true);  // Y <- A_kj^T
+      stage<B, COH>(bsr + size_t(upd[2 * u + 1]) * bb, Y, true); // Y <- A_kj^T
       __syncthreads();
-      tile_product(X, Y, bsr + size_t(upd[2 * u]) * bb, b, 1, nullptr);
+      tile_product<B, COH>(X, Y, bsr + size_t(upd[2 * u]) * bb, 1);
     }
     __syncthreads();
   }
-  __threadfence_block();
-  if (gj_inverse_regs<NE>(bsr + diag_ptr[i] * bb, b, sm) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
+  return gj_inverse<B, COH>(bsr + diag_ptr[i] * bb, sm);
+}
+template <int B> constexpr size_t factor_lds() {
+  return std::max<size_t>(2 * size_t(B) * B, size_t(gj_lds<B>())) * sizeof(double);
+}
+// One row per workgroup, rows of one dependency level in parallel.
+template <int B>
+__global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_rows(double *__restrict__ bsr, const int *__restrict__ rows,
+                                                                  const int *__restrict__ row_ptr,
+                                                                  const int *__restrict__ diag_ptr,
+                                                                  const int *__restrict__ col,
+                                                                  const int *__restrict__ upd_ptr,
+                                                                  const int *__restrict__ upd, int *__restrict__ flags) {
+  extern __shared__ double sm[];
+  if (factor_row<B, false>(bsr, rows[blockIdx.x], row_ptr, diag_ptr, col, upd_ptr, upd, sm) && threadIdx.x == 0) {
+    atomicOr(flags, kErrSingular);
+  }
 }
 
-// Level sweeps: every row's dot products are split over P = blockDim/b
-// threads (columns c = part, part + P, ...) and reduced through LDS in a
-// fixed order.
+// ------------------------------------------------------------ sweeps
+// Dot products of block row r with x, split over P = blockDim/b threads
+// (columns c = part, part + P, ...).
 template <class T>
-__device__ __forceinline__ T row_dot(const T *__restrict__ a, const T *__restrict__ x, int b, int r, int part,
-                                     int P) {
+__device__ __forceinline__ T row_dot(const T *__restrict__ a, const T *__restrict__ x, int b, int r, int part, int P) {
   T acc0 = T(0), acc1 = T(0);
   int c = part;
   for (; c + P < b; c += 2 * P) {
@@ -247,23 +240,22 @@ __device__ __forceinline__ T row_dot(const T *__restrict__ a, const T *__restric
   if (c < b) { acc0 += a[size_t(c) * b + r] * x[c]; }
   return acc0 + acc1;
 }
+constexpr int kSweepThreads = 256;
 // Forward sweep of one level (src/block_ilu.cpp:87-99): w_i = b_i - sum_{j<i} L_ij w_j.
 template <class T>
-__global__ __launch_bounds__(kThreads) void k_ilu_lower_split(const T *__restrict__ bsr, int b,
-                                                              const int *__restrict__ rows,
-                                                              const int *__restrict__ row_ptr,
-                                                              const int *__restrict__ diag_ptr,
-                                                              const int *__restrict__ col, const T *__restrict__ rhs,
-                                                              T *__restrict__ w) {
-  __shared__ T red[kThreads];
+__global__ __launch_bounds__(kSweepThreads) void k_ilu_lower_split(const T *__restrict__ bsr, int b,
+                                                                   const int *__restrict__ rows,
+                                                                   const int *__restrict__ row_ptr,
+                                                                   const int *__restrict__ diag_ptr,
+                                                                   const int *__restrict__ col,
+                                                                   const T *__restrict__ rhs, T *__restrict__ w) {
+  __shared__ T red[kSweepThreads];
   const int i = rows[blockIdx.x];
   const size_t bb = size_t(b) * b;
   const int P = blockDim.x / b, r = threadIdx.x % b, part = threadIdx.x / b;
   T acc = T(0);
   if (part < P) {
-    for (int ij = row_ptr[i]; ij < diag_ptr[i]; ++ij) {
-      acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P);
-    }
+    for (int ij = row_ptr[i]; ij < diag_ptr[i]; ++ij) { acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P); }
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -276,21 +268,19 @@ __global__ __launch_bounds__(kThreads) void k_ilu_lower_split(const T *__restric
 // Backward sweep of one level (src/block_ilu.cpp:100-108):
 // w_i = inv(U_ii) (w_i - sum_{j>i} U_ij w_j).
 template <class T>
-__global__ __launch_bounds__(kThreads) void k_ilu_upper_split(const T *__restrict__ bsr, int b,
-                                                              const int *__restrict__ rows,
-                                                              const int *__restrict__ row_ptr,
-                                                              const int *__restrict__ diag_ptr,
-                                                              const int *__restrict__ col, T *__restrict__ w) {
-  __shared__ T red[kThreads];
-  __shared__ T ts[kThreads];
+__global__ __launch_bounds__(kSweepThreads) void k_ilu_upper_split(const T *__restrict__ bsr, int b,
+                                                                   const int *__restrict__ rows,
+                                                                   const int *__restrict__ row_ptr,
+                                                                   const int *__restrict__ diag_ptr,
+                                                                   const int *__restrict__ col, T *__restrict__ w) {
+  __shared__ T red[kSweepThreads];
+  __shared__ T ts[kSweepThreads];
   const int i = rows[blockIdx.x];
   const size_t bb = size_t(b) * b;
   const int P = blockDim.x / b, r = threadIdx.x % b, part = threadIdx.x / b;
   T acc = T(0);
   if (part < P) {
-    for (int ij = diag_ptr[i] + 1; ij < row_ptr[i + 1]; ++ij) {
-      acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P);
-    }
+    for (int ij = diag_ptr[i] + 1; ij < row_ptr[i + 1]; ++ij) { acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P); }
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -309,26 +299,18 @@ __global__ __launch_bounds__(kThreads) void k_ilu_upper_split(const T *__restric
     w[size_t(i) * b + r] = s;
   }
 }
+
 // ------------------------------------------------------------ dataflow forms
 // One launch per sweep (or per factorisation) instead of one per level: the
-// grid is co-resident (cooperative launch) and every workgroup handles its
+// grid is co-resident (cooperative launch) and each workgroup handles its
 // rows in dependency order (increasing for the lower sweep and the factor,
-// decreasing for the upper sweep). Before a row it waits until each row it
-// depends on has published done[j] >= epoch; after it, it releases its
-// results and publishes done[i] = epoch. Since the rows a workgroup waits on
-// precede its own row in that order and all workgroups are resident, every
-// wait ends; a wait is nevertheless bounded by spin_limit and then raises
-// kErrSpin instead of hanging.
-//
-// Cross-workgroup data (the sweep vector and the flags) is accessed only with
-// relaxed agent-scope atomics, which the gfx950 memory model makes coherent
-// across XCDs without L2 write-back/invalidate; the writer completes its
-// stores (s_waitcnt) before the workgroup barrier that precedes the flag.
-__device__ __forceinline__ void complete_memory_ops() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0);
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
+// decreasing for the upper sweep). Before a row it waits until every row it
+// depends on has published done[j] >= epoch; after the row it completes its
+// stores and publishes done[i] = epoch. As the rows waited on precede the
+// waiting row in that order and all workgroups are resident, every wait
+// ends; each wait is nevertheless bounded by spin_limit and then raises
+// kErrSpin instead of hanging. Data crossing workgroups is read and written
+// with coherent (sc1) accesses only.
 __device__ __forceinline__ void wait_rows(const int *__restrict__ col, int j0, int j1, const int *done, int epoch,
                                           int spin_limit, int *flags) {
   for (int ij = j0 + int(threadIdx.x); ij < j1; ij += blockDim.x) {
@@ -342,7 +324,6 @@ __device__ __forceinline__ void wait_rows(const int *__restrict__ col, int j0, i
       __builtin_amdgcn_s_sleep(1);
     }
   }
-  complete_memory_ops();
   __syncthreads();
 }
 __device__ __forceinline__ void publish_row(int *done, int i, int epoch) {
@@ -350,39 +331,37 @@ __device__ __forceinline__ void publish_row(int *done, int i, int epoch) {
   __syncthreads();
   if (threadIdx.x == 0) { st_coherent(done + i, epoch); }
 }
-// row_dot with the vector entries read coherently (written by other
-// workgroups of the same launch).
+// Gather the dependency vectors of row i (coherently, one element per thread)
+// into LDS: xs[(ij - j0) * b + c] = w[col[ij] * b + c].
 template <class T>
-__device__ __forceinline__ T row_dot_coherent(const T *__restrict__ a, const T *x, int b, int r, int part, int P) {
-  T acc0 = T(0), acc1 = T(0);
-  int c = part;
-  for (; c + P < b; c += 2 * P) {
-    acc0 += a[size_t(c) * b + r] * ld_coherent(x + c);
-    acc1 += a[size_t(c + P) * b + r] * ld_coherent(x + c + P);
+__device__ __forceinline__ void gather_deps(const T *w, const int *__restrict__ col, int j0, int j1, int b, T *xs) {
+  const int total = (j1 - j0) * b;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int d = t / b, c = t % b;
+    xs[t] = ld_coherent(w + size_t(col[j0 + d]) * b + c);
   }
-  if (c < b) { acc0 += a[size_t(c) * b + r] * ld_coherent(x + c); }
-  return acc0 + acc1;
+  __syncthreads();
 }
-
+constexpr int kMaxDeps = 8;
 template <class T, bool LOWER>
-__global__ __launch_bounds__(kThreads) void k_ilu_sweep_df(const T *__restrict__ bsr, int b, int nb,
-                                                           const int *__restrict__ row_ptr,
-                                                           const int *__restrict__ diag_ptr,
-                                                           const int *__restrict__ col, const T *__restrict__ rhs,
-                                                           T *w, int *done, int epoch, int *flags, int spin_limit) {
-  __shared__ T red[kThreads];
-  __shared__ T ts[kThreads];
+__global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_df(const T *__restrict__ bsr, int b, int nb,
+                                                                const int *__restrict__ row_ptr,
+                                                                const int *__restrict__ diag_ptr,
+                                                                const int *__restrict__ col, const T *__restrict__ rhs,
+                                                                T *w, int *done, int epoch, int *flags, int spin_limit) {
+  __shared__ T red[kSweepThreads];
+  __shared__ T ts[kSweepThreads];
+  __shared__ T xs[kMaxDeps * kSweepThreads];
   const size_t bb = size_t(b) * b;
   const int P = blockDim.x / b, r = threadIdx.x % b, part = threadIdx.x / b;
   for (int idx = blockIdx.x; idx < nb; idx += gridDim.x) {
     const int i = LOWER ? idx : nb - 1 - idx;
     const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
     wait_rows(col, j0, j1, done, epoch, spin_limit, flags);
+    gather_deps(w, col, j0, j1, b, xs);
     T acc = T(0);
     if (part < P) {
-      for (int ij = j0; ij < j1; ++ij) {
-        acc += row_dot_coherent(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P);
-      }
+      for (int ij = j0; ij < j1; ++ij) { acc += row_dot(bsr + ij * bb, xs + (ij - j0) * b, b, r, part, P); }
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -406,183 +385,127 @@ __global__ __launch_bounds__(kThreads) void k_ilu_sweep_df(const T *__restrict__
     publish_row(done, i, epoch);
   }
 }
-
-// Dataflow ILU(0) factor: the body of k_ilu_factor_rows per row, rows in
-// increasing order per workgroup, waiting on the rows of the lower pattern.
-template <int NE>
-__global__ __launch_bounds__(kThreads) void k_ilu_factor_df(double *bsr, int b, int nb, const int *__restrict__ row_ptr,
-                                                            const int *__restrict__ diag_ptr,
-                                                            const int *__restrict__ col, const int *__restrict__ upd_ptr,
-                                                            const int *__restrict__ upd, int *done, int epoch,
-                                                            int *flags, int spin_limit) {
+// Dataflow ILU(0) factor: factor_row per row, rows in increasing order per
+// workgroup, waiting on the rows of the lower pattern.
+template <int B>
+__global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_df(double *bsr, int nb, const int *__restrict__ row_ptr,
+                                                                const int *__restrict__ diag_ptr,
+                                                                const int *__restrict__ col,
+                                                                const int *__restrict__ upd_ptr,
+                                                                const int *__restrict__ upd, int *done, int epoch,
+                                                                int *flags, int spin_limit) {
   extern __shared__ double sm[];
-  const size_t bb = size_t(b) * b;
-  double *X = sm, *Y = sm + bb;
   for (int i = blockIdx.x; i < nb; i += gridDim.x) {
     wait_rows(col, row_ptr[i], diag_ptr[i], done, epoch, spin_limit, flags);
-    for (int ik = row_ptr[i]; ik < diag_ptr[i]; ++ik) {
-      const int k = col[ik];
-      double *aik = bsr + ik * bb;
-      stage<true>(aik, Y, b, false);
-      stage<true>(bsr + diag_ptr[k] * bb, X, b, true);
-      __syncthreads();
-      {
-        const int nt4 = b / 4, tiles = nt4 * nt4;
-        for (int tile = threadIdx.x; tile < tiles; tile += blockDim.x) {
-          const int r0 = (tile % nt4) * 4, c0 = (tile / nt4) * 4;
-          double acc[4][4] = {};
-          for (int kk = 0; kk < b; ++kk) {
-            double a[4], y[4];
-#pragma unroll
-            for (int t = 0; t < 4; ++t) { a[t] = Y[r0 + t + kk * b]; y[t] = X[c0 + t + kk * b]; }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-#pragma unroll
-              for (int t = 0; t < 4; ++t) { acc[j][t] += a[t] * y[j]; }
-            }
-          }
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) { st_coherent(aik + (r0 + t) + size_t(c0 + j) * b, acc[j][t]); }
-          }
-        }
-      }
-      __syncthreads();
-      stage<true>(aik, X, b, false);
-      for (int u = upd_ptr[ik]; u < upd_ptr[ik + 1]; ++u) {
-        __syncthreads();
-        stage<true>(bsr + size_t(upd[2 * u + 1]) * bb, Y, b, true);
-        __syncthreads();
-        tile_product<true>(X, Y, bsr + size_t(upd[2 * u]) * bb, b, 1, nullptr);
-      }
-      __syncthreads();
+    if (factor_row<B, true>(bsr, i, row_ptr, diag_ptr, col, upd_ptr, upd, sm) && threadIdx.x == 0) {
+      atomicOr(flags, kErrSingular);
     }
-    if (gj_inverse_regs<NE, true>(bsr + diag_ptr[i] * bb, b, sm) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
     publish_row(done, i, epoch);
   }
 }
 
-int ne_for(int b) {
-  const int R = kThreads / b, need = (b + R - 1) / R;
-  int ne = 1;
-  while (ne < need) { ne *= 2; }
-  return ne;
-}
-} // namespace
-
-#define EG_DISPATCH_NE(ne, CALL)                                                                   \
-  switch (ne) {                                                                                    \
-  case 1: { constexpr int NE = 1; CALL; } break;                                                   \
-  case 2: { constexpr int NE = 2; CALL; } break;                                                   \
-  case 4: { constexpr int NE = 4; CALL; } break;                                                   \
-  case 8: { constexpr int NE = 8; CALL; } break;                                                   \
-  case 16: { constexpr int NE = 16; CALL; } break;                                                 \
-  case 32: { constexpr int NE = 32; CALL; } break;                                                 \
-  default: { constexpr int NE = 64; CALL; } break;                                                 \
-  }
-
-bool fast_dense_ok(int b) { return b <= 128 && b % 4 == 0; }
-void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
-  EG_DISPATCH_NE(ne_for(b), (k_block_inverse_regs<NE><<<count, kThreads, 0, s>>>(blocks, b, index, flags)));
-}
-int ilu_factor_lds_init() {
-  int bad = 0;
-#define EG_SET(NE)                                                                                  \
-  {                                                                                                \
-    hipFuncAttributes attr{};                                                                      \
-    const void *fn = reinterpret_cast<const void *>(k_ilu_factor_rows<NE>);                        \
-    if (hipFuncGetAttributes(&attr, fn) != hipSuccess ||                                           \
-        hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,                        \
-                            163840 - int(attr.sharedSizeBytes)) != hipSuccess) {                   \
-      bad = 1;                                                                                     \
-    }                                                                                              \
-  }
-  EG_SET(1) EG_SET(2) EG_SET(4) EG_SET(8) EG_SET(16) EG_SET(32) EG_SET(64)
-#undef EG_SET
-  (void)hipGetLastError();
-  return bad;
-}
-void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
-                            const int *col, const int *upd_ptr, const int *upd, int *flags, hipStream_t s) {
-  const size_t lds = std::max<size_t>(2 * size_t(b) * b * 8, size_t(kGjLds) * 8);
-  EG_DISPATCH_NE(ne_for(b), (k_ilu_factor_rows<NE><<<nrows, kThreads, lds, s>>>(bsr, b, rows, row_ptr, diag_ptr, col,
-                                                                                 upd_ptr, upd, flags)));
-}
-template <class T>
-void launch_ilu_lower_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
-                           const int *col, const T *rhs, T *w, hipStream_t s) {
-  k_ilu_lower_split<T><<<nrows, kThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w);
-}
-template <class T>
-void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
-                           const int *col, T *w, hipStream_t s) {
-  k_ilu_upper_split<T><<<nrows, kThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w);
-}
 // Largest co-resident grid of a kernel (all CUs x blocks per CU).
-static int coresident(const void *fn, size_t lds) {
+int coresident(const void *fn, int threads, size_t lds) {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kThreads, lds) != hipSuccess) {
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, lds) != hipSuccess) {
     (void)hipGetLastError();
     return 0;
   }
   return cus * per;
 }
+void allow_lds(const void *fn) {
+  hipFuncAttributes attr{};
+  if (hipFuncGetAttributes(&attr, fn) == hipSuccess) {
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 163840 - int(attr.sharedSizeBytes));
+  }
+  (void)hipGetLastError();
+}
 constexpr int kSpinLimit = 1 << 20;
+} // namespace
 
+#define EG_DISPATCH_B(b, CALL)                                                                     \
+  switch (b) {                                                                                     \
+  case 4: { constexpr int B = 4; CALL; } break;                                                    \
+  case 8: { constexpr int B = 8; CALL; } break;                                                    \
+  case 12: { constexpr int B = 12; CALL; } break;                                                  \
+  case 16: { constexpr int B = 16; CALL; } break;                                                  \
+  case 20: { constexpr int B = 20; CALL; } break;                                                  \
+  case 24: { constexpr int B = 24; CALL; } break;                                                  \
+  case 28: { constexpr int B = 28; CALL; } break;                                                  \
+  case 32: { constexpr int B = 32; CALL; } break;                                                  \
+  case 36: { constexpr int B = 36; CALL; } break;                                                  \
+  case 64: { constexpr int B = 64; CALL; } break;                                                  \
+  case 100: { constexpr int B = 100; CALL; } break;                                                \
+  default: break;                                                                                  \
+  }
+
+bool fast_dense_ok(int b) {
+  return (b % 4 == 0 && b >= 4 && b <= 36) || b == 64 || b == 100;
+}
+int ilu_factor_lds_init() {
+#define EG_ALLOW(B)                                                                                \
+  allow_lds(reinterpret_cast<const void *>(k_ilu_factor_rows<B>));                                 \
+  allow_lds(reinterpret_cast<const void *>(k_ilu_factor_df<B>));
+  EG_ALLOW(4) EG_ALLOW(8) EG_ALLOW(12) EG_ALLOW(16) EG_ALLOW(20) EG_ALLOW(24) EG_ALLOW(28) EG_ALLOW(32)
+  EG_ALLOW(36) EG_ALLOW(64) EG_ALLOW(100)
+#undef EG_ALLOW
+  return 0;
+}
+void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
+  EG_DISPATCH_B(b, (k_block_inverse_b<B><<<count, Shape<B>::NT, 0, s>>>(blocks, index, flags)));
+}
+void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                            const int *col, const int *upd_ptr, const int *upd, int *flags, hipStream_t s) {
+  EG_DISPATCH_B(b, (k_ilu_factor_rows<B><<<nrows, Shape<B>::NT, factor_lds<B>(), s>>>(bsr, rows, row_ptr, diag_ptr,
+                                                                                       col, upd_ptr, upd, flags)));
+}
 template <class T>
-bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr,
-                         const int *col, const T *rhs, T *w, int *done, int epoch, int *flags, hipStream_t s) {
-  if (b > kThreads) { return false; }
+void launch_ilu_lower_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                           const int *col, const T *rhs, T *w, hipStream_t s) {
+  k_ilu_lower_split<T><<<nrows, kSweepThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w);
+}
+template <class T>
+void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                           const int *col, T *w, hipStream_t s) {
+  k_ilu_upper_split<T><<<nrows, kSweepThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w);
+}
+template <class T>
+bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
+                         const int *diag_ptr, const int *col, const T *rhs, T *w, int *done, int epoch, int *flags,
+                         hipStream_t s) {
+  if (b > kSweepThreads || max_deps * b > kMaxDeps * kSweepThreads) { return false; }
   const void *fn = lower ? reinterpret_cast<const void *>(k_ilu_sweep_df<T, true>)
                          : reinterpret_cast<const void *>(k_ilu_sweep_df<T, false>);
-  static int cap_l = -1, cap_u = -1;
-  int &cap = lower ? cap_l : cap_u;
-  if (cap < 0) { cap = coresident(fn, 0); }
+  const int cap = coresident(fn, kSweepThreads, 0);
   if (cap <= 0) { return false; }
   const int grid = std::min(nb, cap);
   int spin = kSpinLimit;
   void *args[] = {&bsr, &b, &nb, &row_ptr, &diag_ptr, &col, &rhs, &w, &done, &epoch, &flags, &spin};
-  if (hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kThreads), args, 0, s) != hipSuccess) {
+  if (hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kSweepThreads), args, 0, s) != hipSuccess) {
     (void)hipGetLastError();
     return false;
   }
   return true;
 }
-template bool launch_ilu_sweep_df<double>(bool, const double *, int, int, const int *, const int *, const int *,
-                                          const double *, double *, int *, int, int *, hipStream_t);
-template bool launch_ilu_sweep_df<float>(bool, const float *, int, int, const int *, const int *, const int *,
-                                         const float *, float *, int *, int, int *, hipStream_t);
-
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
                           const int *upd_ptr, const int *upd, int *done, int epoch, int *flags, hipStream_t s) {
-  if (!fast_dense_ok(b) || b > 100) { return false; }
-  const size_t lds = std::max<size_t>(2 * size_t(b) * b * 8, size_t(kGjLds) * 8);
-  const void *fn = nullptr;
-  EG_DISPATCH_NE(ne_for(b), (fn = reinterpret_cast<const void *>(k_ilu_factor_df<NE>)));
-  static int lds_set[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  int slot = 0;
-  for (int ne = ne_for(b); ne > 1; ne >>= 1) { ++slot; }
-  if (!lds_set[slot]) {
-    hipFuncAttributes attr{};
-    if (hipFuncGetAttributes(&attr, fn) == hipSuccess) {
-      (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 163840 - int(attr.sharedSizeBytes));
+  if (!fast_dense_ok(b)) { return false; }
+  bool ok = false;
+  EG_DISPATCH_B(b, ({
+    const void *fn = reinterpret_cast<const void *>(k_ilu_factor_df<B>);
+    const size_t lds = factor_lds<B>();
+    const int cap = coresident(fn, Shape<B>::NT, lds);
+    if (cap > 0) {
+      const int grid = std::min(nb, cap);
+      int spin = kSpinLimit;
+      void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &upd_ptr, &upd, &done, &epoch, &flags, &spin};
+      ok = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(Shape<B>::NT), args, lds, s) == hipSuccess;
+      if (!ok) { (void)hipGetLastError(); }
     }
-    (void)hipGetLastError();
-    lds_set[slot] = 1;
-  }
-  const int cap = coresident(fn, lds);
-  if (cap <= 0) { return false; }
-  const int grid = std::min(nb, cap);
-  int spin = kSpinLimit;
-  void *args[] = {&bsr, &b, &nb, &row_ptr, &diag_ptr, &col, &upd_ptr, &upd, &done, &epoch, &flags, &spin};
-  if (hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kThreads), args, lds, s) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
-  return true;
+  }));
+  return ok;
 }
 
 template void launch_ilu_lower_fast<double>(const double *, int, const int *, int, const int *, const int *,
@@ -593,4 +516,8 @@ template void launch_ilu_upper_fast<double>(const double *, int, const int *, in
                                             const int *, double *, hipStream_t);
 template void launch_ilu_upper_fast<float>(const float *, int, const int *, int, const int *, const int *,
                                            const int *, float *, hipStream_t);
+template bool launch_ilu_sweep_df<double>(bool, const double *, int, int, int, const int *, const int *, const int *,
+                                          const double *, double *, int *, int, int *, hipStream_t);
+template bool launch_ilu_sweep_df<float>(bool, const float *, int, int, int, const int *, const int *, const int *,
+                                         const float *, float *, int *, int, int *, hipStream_t);
 } // namespace eg

@@ -30,7 +30,8 @@ __global__ __launch_bounds__(256) void k_block_inverse(double *__restrict__ bloc
       double best = -1.;
       int arg = b;
       for (int r = k + tid; r < b; r += 64) {
-        const double v = fabs(A[r + k * b]);
+        const double a = A[r + k * b];
+        const double v = isnan(a) ? 0. : fabs(a); // keeps the pivot index in range
         if (v > best) { best = v; arg = r; }
       }
       for (int off = 32; off > 0; off >>= 1) {
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(256) void k_block_inverse(double *__restrict__ bloc
     for (int i = tid; i < b; i += nt) { colk[i] = A[i + k * b]; rowk[i] = A[k + i * b]; }
     __syncthreads();
     const double pivot = rowk[k];
-    if (pivot == 0.) { singular = true; }
+    if (!(fabs(pivot) > 0.) || !isfinite(pivot)) { singular = true; }
     const double inv = 1. / pivot;
     for (int idx = tid; idx < b * b; idx += nt) {
       const int i = idx % b, j = idx / b;
@@ -154,7 +155,8 @@ __global__ __launch_bounds__(256) void k_ilu_factor(double *__restrict__ bsr, in
       double best = -1.;
       int arg = b;
       for (int r = k + tid; r < b; r += 64) {
-        const double v = fabs(A[r + k * b]);
+        const double a = A[r + k * b];
+        const double v = isnan(a) ? 0. : fabs(a); // keeps the pivot index in range
         if (v > best) { best = v; arg = r; }
       }
       for (int off = 32; off > 0; off >>= 1) {
@@ -177,7 +179,7 @@ __global__ __launch_bounds__(256) void k_ilu_factor(double *__restrict__ bsr, in
     for (int r = tid; r < b; r += nt) { colk[r] = A[r + k * b]; rowk[r] = A[k + r * b]; }
     __syncthreads();
     const double pivot = rowk[k];
-    if (pivot == 0.) { singular = true; }
+    if (!(fabs(pivot) > 0.) || !isfinite(pivot)) { singular = true; }
     const double inv = 1. / pivot;
     for (int idx = tid; idx < bb; idx += nt) {
       const int r = idx % b, c = idx / b;

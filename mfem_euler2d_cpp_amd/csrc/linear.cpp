@@ -148,10 +148,10 @@ template <class T> static void precondition(LinearSolver &s, const T *bsr, const
   const auto &pt = *s.pattern;
   if (s.dataflow) {
     const int e0 = ++s.epoch_sweep;
-    if (eg::launch_ilu_sweep_df<T>(true, bsr, s.block, pt.nb, pt.d_row, pt.d_diag, pt.d_col, b, y, s.done_sweep, e0,
+    if (eg::launch_ilu_sweep_df<T>(true, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col, b, y, s.done_sweep, e0,
                                    s.p.flags, s.p.stream)) {
       const int e1 = ++s.epoch_sweep;
-      if (!eg::launch_ilu_sweep_df<T>(false, bsr, s.block, pt.nb, pt.d_row, pt.d_diag, pt.d_col, b, y, s.done_sweep,
+      if (!eg::launch_ilu_sweep_df<T>(false, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col, b, y, s.done_sweep,
                                       e1, s.p.flags, s.p.stream)) {
         throw std::runtime_error("Co-resident launch of the upper ILU sweep failed");
       }

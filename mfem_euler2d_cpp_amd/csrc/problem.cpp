@@ -498,6 +498,7 @@ BlockPattern::BlockPattern(const std::vector<std::vector<int>> &graph, int size,
     row.push_back(col.size());
   }
   nnzb = col.size();
+  for (int i = 0; i < nb; ++i) { max_deps = std::max({max_deps, diag[i] - row[i], row[i + 1] - diag[i] - 1}); }
   upd_ptr.assign(1, 0);
   for (int i = 0; i < nb; ++i) {
     for (int ik = row[i]; ik < row[i + 1]; ++ik) {
