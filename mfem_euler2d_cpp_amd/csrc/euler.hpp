@@ -176,11 +176,14 @@ public:
   DeviceArray<float> V32, w32, z32, comb32, corr32, res32, x32, b32, blocks32;
   std::unique_ptr<BlockPattern> pattern;
   DeviceArray<int> dslot, slot, contrib;
-  // Dataflow ILU: per-row completion flags and epochs of the single-launch
-  // sweeps / factor (EG_ILU_LEVELS=1 forces one launch per level instead).
-  DeviceArray<int> done_sweep, done_factor;
-  int epoch_sweep = 0, epoch_factor = 0;
-  bool dataflow = true;
+  // Dataflow ILU schedules (EG_ILU_SCHEDULE): the factor's per-row
+  // completion flags and epoch, and the sweeps' intermediate vector, kept
+  // armed with the sentinel between applications.
+  DeviceArray<int> done_factor;
+  int epoch_factor = 0;
+  DeviceArray<double> sweep;
+  DeviceArray<float> sweep32;
+  bool dataflow = true, dataflow_factor = false;
   std::vector<double> h, cosine, sine, rotated, coefficients;
 
   LinearSolver(Problem &p, int form, Hybrid *hybrid = nullptr);

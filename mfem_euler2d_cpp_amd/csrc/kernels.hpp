@@ -75,10 +75,13 @@ void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, cons
 // Single-launch dataflow forms (cooperative grid, per-row completion flags);
 // return false when a co-resident launch is not possible (caller falls back
 // to the per-level launches).
+// Sweeps: src/dst/arm as documented at k_ilu_sweep_sn (lower: src = rhs,
+// dst = intermediate, arm = final output; upper: src = intermediate, dst =
+// final output). The intermediate must be armed before the first lower sweep.
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
-                         const int *diag_ptr, const int *col, const T *rhs, T *w, int *done, int epoch, int *flags,
-                         hipStream_t s);
+                         const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, hipStream_t s);
+template <class T> void launch_arm(T *v, size_t n, hipStream_t s);
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
                           const int *upd_ptr, const int *upd, int *done, int epoch, int *flags, hipStream_t s);
 

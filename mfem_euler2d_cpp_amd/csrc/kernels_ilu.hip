@@ -34,33 +34,74 @@ __device__ __forceinline__ void complete_memory_ops() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// Gauss-Jordan thread layout for a B x B block: column c is owned by R
+// consecutive lanes of one wavefront (R a power of two <= 16, so the group
+// is one aligned DPP row or a part of one), lane r holding rows r, r + R, ...
 template <int B> struct Shape {
-  static constexpr int R = std::min((B >= 64 ? 1024 : 256) / B, B); // row groups
-  static constexpr int NT = (B * R + 63) / 64 * 64;                  // threads per workgroup
-  static constexpr int NE = (B + R - 1) / R;                         // register entries per thread
+  static_assert(B >= 1 && B <= 127, "pivot keys hold the row index in 7 bits");
+  static constexpr int R = B <= 4 ? 4 : (B <= 8 ? 8 : (16 * B <= 1024 ? 16 : 8)); // lanes per column
+  static constexpr int NT = (B * R + 63) / 64 * 64;                                // threads per workgroup
+  static constexpr int NE = (B + R - 1) / R;                                       // register entries per thread
+  static_assert(NE <= 32, "one bit per register entry");
 };
-// LDS scratch of the Gauss-Jordan (doubles): column and pivot row (double
-// buffered), R candidates (double buffered), then ints.
-template <int B> constexpr int gj_lds() { return 4 * B + 2 * Shape<B>::R + (2 * Shape<B>::R + 2 * B + 1) / 2 + 1; }
+template <int B> struct GjLds {
+  double col[2][B];              // pivot column, double buffered
+  double pval[2];                // pivot value
+  unsigned long long key[2];     // winning pivot key
+  double scale[B];               // row scale: true row = scale * stored row
+  int piv[B], pinv[B];           // step -> pivot row, pivot row -> step
+};
+template <int B> constexpr int gj_lds() { return int((sizeof(GjLds<B>) + 7) / 8); } // in doubles
+
+// Max of a 64-bit key over an aligned group of R lanes (R = 4, 8 or 16)
+// with DPP row permutations: xor 1, xor 2, half-row mirror, row mirror.
+template <int CTRL> __device__ __forceinline__ unsigned long long dpp_max(unsigned long long k) {
+  const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(k)), CTRL, 0xF, 0xF, false));
+  const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(k >> 32)), CTRL, 0xF, 0xF, false));
+  const unsigned long long o = (static_cast<unsigned long long>(hi) << 32) | lo;
+  return o > k ? o : k;
+}
+template <int R> __device__ __forceinline__ unsigned long long group_max(unsigned long long k) {
+  k = dpp_max<0xB1>(k); // quad_perm [1,0,3,2]
+  k = dpp_max<0x4E>(k); // quad_perm [2,3,0,1]
+  if (R >= 8) { k = dpp_max<0x141>(k); }
+  if (R >= 16) { k = dpp_max<0x140>(k); }
+  return k;
+}
+// Double from lane `src` of the wavefront.
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = unsigned(__builtin_amdgcn_ds_bpermute(src * 4, int(unsigned(u))));
+  const unsigned hi = unsigned(__builtin_amdgcn_ds_bpermute(src * 4, int(unsigned(u >> 32))));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
 
 // In-place inverse of the B x B column-major block A by one workgroup of
-// Shape<B>::NT threads. Thread (tc, tr) = (tid % B, tid / B) owns column tc
-// at rows tr, tr + R, ... in registers. Column k is eliminated with pivot
-// row p = the first unused row of maximal |S(p,k)| (NaN counts as 0 so the
-// index stays in range); the stored matrix S then satisfies
-// A^{-1}(pinv[i], piv[j]) = S(i,j). The owners of column k publish it with
-// one pivot candidate per row group, so the choice costs R LDS reads.
+// Shape<B>::NT threads; thread (tc, tr) = (tid / R, tid % R) keeps column tc
+// at rows tr + e R in registers. Step k eliminates column k with pivot row
+// p = an unused row of maximal |S(p,k)|: the column's lanes reduce the key
+// (|S(i,k)| bits with the low 7 replaced by 127 - i, used rows 0) in
+// registers, so near-ties go to the lowest row and p is always an unused row
+// (a NaN ranks high and is reported through its pivot value). The pivot
+// row's value for each column comes from a lane of the same group
+// (ds_bpermute), so a step needs one barrier.
+// Rows are stored unscaled: the pivot row keeps its values and records
+// scale[p] = 1 / S(p,k), and the broadcast column holds 0 at row p, so every
+// entry takes the same update S(i,j) -= S(i,k) S(p,j) / S(p,k); only the
+// column k itself is replaced (-S(i,k) / S(p,k), and 1 at row p). With the
+// scales applied at the end, A^{-1}(pinv[i], piv[j]) = scale[i] S(i,j).
 // Mathematically the inverse of the block whose LU the reference uses
 // (src/linear.cpp:198-202, src/block_ilu.cpp:83-84, src/condensation.cpp:137-140).
+// Returns true on a zero or non-finite pivot.
 template <int B, bool COH>
 __device__ __forceinline__ bool gj_inverse(double *__restrict__ A, double *lds) {
   constexpr int R = Shape<B>::R, NE = Shape<B>::NE;
-  double *colk = lds, *rowp = lds + 2 * B, *cbest = lds + 4 * B;
-  int *carg = reinterpret_cast<int *>(cbest + 2 * R), *piv = carg + 2 * R, *pinv = piv + B;
-  const int tid = threadIdx.x, tc = tid % B, tr = tid / B;
-  const bool active = tr < R;
+  GjLds<B> &L = *reinterpret_cast<GjLds<B> *>(lds);
+  const int tid = threadIdx.x, tc = tid / R, tr = tid % R;
+  const bool active = tc < B;
+  const int group = (tid & 63) & ~(R - 1); // first lane of this column's group
   double v[NE];
-  unsigned long long used = 0ull;
+  unsigned used = 0u; // bit e: row tr + e R is a pivot row already
 #pragma unroll
   for (int e = 0; e < NE; ++e) {
     const int i = tr + e * R;
@@ -69,58 +110,58 @@ __device__ __forceinline__ bool gj_inverse(double *__restrict__ A, double *lds) 
   __syncthreads();
   bool singular = false;
   for (int k = 0; k < B; ++k) {
-    double *ck = colk + (k & 1) * B, *rp = rowp + (k & 1) * B, *cb = cbest + (k & 1) * R;
-    int *ca = carg + (k & 1) * R;
-    if (active && tc == k) {
-      double best = -1.;
-      int arg = B;
+    const int buf = k & 1;
+    double *ck = L.col[buf];
+    if (tc == k) {
+      unsigned long long key = 0ull;
+      double bv = 0.;
 #pragma unroll
       for (int e = 0; e < NE; ++e) {
         const int i = tr + e * R;
         if (i < B) {
           ck[i] = v[e];
-          const double a = isnan(v[e]) ? 0. : fabs(v[e]);
-          if (!((used >> e) & 1ull) && a > best) { best = a; arg = i; }
+          const unsigned long long kk =
+              (static_cast<unsigned long long>(__double_as_longlong(v[e])) & 0x7FFFFFFFFFFFFF80ull) |
+              unsigned(127 - i);
+          if (!((used >> e) & 1u) && kk > key) { key = kk; bv = v[e]; }
         }
       }
-      cb[tr] = best;
-      ca[tr] = arg;
-    }
-    __syncthreads();
-    double best = -1.;
-    int p = B;
-#pragma unroll 4
-    for (int g = 0; g < R; ++g) {
-      const double ob = cb[g];
-      const int oa = ca[g];
-      if (ob > best || (ob == best && oa < p)) { best = ob; p = oa; }
-    }
-    const double ap = ck[p];
-    if (!(fabs(ap) > 0.) || !isfinite(ap)) { singular = true; }
-    const double inv = 1. / ap;
-    const int ep = p / R;
-    if (active && tr == p - ep * R) {
-      used |= 1ull << ep;
-      double sel = 0.;
-#pragma unroll
-      for (int e = 0; e < NE; ++e) { sel = e == ep ? v[e] : sel; }
-      rp[tc] = sel;
-    }
-    __syncthreads();
-    if (tid == 0) { piv[k] = p; pinv[p] = k; }
-    if (active) {
-      const double rpj = rp[tc] * inv;
-      const bool is_col = tc == k;
-#pragma unroll
-      for (int e = 0; e < NE; ++e) {
-        const int i = tr + e * R;
-        const double c = ck[i < B ? i : 0];
-        const bool is_row = i == p;
-        double nv = v[e] - c * rpj;
-        nv = is_row ? v[e] * inv : nv;
-        nv = is_col ? (is_row ? inv : -c * inv) : nv;
-        v[e] = nv;
+      const unsigned long long best = group_max<R>(key);
+      if (key == best) {
+        L.key[buf] = best;
+        L.pval[buf] = bv;
+        ck[127 - int(best & 127ull)] = 0.;
       }
+    }
+    __syncthreads();
+    const int p = 127 - int(L.key[buf] & 127ull);
+    const double ap = L.pval[buf];
+    if (!(fabs(ap) > 0.) || !isfinite(ap)) { singular = true; }
+    const int ep = p / R, lp = p % R;
+    const bool mine = tr == lp;
+    double sel = 0.;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) { sel = e == ep ? v[e] : sel; }
+    if (mine) { used |= 1u << ep; }
+    const double rpv = lane_value(sel, group + lp);
+    const double inv = 1. / ap;
+    if (tid == 0) {
+      L.piv[k] = p;
+      L.pinv[p] = k;
+      L.scale[p] = inv;
+    }
+    const double rpj = rpv * inv;
+    double c[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int i = tr + e * R;
+      c[e] = ck[i < B ? i : B - 1];
+      v[e] = fma(-c[e], rpj, v[e]);
+    }
+    if (tc == k) {
+      asm volatile(""); // keep this a branch taken by one lane group only
+#pragma unroll
+      for (int e = 0; e < NE; ++e) { v[e] = (mine && e == ep) ? 1. : -c[e] * inv; }
     }
   }
   __syncthreads();
@@ -128,7 +169,7 @@ __device__ __forceinline__ bool gj_inverse(double *__restrict__ A, double *lds) 
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       const int i = tr + e * R;
-      if (i < B) { st<COH>(A + pinv[i] + piv[tc] * B, v[e]); }
+      if (i < B) { st<COH>(A + L.pinv[i] + L.piv[tc] * B, v[e] * L.scale[i]); }
     }
   }
   __syncthreads();
@@ -304,13 +345,15 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_upper_split(const T *__re
 // One launch per sweep (or per factorisation) instead of one per level: the
 // grid is co-resident (cooperative launch) and each workgroup handles its
 // rows in dependency order (increasing for the lower sweep and the factor,
-// decreasing for the upper sweep). Before a row it waits until every row it
-// depends on has published done[j] >= epoch; after the row it completes its
-// stores and publishes done[i] = epoch. As the rows waited on precede the
-// waiting row in that order and all workgroups are resident, every wait
-// ends; each wait is nevertheless bounded by spin_limit and then raises
-// kErrSpin instead of hanging. Data crossing workgroups is read and written
-// with coherent (sc1) accesses only.
+// decreasing for the upper sweep). As the rows waited on precede the waiting
+// row in that order and all workgroups are resident, every wait ends; each
+// wait is nevertheless bounded by spin_limit and then raises kErrSpin
+// instead of hanging. Data crossing workgroups is read and written with
+// coherent (sc1) accesses only.
+//
+// Factor: before a row the workgroup waits until every row it depends on has
+// published done[j] >= epoch; after the row it completes its stores and
+// publishes done[i] = epoch.
 __device__ __forceinline__ void wait_rows(const int *__restrict__ col, int j0, int j1, const int *done, int epoch,
                                           int spin_limit, int *flags) {
   for (int ij = j0 + int(threadIdx.x); ij < j1; ij += blockDim.x) {
@@ -331,58 +374,134 @@ __device__ __forceinline__ void publish_row(int *done, int i, int epoch) {
   __syncthreads();
   if (threadIdx.x == 0) { st_coherent(done + i, epoch); }
 }
-// Gather the dependency vectors of row i (coherently, one element per thread)
-// into LDS: xs[(ij - j0) * b + c] = w[col[ij] * b + c].
-template <class T>
-__device__ __forceinline__ void gather_deps(const T *w, const int *__restrict__ col, int j0, int j1, int b, T *xs) {
-  const int total = (j1 - j0) * b;
-  for (int t = threadIdx.x; t < total; t += blockDim.x) {
-    const int d = t / b, c = t % b;
-    xs[t] = ld_coherent(w + size_t(col[j0 + d]) * b + c);
-  }
-  __syncthreads();
+
+// Sweeps: the output vector is armed with a signalling-NaN sentinel before
+// the sweep (arithmetic never produces a signalling NaN), and a row polls the
+// values of the rows it depends on until none is the sentinel any more: the
+// data is its own completion flag, one coherent round trip per dependency.
+// The row's matrix blocks are loaded into registers before the wait.
+template <class T> struct Sentinel;
+template <> struct Sentinel<double> {
+  using U = unsigned long long;
+  static constexpr U bits = 0x7FF4DEADBEEF0001ull;
+};
+template <> struct Sentinel<float> {
+  using U = unsigned;
+  static constexpr U bits = 0x7FA0DEADu;
+};
+template <class T> __device__ __forceinline__ T sentinel() {
+  return __builtin_bit_cast(T, Sentinel<T>::bits);
 }
-constexpr int kMaxDeps = 8;
-template <class T, bool LOWER>
-__global__ __launch_bounds__(kSweepThreads) void k_ilu_sweep_df(const T *__restrict__ bsr, int b, int nb,
-                                                                const int *__restrict__ row_ptr,
-                                                                const int *__restrict__ diag_ptr,
-                                                                const int *__restrict__ col, const T *__restrict__ rhs,
-                                                                T *w, int *done, int epoch, int *flags, int spin_limit) {
-  __shared__ T red[kSweepThreads];
-  __shared__ T ts[kSweepThreads];
-  __shared__ T xs[kMaxDeps * kSweepThreads];
-  const size_t bb = size_t(b) * b;
-  const int P = blockDim.x / b, r = threadIdx.x % b, part = threadIdx.x / b;
+template <class T> __device__ __forceinline__ bool armed(T v) {
+  return __builtin_bit_cast(typename Sentinel<T>::U, v) == Sentinel<T>::bits;
+}
+// Threads per sweep workgroup and the split of a block row over them:
+// thread (r, part) = (tid % B, tid / B) holds columns part + v P of row r of
+// every block of its row (VB values per block).
+template <int B> struct SweepShape {
+  static constexpr int NT = B > 64 ? 1024 : 256;
+  static constexpr int P = NT / B;
+  static constexpr int VB = (B + P - 1) / P;
+};
+// LOWER: w_i = b_i - sum_{j<i} L_ij w_j (src/block_ilu.cpp:87-99); src = b,
+// dst = w, and the kernel arms `arm` (the upper sweep's output).
+// UPPER: y_i = inv(U_ii) (w_i - sum_{j>i} U_ij y_j) (src/block_ilu.cpp:100-108);
+// src = w, re-armed row by row after it is read, dst = y.
+template <class T, int B, int KD, bool LOWER>
+__global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__restrict__ bsr, int nb,
+                                                                    const int *__restrict__ row_ptr,
+                                                                    const int *__restrict__ diag_ptr,
+                                                                    const int *__restrict__ col, T *src, T *dst,
+                                                                    T *arm, int *flags, int spin_limit) {
+  constexpr int NT = SweepShape<B>::NT, P = SweepShape<B>::P, VB = SweepShape<B>::VB;
+  constexpr int KB = LOWER ? KD : KD + 1; // blocks held per row (upper: + inv(U_ii))
+  constexpr size_t bb = size_t(B) * B;
+  __shared__ T xs[KD * B];
+  __shared__ T red[NT];
+  __shared__ T ts[B];
+  const int tid = threadIdx.x, r = tid % B, part = tid / B;
+  const bool worker = part < P;
+  if (LOWER) {
+    for (size_t t = size_t(blockIdx.x) * NT + tid; t < size_t(nb) * B; t += size_t(gridDim.x) * NT) {
+      arm[t] = sentinel<T>();
+    }
+  }
   for (int idx = blockIdx.x; idx < nb; idx += gridDim.x) {
     const int i = LOWER ? idx : nb - 1 - idx;
     const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
-    wait_rows(col, j0, j1, done, epoch, spin_limit, flags);
-    gather_deps(w, col, j0, j1, b, xs);
-    T acc = T(0);
-    if (part < P) {
-      for (int ij = j0; ij < j1; ++ij) { acc += row_dot(bsr + ij * bb, xs + (ij - j0) * b, b, r, part, P); }
+    const int nd = j1 - j0;
+    T a[KB][VB];
+#pragma unroll
+    for (int d = 0; d < KB; ++d) {
+      const int slot = (!LOWER && d == KD) ? diag_ptr[i] : j0 + d;
+      const bool use = (!LOWER && d == KD) || d < nd;
+#pragma unroll
+      for (int v = 0; v < VB; ++v) {
+        const int c = part + v * P;
+        a[d][v] = (use && worker && c < B) ? bsr[size_t(slot) * bb + size_t(c) * B + r] : T(0);
+      }
     }
-    red[threadIdx.x] = acc;
+    T own = T(0);
+    if (part == 0) {
+      own = src[size_t(i) * B + r];
+      if (!LOWER) { src[size_t(i) * B + r] = sentinel<T>(); }
+    }
+    for (int t = tid; t < nd * B; t += NT) {
+      const T *p = dst + size_t(col[j0 + t / B]) * B + t % B;
+      T x = ld_coherent(p);
+      int spins = 0;
+      while (armed(x)) {
+        if (++spins > spin_limit) {
+          atomicOr(flags, kErrSpin);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        x = ld_coherent(p);
+      }
+      xs[t] = x;
+    }
+    __syncthreads();
+    T acc = T(0);
+#pragma unroll
+    for (int d = 0; d < KD; ++d) {
+      if (d < nd) {
+#pragma unroll
+        for (int v = 0; v < VB; ++v) {
+          const int c = part + v * P;
+          if (c < B) { acc += a[d][v] * xs[d * B + c]; }
+        }
+      }
+    }
+    red[tid] = acc;
     __syncthreads();
     if (part == 0) {
-      T s = T(0);
-      for (int q = 0; q < P; ++q) { s += red[q * b + r]; }
-      if (LOWER) { st_coherent(w + size_t(i) * b + r, rhs[size_t(i) * b + r] - s); }
-      else { ts[r] = w[size_t(i) * b + r] - s; }
+      T sum = T(0);
+      for (int q = 0; q < P; ++q) { sum += red[q * B + r]; }
+      if (LOWER) { st_coherent(dst + size_t(i) * B + r, own - sum); }
+      else { ts[r] = own - sum; }
     }
     if (!LOWER) {
       __syncthreads();
-      acc = part < P ? row_dot(bsr + diag_ptr[i] * bb, ts, b, r, part, P) : T(0);
-      red[threadIdx.x] = acc;
+      acc = T(0);
+#pragma unroll
+      for (int v = 0; v < VB; ++v) {
+        const int c = part + v * P;
+        if (c < B) { acc += a[KD][v] * ts[c]; }
+      }
+      red[tid] = acc;
       __syncthreads();
       if (part == 0) {
-        T s = T(0);
-        for (int q = 0; q < P; ++q) { s += red[q * b + r]; }
-        st_coherent(w + size_t(i) * b + r, s);
+        T sum = T(0);
+        for (int q = 0; q < P; ++q) { sum += red[q * B + r]; }
+        st_coherent(dst + size_t(i) * B + r, sum);
       }
     }
-    publish_row(done, i, epoch);
+    __syncthreads();
+  }
+}
+template <class T> __global__ void k_arm(T *v, size_t n) {
+  for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += size_t(gridDim.x) * blockDim.x) {
+    v[t] = sentinel<T>();
   }
 }
 // Dataflow ILU(0) factor: factor_row per row, rows in increasing order per
@@ -425,6 +544,19 @@ void allow_lds(const void *fn) {
 constexpr int kSpinLimit = 1 << 20;
 } // namespace
 
+#define EG_DISPATCH_B_SMALL(b, CALL)                                                               \
+  switch (b) {                                                                                     \
+  case 4: { constexpr int B = 4; CALL; } break;                                                    \
+  case 8: { constexpr int B = 8; CALL; } break;                                                    \
+  case 12: { constexpr int B = 12; CALL; } break;                                                  \
+  case 16: { constexpr int B = 16; CALL; } break;                                                  \
+  case 20: { constexpr int B = 20; CALL; } break;                                                  \
+  case 24: { constexpr int B = 24; CALL; } break;                                                  \
+  case 28: { constexpr int B = 28; CALL; } break;                                                  \
+  case 32: { constexpr int B = 32; CALL; } break;                                                  \
+  case 36: { constexpr int B = 36; CALL; } break;                                                  \
+  default: break;                                                                                  \
+  }
 #define EG_DISPATCH_B(b, CALL)                                                                     \
   switch (b) {                                                                                     \
   case 4: { constexpr int B = 4; CALL; } break;                                                    \
@@ -473,21 +605,32 @@ void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, cons
 }
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
-                         const int *diag_ptr, const int *col, const T *rhs, T *w, int *done, int epoch, int *flags,
-                         hipStream_t s) {
-  if (b > kSweepThreads || max_deps * b > kMaxDeps * kSweepThreads) { return false; }
-  const void *fn = lower ? reinterpret_cast<const void *>(k_ilu_sweep_df<T, true>)
-                         : reinterpret_cast<const void *>(k_ilu_sweep_df<T, false>);
-  const int cap = coresident(fn, kSweepThreads, 0);
-  if (cap <= 0) { return false; }
-  const int grid = std::min(nb, cap);
-  int spin = kSpinLimit;
-  void *args[] = {&bsr, &b, &nb, &row_ptr, &diag_ptr, &col, &rhs, &w, &done, &epoch, &flags, &spin};
-  if (hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kSweepThreads), args, 0, s) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
-  return true;
+                         const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, hipStream_t s) {
+  if (max_deps > 8) { return false; }
+  const int kd = max_deps <= 4 ? 4 : 8;
+  bool ok = false, known = false;
+  auto go = [&](const void *fn, int threads) {
+    known = true;
+    const int cap = coresident(fn, threads, 0);
+    if (cap <= 0) { return; }
+    const int grid = std::min(nb, cap);
+    int spin = kSpinLimit;
+    void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin};
+    ok = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s) == hipSuccess;
+    if (!ok) { (void)hipGetLastError(); }
+  };
+#define EG_SWEEP(DISPATCH, KD)                                                                     \
+  DISPATCH(b, (go(lower ? reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, true>)           \
+                        : reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, false>),         \
+                  SweepShape<B>::NT)))
+  // Cell blocks (DG, at most 4 lower or upper neighbours) or face blocks
+  // (HDG, at most 7).
+  if (kd == 4) { EG_SWEEP(EG_DISPATCH_B, 4) } else { EG_SWEEP(EG_DISPATCH_B_SMALL, 8) }
+#undef EG_SWEEP
+  return known && ok;
+}
+template <class T> void launch_arm(T *v, size_t n, hipStream_t s) {
+  k_arm<T><<<int(std::min<size_t>((n + 255) / 256, 4096)), 256, 0, s>>>(v, n);
 }
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
                           const int *upd_ptr, const int *upd, int *done, int epoch, int *flags, hipStream_t s) {
@@ -517,7 +660,9 @@ template void launch_ilu_upper_fast<double>(const double *, int, const int *, in
 template void launch_ilu_upper_fast<float>(const float *, int, const int *, int, const int *, const int *,
                                            const int *, float *, hipStream_t);
 template bool launch_ilu_sweep_df<double>(bool, const double *, int, int, int, const int *, const int *, const int *,
-                                          const double *, double *, int *, int, int *, hipStream_t);
+                                          double *, double *, double *, int *, hipStream_t);
 template bool launch_ilu_sweep_df<float>(bool, const float *, int, int, int, const int *, const int *, const int *,
-                                         const float *, float *, int *, int, int *, hipStream_t);
+                                         float *, float *, float *, int *, hipStream_t);
+template void launch_arm<double>(double *, size_t, hipStream_t);
+template void launch_arm<float>(float *, size_t, hipStream_t);
 } // namespace eg

@@ -38,12 +38,18 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
   if (ilu) {
     pattern = std::make_unique<BlockPattern>(p.SolverGraph(form == 1), block, p.stream);
     blocks.resize(size_t(pattern->nnzb) * block * block);
-    const char *lv = std::getenv("EG_ILU_LEVELS");
-    dataflow = !(lv && lv[0] == '1');
-    done_sweep.resize(pattern->nb);
+    // EG_ILU_SCHEDULE: "levels" (one launch per dependency level for the
+    // factor and the sweeps), "dataflow" (default: single-launch sweeps,
+    // per-level factor) or "dataflow-all" (single-launch factor as well).
+    const char *sched = std::getenv("EG_ILU_SCHEDULE");
+    const std::string mode = sched ? sched : "dataflow";
+    require(mode == "levels" || mode == "dataflow" || mode == "dataflow-all", "Invalid EG_ILU_SCHEDULE");
+    dataflow = mode != "levels";
+    dataflow_factor = mode == "dataflow-all";
     done_factor.resize(pattern->nb);
-    hip_check(hipMemsetAsync(done_sweep, 0, sizeof(int) * pattern->nb, p.stream), "memset");
     hip_check(hipMemsetAsync(done_factor, 0, sizeof(int) * pattern->nb, p.stream), "memset");
+    sweep.resize(N);
+    if (mixed) { sweep32.resize(N); }
     if (form == 0) {
       std::vector<int> ds(p.n), sl(4 * p.n);
       for (int e = 0; e < p.n; ++e) {
@@ -116,7 +122,7 @@ void LinearSolver::Setup() {
   }
   if (ilu) {
     const auto &pt = *pattern;
-    const bool df = dataflow && eg::launch_ilu_factor_df(blocks, b, pt.nb, pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr,
+    const bool df = dataflow_factor && eg::launch_ilu_factor_df(blocks, b, pt.nb, pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr,
                                                          pt.d_upd, done_factor, ++epoch_factor, p.flags, p.stream);
     if (!df) {
       for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
@@ -129,6 +135,10 @@ void LinearSolver::Setup() {
     eg::launch_block_inverse(blocks, b, N / b, nullptr, p.flags, p.stream);
   }
   if (mixed) { eg::launch_convert<double, float>(blocks, blocks32, blocks.n, p.stream); }
+  if (ilu && dataflow) {
+    eg::launch_arm<double>(sweep, N, p.stream);
+    if (mixed) { eg::launch_arm<float>(sweep32, N, p.stream); }
+  }
   p.CheckFlags(ilu ? "Singular BSR ILU diagonal" : "Singular block Jacobi factor");
 }
 
@@ -147,12 +157,12 @@ template <class T> static void precondition(LinearSolver &s, const T *bsr, const
   }
   const auto &pt = *s.pattern;
   if (s.dataflow) {
-    const int e0 = ++s.epoch_sweep;
-    if (eg::launch_ilu_sweep_df<T>(true, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col, b, y, s.done_sweep, e0,
-                                   s.p.flags, s.p.stream)) {
-      const int e1 = ++s.epoch_sweep;
-      if (!eg::launch_ilu_sweep_df<T>(false, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col, b, y, s.done_sweep,
-                                      e1, s.p.flags, s.p.stream)) {
+    require(b != y, "ILU apply needs distinct input and output vectors");
+    T *mid = sizeof(T) == 8 ? reinterpret_cast<T *>(s.sweep.p) : reinterpret_cast<T *>(s.sweep32.p);
+    if (eg::launch_ilu_sweep_df<T>(true, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col,
+                                   const_cast<T *>(b), mid, y, s.p.flags, s.p.stream)) {
+      if (!eg::launch_ilu_sweep_df<T>(false, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col, mid,
+                                      y, y, s.p.flags, s.p.stream)) {
         throw std::runtime_error("Co-resident launch of the upper ILU sweep failed");
       }
       return;

@@ -293,12 +293,13 @@ def test_cli_backward_euler_update_check(gpu, tmp_path):
 
 @pytest.mark.parametrize("args", ["--nx 16 --ny 16 --k 3", "--nx 5 --ny 3 --k 2", "--nx 4 --ny 6 --k 0"])
 def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
-    """The single-launch dataflow ILU(0) factor and sweeps (per-row completion
-    flags) give the preconditioner of the one-launch-per-level schedule."""
+    """The single-launch dataflow ILU(0) sweeps (sentinel-armed dependency
+    polling) and factor (per-row completion flags) give the preconditioner of
+    the one-launch-per-level schedule."""
     import torch
     out = {}
-    for mode in ("0", "1"):
-        monkeypatch.setenv("EG_ILU_LEVELS", mode)
+    for mode in ("levels", "dataflow", "dataflow-all"):
+        monkeypatch.setenv("EG_ILU_SCHEDULE", mode)
         o, c, u, du, *_ = hdg_case(gpu, oracle_lib, args + " --precond block-ilu0")
         res = []
         for form, size in ((1, o.nl), (0, o.nu)):
@@ -312,5 +313,6 @@ def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
             res.append(y.cpu().numpy())
             assert rel(res[-1], o.precondition(form, 1, b)) < 1e-10
         out[mode] = res
-    for a, b in zip(out["0"], out["1"]):
-        assert rel(a, b) < 1e-13
+    for mode in ("dataflow", "dataflow-all"):
+        for a, b in zip(out["levels"], out[mode]):
+            assert rel(a, b) < 1e-13
