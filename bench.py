@@ -28,6 +28,91 @@ sys.path.insert(0, str(ROOT))
 
 WORKLOAD = "--nx 16 --ny 16 --k 3 --alpha-deg 45 --cfl 1 --steps 3 --newton-steps 3 --precond block-ilu0"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (matrix and vector)
+# Operations timed for the roofline entry; the setups and the condensation are
+# dense FP64 block work (flop-bound), the rest streams vectors (HBM-bound).
+KERNELS = ["pc_dg", "dg_apply", "pc_hdg", "schur_apply", "residual", "pc_setup_dg", "pc_setup_hdg", "hdg_condense"]
+FLOP_BOUND = {"pc_setup_dg", "pc_setup_hdg", "hdg_condense"}
+
+
+def counter_kib(counter: str, kernel: str, workload: str, names: tuple[str, ...]) -> float | None:
+    """Per-call KiB of one rocprofv3 counter (FETCH_SIZE or WRITE_SIZE) summed
+    over the dispatches of `kernel` run alone through --only-kernel, for the
+    dispatches whose names start with one of `names` (the operation's own
+    kernels). Returns None when the profiler is unavailable or fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None
+    reps = 5
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        cmd = ["timeout", "-s", "KILL", "90", "rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", tmp,
+               "-o", "p", "--", sys.executable, str(ROOT / "bench.py"), "--only-kernel", kernel, "--reps",
+               str(reps), "--workload", workload]
+        env = dict(os.environ, TMPDIR="/tmp")
+        r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        files = list(Path(tmp).rglob("*counter_collection.csv"))
+        if r.returncode != 0 or not files:
+            return None
+        total, calls = 0.0, 0
+        for row in csv.DictReader(open(files[0])):
+            if row["Counter_Name"] == counter and any(n in row["Kernel_Name"] for n in names):
+                total += float(row["Counter_Value"])
+        # the hook makes one warm call before the reps timed ones, and its set-up
+        # already calls the setups and the residual once; copy: reps calls
+        calls = reps if kernel == "copy" else reps + (2 if kernel.startswith("pc_setup") or kernel == "residual" else 1)
+        return total / calls
+
+
+# kernels (name prefixes as the profiler reports them) making up each operation
+OP_KERNELS = {
+    "pc_setup_dg": ("k_dg_blocks", "k_ilu_factor", "k_block_inverse", "k_arm", "k_convert"),
+    "pc_setup_hdg": ("k_hdg_fill", "k_hdg_diag_blocks", "k_ilu_factor", "k_block_inverse", "k_arm", "k_convert"),
+    "hdg_condense": ("k_hdg_build", "k_hdg_solve", "k_hdg_panels", "k_hdg_face", "k_block_inverse"),
+    "pc_dg": ("k_ilu_sweep", "k_ilu_lower", "k_ilu_upper", "k_block_gemv"),
+    "pc_hdg": ("k_ilu_sweep", "k_ilu_lower", "k_ilu_upper", "k_block_gemv"),
+    "dg_apply": ("k_dg_apply",), "schur_apply": ("k_schur_apply",), "residual": ("k_residual",),
+    "copy": ("copyBuffer",),
+}
+
+
+def measured_traffic(kernel: str, workload: str) -> dict | None:
+    """HBM bytes per call of `kernel` from FETCH_SIZE and WRITE_SIZE passes,
+    each scaled by what the counter reads for a device copy of known size."""
+    mib = 256.0
+    raw = {c: counter_kib(c, kernel, workload, OP_KERNELS[kernel]) for c in ("FETCH_SIZE", "WRITE_SIZE")}
+    cal = {c: counter_kib(c, "copy", workload, OP_KERNELS["copy"]) for c in ("FETCH_SIZE", "WRITE_SIZE")}
+    if any(v is None for v in raw.values()) or any(not v for v in cal.values()):
+        return None
+    scale = {c: mib * 1024 / cal[c] for c in cal}  # copy reads and writes mib MiB
+    read = raw["FETCH_SIZE"] * scale["FETCH_SIZE"] * 1024
+    write = raw["WRITE_SIZE"] * scale["WRITE_SIZE"] * 1024
+    return {"bytes": read + write, "read_bytes": read, "write_bytes": write,
+            "fetch_scale": round(scale["FETCH_SIZE"], 4), "write_scale": round(scale["WRITE_SIZE"], 4)}
+
+
+def only_kernel(args) -> int:
+    """Time one operation alone (for profiler counter passes): `copy` is a
+    device-to-device copy of --copy-mib MiB used to calibrate the HBM byte
+    counters; any other name goes through the library's timing hook."""
+    import torch
+    import mfem_euler2d_cpp_amd as eg
+    if args.only_kernel == "copy":
+        n = int(args.copy_mib * (1 << 20)) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(args.reps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"kernel": "copy", "bytes_read": 8 * n, "bytes_written": 8 * n, "reps": args.reps}))
+        return 0
+    ctx = eg.Context(args.workload)
+    ms, nbytes, flops = ctx.kernel_time(args.only_kernel, reps=args.reps)
+    print(json.dumps({"kernel": args.only_kernel, "ms": ms, "algorithmic_bytes": nbytes, "flops": flops,
+                      "reps": args.reps}))
+    return 0
 
 
 def main() -> int:
@@ -38,6 +123,10 @@ def main() -> int:
     ap.add_argument("--kernel", default="auto", help="kernel for the roofline entry")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--workload", default=WORKLOAD)
+    ap.add_argument("--only-kernel", default=None, help="time one operation alone (profiling) and exit")
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--copy-mib", type=float, default=256.0)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: independent solves shard across GPUs without any exchange, "
@@ -51,6 +140,8 @@ def main() -> int:
     if not eg.LIB.exists():
         eg.build(force=False)
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.only_kernel:
+        return only_kernel(args)
     ctx = eg.Context(args.workload)
 
     # warm-up (also JIT/first-touch of every kernel)
@@ -81,28 +172,39 @@ def main() -> int:
         gpu_inc[form] = (u, inc[:st.updates])
         krylov[form] = kry[:st.updates]
 
-    # roofline of the dominant kernel (live HIP-event timing on the library stream)
-    kernels = ["pc_dg", "dg_apply", "schur_apply", "pc_hdg", "residual"]
-    timed = {}
-    for k in kernels:
-        ms, nbytes = ctx.kernel_time(k, reps=50)
-        timed[k] = (ms, nbytes)
-    # dominant = largest per-update time share: launches per update estimated
-    # from the Krylov counts (one PC + one apply per iteration)
+    # roofline of the dominant operation (live HIP-event timing on the library
+    # stream): the one with the largest time per accepted update, counting one
+    # preconditioner setup (and, for HDG, one condensation) per update and one
+    # preconditioner application and operator action per Krylov iteration
+    # (+2 per update for the restart and the true residual).
+    timed = {k: ctx.kernel_time(k, reps=20 if k in FLOP_BOUND else 50) for k in KERNELS}
     it_dg = float(np.mean(krylov["dg"])) + 2
     it_hdg = float(np.mean(krylov["hdg"])) + 2
-    share = {"pc_dg": timed["pc_dg"][0] * it_dg, "dg_apply": timed["dg_apply"][0] * it_dg,
-             "pc_hdg": timed["pc_hdg"][0] * it_hdg, "schur_apply": timed["schur_apply"][0] * it_hdg,
-             "residual": timed["residual"][0] * 2}
+    per_update = {"pc_dg": it_dg, "dg_apply": it_dg, "pc_hdg": it_hdg, "schur_apply": it_hdg, "residual": 2,
+                  "pc_setup_dg": 1, "pc_setup_hdg": 1, "hdg_condense": 1}
+    share = {k: timed[k][0] * per_update[k] for k in KERNELS}
     kname = max(share, key=share.get) if args.kernel == "auto" else args.kernel
     if kname not in timed:
         timed[kname] = ctx.kernel_time(kname, reps=50)
-    ms, nbytes = timed[kname]
-    achieved = nbytes / (ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "avg_launch_us": round(ms * 1e3, 3), "algorithmic_bytes": nbytes,
-                "all_kernels_us": {k: round(v[0] * 1e3, 3) for k, v in timed.items()}}
+    ms, nbytes, flops = timed[kname]
+    if kname in FLOP_BOUND:
+        achieved = flops / (ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "kernel": kname, "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 6), "traffic": None,
+                    "avg_launch_us": round(ms * 1e3, 3), "algorithmic_flops": flops}
+    else:
+        achieved = nbytes / (ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "avg_launch_us": round(ms * 1e3, 3), "algorithmic_bytes": nbytes}
+    if not args.no_traffic:
+        t = measured_traffic(kname, args.workload)
+        if t is not None:
+            roofline["traffic"] = t["bytes"]
+            roofline["traffic_detail"] = t
+    roofline["all_kernels"] = {k: {"us": round(v[0] * 1e3, 3), "us_per_update": round(share[k] * 1e3, 1),
+                                   "GB/s": round(v[1] / (v[0] * 1e-3) / 1e9, 2),
+                                   "TFLOP/s": round(v[2] / (v[0] * 1e-3) / 1e12, 4)} for k, v in timed.items()}
 
     cpu = None
     err = {}

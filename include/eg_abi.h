@@ -117,13 +117,15 @@ int eg_run(eg_ctx *ctx, int form, const char *prefix, double *host_u_final, eg_r
 
 /* Measurement hook (no counterpart in the reference, which only times whole
  * updates with steady_clock, include/euler.hpp:24-28): sets up the context's
- * problem at its initial state and times `reps` back-to-back launches of one
- * kernel with HIP events on the context's stream. kernel: "residual",
+ * problem at its initial state and times `reps` back-to-back calls of one
+ * operation with HIP events on the context's stream. kernel: "residual",
  * "dg_apply", "dg_apply_f32", "schur_apply", "pc_dg", "pc_hdg",
- * "hdg_condense", "pc_setup_dg", "pc_setup_hdg". avg_ms: mean duration per
- * launch; bytes: algorithmic HBM bytes of one launch (each array counted
- * once). */
-int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, double *bytes);
+ * "hdg_condense", "pc_setup_dg", "pc_setup_hdg" (the setups are the block
+ * fill plus the factor's launches). avg_ms: mean duration per call; bytes:
+ * algorithmic HBM bytes of one call (each array counted once); flops (may be
+ * NULL): algorithmic FP64 flops of one call for the dense block work
+ * (condensation and preconditioner setup, LU-based counts), else 0. */
+int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, double *bytes, double *flops);
 
 #ifdef __cplusplus
 }

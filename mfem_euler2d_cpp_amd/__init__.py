@@ -67,7 +67,7 @@ _SIGNATURES = {
     "eg_recovered_residual": (ctypes.c_int, [_vp, _vp, _vp, _c_double_p]),
     "eg_linear_solve": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(LinearStats)]),
-    "eg_kernel_time": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p]),
+    "eg_kernel_time": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p]),
     "eg_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, _c_double_p, ctypes.POINTER(RunStats),
                               _c_double_p, ctypes.c_int, _c_int_p]),
 }
@@ -214,9 +214,11 @@ class Context:
         return st
 
     def kernel_time(self, kernel, reps=50):
-        ms, nbytes = ctypes.c_double(), ctypes.c_double()
-        self.call("eg_kernel_time", kernel.encode(), reps, ctypes.byref(ms), ctypes.byref(nbytes))
-        return ms.value, nbytes.value
+        """(mean ms per call, algorithmic bytes, algorithmic FP64 flops) of one
+        operation timed with HIP events on the library stream."""
+        ms, nbytes, flops = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self.call("eg_kernel_time", kernel.encode(), reps, ctypes.byref(ms), ctypes.byref(nbytes), ctypes.byref(flops))
+        return ms.value, nbytes.value, flops.value
 
     def run(self, form, prefix=None, increments=0):
         import numpy as np

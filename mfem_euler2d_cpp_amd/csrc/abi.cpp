@@ -194,7 +194,7 @@ int eg_linear_solve(eg_ctx *ctx, int form, const double *dev_rhs, double *dev_x,
     }
   });
 }
-int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, double *bytes) {
+int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, double *bytes, double *flops) {
   return guard(ctx, [&] {
     auto &p = *ctx->problem;
     const std::string k(kernel);
@@ -212,7 +212,16 @@ int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, do
     ctx->state = u;
     const double nu = p.nu, nl = p.nl, nfq = double(p.nf) * p.q;
     std::function<void()> fn;
-    double b = 0.;
+    double b = 0., fl = 0.;
+    // LU-based count of the ILU(0) factor (src/block_ilu.cpp:73-86): 2 b^3
+    // per lower block (right solve) and per update product, 2/3 b^3 per
+    // diagonal LU; block Jacobi: 2/3 b^3 per block.
+    auto setup_flops = [](const LinearSolver &s) {
+      const double b3 = double(s.block) * s.block * s.block;
+      if (!s.pattern) { return 2. / 3. * b3 * (s.N / s.block); }
+      const auto &pt = *s.pattern;
+      return 2. * b3 * (double(pt.nnzb - pt.nb) / 2. + double(pt.upd.size() / 2)) + 2. / 3. * b3 * pt.nb;
+    };
     if (k == "residual") {
       fn = [&] { eg::launch_residual(p.geo, p.frozen, u, nullptr, 0., p.old, y, nullptr, p.partial, p.flags, p.stream); };
       b = 3 * nu * 8 + nfq * 5 * 8;
@@ -226,6 +235,10 @@ int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, do
     } else if (k == "hdg_condense") {
       fn = [&] { ctx->hyb().StreamCondense(u, r); };
       b = double(p.n) * p.m * (4 * p.t + 1) * 8 + 2 * nu * 8 + 2 * nl * 8 + double(p.nf) * 8 * p.t * p.t * 8;
+      // src/condensation.cpp:131-148: LU of A_e, X_e = A_e^{-1} B_e (4t
+      // columns) and y_e, then the panels C_e X_e (C_e: 4q nonzeros a row).
+      const double m = p.m, t = p.t;
+      fl = double(p.n) * (2. / 3. * m * m * m + 2. * m * m * (4. * t + 1.) + 2. * (4. * t) * (4. * t) * (4. * p.q));
     } else if (k == "schur_apply") {
       ctx->hyb().StreamCondense(u, r);
       hip_check(hipMemcpyAsync(tl, ctx->hyb().rhs, sizeof(double) * p.nl, hipMemcpyDeviceToDevice, p.stream), "copy");
@@ -237,6 +250,7 @@ int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, do
       if (k == "pc_dg") { fn = [&] { s.Precondition(x.p, y.p); }; }
       else { fn = [&] { s.Setup(); }; }
       b = double(s.blocks.n) * 8 + 2 * nu * 8;
+      if (k == "pc_setup_dg") { fl = setup_flops(s); }
     } else if (k == "pc_hdg" || k == "pc_setup_hdg") {
       ctx->hyb().StreamCondense(u, r);
       hip_check(hipMemcpyAsync(tl, ctx->hyb().rhs, sizeof(double) * p.nl, hipMemcpyDeviceToDevice, p.stream), "copy");
@@ -245,6 +259,7 @@ int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, do
       if (k == "pc_hdg") { fn = [&] { s.Precondition(tl.p, tl2.p); }; }
       else { fn = [&] { s.Setup(); }; }
       b = double(s.blocks.n) * 8 + 2 * nl * 8;
+      if (k == "pc_setup_hdg") { fl = setup_flops(s); }
     } else {
       throw std::runtime_error("Unknown kernel name: " + k);
     }
@@ -266,6 +281,7 @@ int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, do
     if (ctx->solver[0]) { ctx->solver[0]->Refresh(nullptr); }
     *avg_ms = double(ms) / reps;
     *bytes = b;
+    if (flops) { *flops = fl; }
   });
 }
 int eg_run(eg_ctx *ctx, int form, const char *prefix, double *host_u_final, eg_run_stats *stats,

@@ -18,7 +18,7 @@ RunResult Run(Problem &p, const std::string &form, const std::string &prefix,
   require(p.Admissible(out.u.data()), "Initial projection not positive; refine mesh");
   const auto initial = p.Integral(out.u.data());
   std::ofstream log;
-  if (!o.timing_only) {
+  if (!o.timing_only && !prefix.empty()) {
     log.open(prefix + "_iterations.csv");
     require(bool(log), "Cannot open iteration output");
     log << "form,matrix_free,step,stage,newton,time,stage_time,dt,cfl,"
@@ -221,7 +221,7 @@ RunResult Run(Problem &p, const std::string &form, const std::string &prefix,
   out.errors = p.Errors(out.u.data(), time);
   const auto final_integral = p.Integral(out.u.data());
   for (int c = 0; c < 4; ++c) { out.drift = std::max(out.drift, std::abs(final_integral[c] - initial[c])); }
-  if (o.save_solution) { p.Save(out.u.data(), prefix); }
+  if (o.save_solution && !prefix.empty()) { p.Save(out.u.data(), prefix); }
   if (!prefix.empty()) {
     std::ofstream summary(prefix + "_summary.csv");
     require(bool(summary), "Cannot open summary output");
