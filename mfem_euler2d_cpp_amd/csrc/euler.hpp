@@ -183,6 +183,7 @@ public:
   int epoch_factor = 0;
   DeviceArray<double> sweep;
   DeviceArray<float> sweep32;
+  DeviceArray<double> factor_scratch; // large blocks: one block per row of the widest level
   bool dataflow = true, dataflow_factor = false;
   std::vector<double> h, cosine, sine, rotated, coefficients;
 

@@ -13,9 +13,6 @@ enum : int {
   kErrNonfinite = 32,
   kErrSpin = 64,         // a dataflow dependency wait exceeded its spin bound
 };
-// Largest dense block held in LDS by the Gauss-Jordan inverse (b*b doubles
-// plus 3b scratch within the 160 KB gfx950 workgroup limit).
-constexpr int kMaxLdsBlock = 140;
 
 int cell_blocks(const Geo &g);
 int freeze_blocks(const Geo &g);
@@ -44,13 +41,17 @@ void launch_hdg_fill(const Geo &g, const double *panels, const double *D, const 
                      const int *diag_ptr, double *bsr, hipStream_t s);
 
 // Dense / block linear algebra.
-int la_init(); // nonzero if the large-LDS attribute could not be set
+int la_init(); // nonzero if the large-LDS attribute could not be set (clears the error state)
 void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s);
 template <class T>
 void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s);
+// scratch: nrows b x b blocks, used for large blocks (big_block(b)) only.
 void launch_ilu_factor_level(double *bsr, int b, const int *rows, int nrows, const int *row_ptr,
                              const int *diag_ptr, const int *col, const int *upd_ptr, const int *upd,
-                             int *flags, hipStream_t s);
+                             double *scratch, int *flags, hipStream_t s);
+// Blocks handled by the global-memory large-block kernels, and their limit.
+bool big_block(int b);
+int big_block_max();
 template <class T>
 void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
                             const int *diag_ptr, const int *col, const T *rhs, T *w, hipStream_t s);

@@ -1,84 +1,185 @@
-// Dense-block and Krylov kernels: batched partially pivoted Gauss-Jordan
-// inverses held in LDS, block GEMV for block-Jacobi, level-scheduled BSR
-// block ILU(0) factor and triangular sweeps, and the vector kernels of the
+// Dense-block and Krylov kernels: large-block (b > 100) Gauss-Jordan inverse
+// and ILU(0) factor in global memory, block GEMV for block-Jacobi, generic
+// level-scheduled BSR triangular sweeps, and the vector kernels of the
 // weighted restarted GMRES (fused CGS2 passes with deterministic two-stage
 // reductions).
 #include "kernels.hpp"
 
 namespace eg {
 namespace {
-constexpr int kLdsBytes = 163840;
+
+// ------------------------------------------------------------ large blocks
+// Blocks above 100 x 100 (cell blocks of k >= 5: 144, 196, 256, 324) do not
+// fit one CU's registers and LDS, so they are worked on in global memory (L2)
+// by one 1024-thread workgroup, with the pivot column and row staged in LDS.
+constexpr int kBigThreads = 1024;
+constexpr int kBigMax = 511; // pivot keys keep the row in 9 bits
+
+struct BigLds {
+  double col[kBigMax], row[kBigMax];
+  unsigned long long key[kBigThreads / 64];
+  int piv[kBigMax];
+};
+
+__device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
+__device__ __forceinline__ unsigned long long shfl_xor64(unsigned long long v, int mask) {
+  const int lo = __shfl_xor(int(unsigned(v)), mask, 64), hi = __shfl_xor(int(unsigned(v >> 32)), mask, 64);
+  return (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo);
+}
 
 // Partially pivoted in-place Gauss-Jordan inverse of one b x b column-major
-// block per workgroup (pivot: first row of maximal |a|, as LAPACK idamax).
-// Mathematically the inverse of the block whose LU the reference uses
-// (src/linear.cpp:198-202, src/block_ilu.cpp:83-84).
-__global__ __launch_bounds__(256) void k_block_inverse(double *__restrict__ blocks, int b,
-                                                       const int *__restrict__ index, int *__restrict__ flags) {
-  extern __shared__ double sm[];
-  double *A = sm, *colk = sm + b * b, *rowk = colk + b;
-  int *piv = reinterpret_cast<int *>(rowk + b);
-  __shared__ int s_piv;
-  const int blk = index ? index[blockIdx.x] : blockIdx.x;
-  double *gA = blocks + size_t(blk) * b * b;
-  const int nt = blockDim.x, tid = threadIdx.x;
-  for (int i = tid; i < b * b; i += nt) { A[i] = gA[i]; }
-  __syncthreads();
+// block in global memory (pivot: the row of maximal |a| among rows k..b-1,
+// near-ties to the lowest row; explicit row interchanges, undone on the
+// columns at the end). Mathematically the inverse of the block whose LU the
+// reference uses (src/linear.cpp:198-202, src/block_ilu.cpp:83-84,
+// src/condensation.cpp:137-140). Returns true on a zero or non-finite pivot.
+__device__ bool gj_global(double *__restrict__ A, int b, BigLds &L) {
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
   bool singular = false;
   for (int k = 0; k < b; ++k) {
-    if (tid < 64) {
-      double best = -1.;
-      int arg = b;
-      for (int r = k + tid; r < b; r += 64) {
-        const double a = A[r + k * b];
-        const double v = isnan(a) ? 0. : fabs(a); // keeps the pivot index in range
-        if (v > best) { best = v; arg = r; }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_down(best, off, 64);
-        const int oa = __shfl_down(arg, off, 64);
-        if (ob > best || (ob == best && oa < arg)) { best = ob; arg = oa; }
-      }
-      if (tid == 0) { s_piv = arg; piv[k] = arg; }
+    unsigned long long key = 0ull;
+    for (int i = k + tid; i < b; i += nt) {
+      const unsigned long long kk =
+          (static_cast<unsigned long long>(__double_as_longlong(A[i + size_t(k) * b])) & 0x7FFFFFFFFFFFFE00ull) |
+          unsigned(kBigMax - i);
+      key = umax64(key, kk);
     }
+    for (int m = 32; m > 0; m >>= 1) { key = umax64(key, shfl_xor64(key, m)); }
+    if (lane == 0) { L.key[wave] = key; }
     __syncthreads();
-    const int p = s_piv;
+    unsigned long long best = 0ull;
+    for (int w = 0; w < nt / 64; ++w) { best = umax64(best, L.key[w]); }
+    const int p = kBigMax - int(best & 511ull); // best > 0: rows k..b-1 all carry their index
+    if (tid == 0) { L.piv[k] = p; }
     if (p != k) {
       for (int j = tid; j < b; j += nt) {
-        const double t = A[k + j * b];
-        A[k + j * b] = A[p + j * b];
-        A[p + j * b] = t;
+        const double t = A[k + size_t(j) * b];
+        A[k + size_t(j) * b] = A[p + size_t(j) * b];
+        A[p + size_t(j) * b] = t;
       }
       __syncthreads();
     }
-    for (int i = tid; i < b; i += nt) { colk[i] = A[i + k * b]; rowk[i] = A[k + i * b]; }
+    for (int i = tid; i < b; i += nt) { L.col[i] = A[i + size_t(k) * b]; L.row[i] = A[k + size_t(i) * b]; }
     __syncthreads();
-    const double pivot = rowk[k];
+    const double pivot = L.row[k];
     if (!(fabs(pivot) > 0.) || !isfinite(pivot)) { singular = true; }
     const double inv = 1. / pivot;
     for (int idx = tid; idx < b * b; idx += nt) {
       const int i = idx % b, j = idx / b;
       double v;
-      if (i == k) { v = j == k ? inv : rowk[j] * inv; }
-      else if (j == k) { v = -colk[i] * inv; }
-      else { v = A[idx] - colk[i] * (rowk[j] * inv); }
+      if (i == k) { v = j == k ? inv : L.row[j] * inv; }
+      else if (j == k) { v = -L.col[i] * inv; }
+      else { v = A[idx] - L.col[i] * (L.row[j] * inv); }
       A[idx] = v;
     }
     __syncthreads();
   }
   for (int k = b - 1; k >= 0; --k) {
-    const int p = piv[k];
+    const int p = L.piv[k];
     if (p != k) {
       for (int i = tid; i < b; i += nt) {
-        const double t = A[i + k * b];
-        A[i + k * b] = A[i + p * b];
-        A[i + p * b] = t;
+        const double t = A[i + size_t(k) * b];
+        A[i + size_t(k) * b] = A[i + size_t(p) * b];
+        A[i + size_t(p) * b] = t;
       }
       __syncthreads();
     }
   }
-  for (int i = tid; i < b * b; i += nt) { gA[i] = A[i]; }
-  if (singular && tid == 0) { atomicOr(flags, kErrSingular); }
+  return singular;
+}
+
+__global__ __launch_bounds__(kBigThreads) void k_block_inverse_big(double *__restrict__ blocks, int b,
+                                                                   const int *__restrict__ index,
+                                                                   int *__restrict__ flags) {
+  __shared__ BigLds L;
+  const int blk = index ? index[blockIdx.x] : blockIdx.x;
+  if (gj_global(blocks + size_t(blk) * b * b, b, L) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
+}
+
+// C = X Y (mode 0) or C -= X Y (mode 1) for b x b column-major blocks in
+// global memory: four 256-thread groups each compute 64 x 64 tiles of C
+// (4 x 4 per thread), staging 64 x 16 panels of X and 16 x 64 panels of Y in
+// LDS. Every group runs the same number of barriers.
+constexpr int kTile = 64, kPanel = 16;
+struct ProductLds {
+  double x[4][kTile * kPanel], y[4][kPanel * kTile];
+};
+__device__ void tile_product_global(const double *__restrict__ X, const double *__restrict__ Y, double *C, int b,
+                                    int mode, ProductLds &S) {
+  const int tid = threadIdx.x, g = tid >> 8, lt = tid & 255, tr = lt % 16, tc = lt / 16;
+  const int nt = (b + kTile - 1) / kTile, tiles = nt * nt;
+  double *xs = S.x[g], *ys = S.y[g];
+  for (int t0 = 0; t0 < tiles; t0 += 4) {
+    const int tile = t0 + g;
+    const bool act = tile < tiles;
+    const int r0 = (tile % nt) * kTile, c0 = (tile / nt) * kTile;
+    double acc[4][4] = {};
+    for (int k0 = 0; k0 < b; k0 += kPanel) {
+      for (int e = lt; e < kTile * kPanel; e += 256) {
+        const int rr = e % kTile, kk = e / kTile;
+        const int r = r0 + rr, kx = k0 + kk;
+        xs[e] = (act && r < b && kx < b) ? X[r + size_t(kx) * b] : 0.;
+        const int ky = k0 + (e % kPanel), cc = e / kPanel, c = c0 + cc;
+        ys[e] = (act && ky < b && c < b) ? Y[ky + size_t(c) * b] : 0.;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < kPanel; ++kk) {
+        double a[4], y[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) { a[i] = xs[tr * 4 + i + kk * kTile]; y[i] = ys[kk + (tc * 4 + i) * kPanel]; }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) { acc[j][i] += a[i] * y[j]; }
+        }
+      }
+      __syncthreads();
+    }
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = r0 + tr * 4 + i, c = c0 + tc * 4 + j;
+          if (r < b && c < b) {
+            double *cp = C + r + size_t(c) * b;
+            *cp = mode == 0 ? acc[j][i] : *cp - acc[j][i];
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// One BSR row of ILU(0) (src/block_ilu.cpp:73-86) per workgroup for large
+// blocks, rows of one dependency level in parallel: L_ik = A_ik inv(A_kk)
+// through the workgroup's scratch block, A_ij -= L_ik A_kj for the cached
+// update pairs, then the updated diagonal block is replaced by its inverse.
+__global__ __launch_bounds__(kBigThreads) void k_ilu_factor_big(double *__restrict__ bsr, int b,
+                                                                const int *__restrict__ rows,
+                                                                const int *__restrict__ row_ptr,
+                                                                const int *__restrict__ diag_ptr,
+                                                                const int *__restrict__ col,
+                                                                const int *__restrict__ upd_ptr,
+                                                                const int *__restrict__ upd, double *scratch,
+                                                                int *__restrict__ flags) {
+  __shared__ ProductLds S;
+  __shared__ BigLds L;
+  const size_t bb = size_t(b) * b;
+  const int i = rows[blockIdx.x];
+  double *tmp = scratch + size_t(blockIdx.x) * bb;
+  for (int ik = row_ptr[i]; ik < diag_ptr[i]; ++ik) {
+    double *aik = bsr + size_t(ik) * bb;
+    tile_product_global(aik, bsr + size_t(diag_ptr[col[ik]]) * bb, tmp, b, 0, S);
+    for (size_t t = threadIdx.x; t < bb; t += blockDim.x) { aik[t] = tmp[t]; }
+    __syncthreads();
+    for (int u = upd_ptr[ik]; u < upd_ptr[ik + 1]; ++u) {
+      tile_product_global(aik, bsr + size_t(upd[2 * u + 1]) * bb, bsr + size_t(upd[2 * u]) * bb, b, 1, S);
+    }
+  }
+  if (gj_global(bsr + size_t(diag_ptr[i]) * bb, b, L) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
 }
 
 // y_blk = A_blk x_blk for count independent b x b blocks; one wave per block.
@@ -101,109 +202,6 @@ __global__ __launch_bounds__(256) void k_block_gemv(const T *__restrict__ blocks
       y[size_t(blk) * b + r] = s;
     }
   }
-}
-
-// One BSR row of ILU(0) (src/block_ilu.cpp:73-86) per workgroup, rows of one
-// dependency level in parallel. Diagonal blocks are replaced by the inverse
-// of their updated values, so L_ik = A_ik * inv(A_kk) is a block product.
-__global__ __launch_bounds__(256) void k_ilu_factor(double *__restrict__ bsr, int b, const int *__restrict__ rows,
-                                                    const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
-                                                    const int *__restrict__ col, const int *__restrict__ upd_ptr,
-                                                    const int *__restrict__ upd, int *__restrict__ flags) {
-  extern __shared__ double sm[];
-  const int bb = b * b;
-  double *Ls = sm, *As = sm + bb;
-  const int i = rows[blockIdx.x];
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int ik = row_ptr[i]; ik < diag_ptr[i]; ++ik) {
-    const int k = col[ik];
-    double *aik = bsr + size_t(ik) * bb;
-    const double *dk = bsr + size_t(diag_ptr[k]) * bb;
-    for (int t = tid; t < bb; t += nt) { As[t] = aik[t]; }
-    __syncthreads();
-    for (int idx = tid; idx < bb; idx += nt) {
-      const int r = idx % b, c = idx / b;
-      double s = 0.;
-      for (int t = 0; t < b; ++t) { s += As[r + t * b] * dk[t + size_t(c) * b]; }
-      Ls[idx] = s;
-      aik[idx] = s;
-    }
-    __syncthreads();
-    for (int u = upd_ptr[ik]; u < upd_ptr[ik + 1]; ++u) {
-      double *tgt = bsr + size_t(upd[2 * u]) * bb;
-      const double *up = bsr + size_t(upd[2 * u + 1]) * bb;
-      for (int idx = tid; idx < bb; idx += nt) {
-        const int r = idx % b, c = idx / b;
-        double s = 0.;
-        for (int t = 0; t < b; ++t) { s += Ls[r + t * b] * up[t + size_t(c) * b]; }
-        tgt[idx] -= s;
-      }
-    }
-    __syncthreads();
-  }
-  // Invert the updated diagonal block in place (same algorithm as
-  // k_block_inverse, inlined: one row per workgroup).
-  double *gA = bsr + size_t(diag_ptr[i]) * bb;
-  double *A = sm, *colk = sm + bb, *rowk = colk + b;
-  int *piv = reinterpret_cast<int *>(rowk + b);
-  __shared__ int s_piv;
-  for (int t = tid; t < bb; t += nt) { A[t] = gA[t]; }
-  __syncthreads();
-  bool singular = false;
-  for (int k = 0; k < b; ++k) {
-    if (tid < 64) {
-      double best = -1.;
-      int arg = b;
-      for (int r = k + tid; r < b; r += 64) {
-        const double a = A[r + k * b];
-        const double v = isnan(a) ? 0. : fabs(a); // keeps the pivot index in range
-        if (v > best) { best = v; arg = r; }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ob = __shfl_down(best, off, 64);
-        const int oa = __shfl_down(arg, off, 64);
-        if (ob > best || (ob == best && oa < arg)) { best = ob; arg = oa; }
-      }
-      if (tid == 0) { s_piv = arg; piv[k] = arg; }
-    }
-    __syncthreads();
-    const int p = s_piv;
-    if (p != k) {
-      for (int j = tid; j < b; j += nt) {
-        const double t = A[k + j * b];
-        A[k + j * b] = A[p + j * b];
-        A[p + j * b] = t;
-      }
-      __syncthreads();
-    }
-    for (int r = tid; r < b; r += nt) { colk[r] = A[r + k * b]; rowk[r] = A[k + r * b]; }
-    __syncthreads();
-    const double pivot = rowk[k];
-    if (!(fabs(pivot) > 0.) || !isfinite(pivot)) { singular = true; }
-    const double inv = 1. / pivot;
-    for (int idx = tid; idx < bb; idx += nt) {
-      const int r = idx % b, c = idx / b;
-      double v;
-      if (r == k) { v = c == k ? inv : rowk[c] * inv; }
-      else if (c == k) { v = -colk[r] * inv; }
-      else { v = A[idx] - colk[r] * (rowk[c] * inv); }
-      A[idx] = v;
-    }
-    __syncthreads();
-  }
-  for (int k = b - 1; k >= 0; --k) {
-    const int p = piv[k];
-    if (p != k) {
-      for (int r = tid; r < b; r += nt) {
-        const double t = A[r + k * b];
-        A[r + k * b] = A[r + p * b];
-        A[r + p * b] = t;
-      }
-      __syncthreads();
-    }
-  }
-  for (int t = tid; t < bb; t += nt) { gA[t] = A[t]; }
-  if (singular && tid == 0) { atomicOr(flags, kErrSingular); }
 }
 
 // Forward sweep of one level (src/block_ilu.cpp:87-99): w_i = b_i - sum_{j<i} L_ij w_j.
@@ -466,25 +464,15 @@ int grid_for(long n) {
 }
 } // namespace
 
-// Allow the LDS-resident dense kernels the whole 160 KB workgroup budget
-// (dynamic part = budget - static part). Errors are cleared so that they do
-// not surface later as a sticky error on an unrelated call.
+// Clears the error state and lets the LDS-resident factor kernels use the
+// whole 160 KB workgroup budget.
 int la_init() {
-  const void *fns[2] = {reinterpret_cast<const void *>(k_block_inverse), reinterpret_cast<const void *>(k_ilu_factor)};
-  int bad = 0;
-  for (const void *fn : fns) {
-    hipFuncAttributes attr{};
-    if (hipFuncGetAttributes(&attr, fn) != hipSuccess) { bad = 1; continue; }
-    const int dyn = kLdsBytes - int(attr.sharedSizeBytes);
-    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess) { bad = 1; }
-  }
   (void)hipGetLastError();
-  return bad | ilu_factor_lds_init();
+  return ilu_factor_lds_init();
 }
 void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
   if (fast_dense_ok(b)) { launch_block_inverse_fast(blocks, b, count, index, flags, s); return; }
-  const size_t lds = size_t(b) * b * 8 + 2 * b * 8 + b * 4;
-  k_block_inverse<<<count, 256, lds, s>>>(blocks, b, index, flags);
+  k_block_inverse_big<<<count, kBigThreads, 0, s>>>(blocks, b, index, flags);
 }
 template <class T>
 void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s) {
@@ -494,15 +482,16 @@ template void launch_block_gemv<double>(const double *, int, int, const double *
 template void launch_block_gemv<float>(const float *, int, int, const float *, float *, hipStream_t);
 
 void launch_ilu_factor_level(double *bsr, int b, const int *rows, int nrows, const int *row_ptr,
-                             const int *diag_ptr, const int *col, const int *upd_ptr, const int *upd, int *flags,
-                             hipStream_t s) {
-  if (fast_dense_ok(b) && b <= 100) {
+                             const int *diag_ptr, const int *col, const int *upd_ptr, const int *upd, double *scratch,
+                             int *flags, hipStream_t s) {
+  if (fast_dense_ok(b)) {
     launch_ilu_factor_fast(bsr, b, rows, nrows, row_ptr, diag_ptr, col, upd_ptr, upd, flags, s);
     return;
   }
-  const size_t lds = 2 * size_t(b) * b * 8;
-  k_ilu_factor<<<nrows, 256, lds, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, upd_ptr, upd, flags);
+  k_ilu_factor_big<<<nrows, kBigThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, upd_ptr, upd, scratch, flags);
 }
+bool big_block(int b) { return !fast_dense_ok(b); }
+int big_block_max() { return kBigMax; }
 template <class T>
 void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
                             const int *diag_ptr, const int *col, const T *rhs, T *w, hipStream_t s) {

@@ -25,7 +25,6 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
   require(p.opt.solver == "gmres", "Invalid solver");
   require(restart + 1 <= 64, "GMRES restart above 63 is not supported by the device solver");
   require(form == 0 || hybrid != nullptr, "Schur solver needs its condensed state");
-  if (form == 0) { require(p.m <= eg::kMaxLdsBlock, "DG block preconditioners on the GPU support k <= 4 in this build"); }
   V.resize(size_t(N) * (restart + 1));
   for (auto *v : {&w, &z, &comb, &corr, &res, &x, &act}) { v->resize(N); }
   partA.resize(size_t(64) * eg::kReduceBlocks);
@@ -50,6 +49,13 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     hip_check(hipMemsetAsync(done_factor, 0, sizeof(int) * pattern->nb, p.stream), "memset");
     sweep.resize(N);
     if (mixed) { sweep32.resize(N); }
+    if (eg::big_block(block)) {
+      int widest = 0;
+      for (size_t l = 0; l + 1 < pattern->lower_level_ptr.size(); ++l) {
+        widest = std::max(widest, pattern->lower_level_ptr[l + 1] - pattern->lower_level_ptr[l]);
+      }
+      factor_scratch.resize(size_t(widest) * block * block);
+    }
     if (form == 0) {
       std::vector<int> ds(p.n), sl(4 * p.n);
       for (int e = 0; e < p.n; ++e) {
@@ -128,7 +134,7 @@ void LinearSolver::Setup() {
       for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
         const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
         eg::launch_ilu_factor_level(blocks, b, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col,
-                                    pt.d_upd_ptr, pt.d_upd, p.flags, p.stream);
+                                    pt.d_upd_ptr, pt.d_upd, factor_scratch.p, p.flags, p.stream);
       }
     }
   } else {

@@ -201,9 +201,8 @@ Problem::Problem(const Options &o, const EulerGas &g, hipStream_t s)
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     own_stream = true;
   }
-  // Dense blocks above 64 KB need the extended LDS attribute.
-  require(eg::la_init() == 0 || 2 * size_t(std::max(m, t)) * std::max(m, t) * 8 <= 65536,
-          "Cannot enable 160 KB LDS for the dense block kernels");
+  require(eg::la_init() == 0, "Cannot enable 160 KB LDS for the dense block kernels");
+  require(m <= eg::big_block_max(), "Cell blocks above the dense block kernels' limit");
   frozen.resize(size_t(nf) * q);
   old.resize(nu);
   d_cell_metric.resize(nu);
@@ -559,7 +558,6 @@ int BlockPattern::Find(int r, int c) const {
 // ------------------------------------------------------------- Hybrid
 // src/condensation.cpp:47-85: storage allocated once before the time loop.
 Hybrid::Hybrid(Problem &problem, bool mixed) : p(problem) {
-  require(p.m <= eg::kMaxLdsBlock, "HDG condensation on the GPU supports k <= 4 in this build");
   A.resize(size_t(p.n) * p.m * p.m);
   C.resize(size_t(p.n) * 4 * p.q * 16);
   Xy.resize(size_t(p.n) * p.m * (4 * p.t + 1));

@@ -6,8 +6,8 @@ CPU: the oracle still reproduces the fixture bit for bit, and the fixture
 satisfies what the mathematics fixes (weights summing to one, derivative of a
 constant zero, nodal interpolation of the end values, positivity, linearity
 of the Jacobian action). GPU: the HIP library, called through the C ABI,
-reproduces the fixture (initial projection and GL tables to 1e-15, residual
-and Jacobian action to 1e-12 relative)."""
+reproduces the fixture (initial projection and GL tables to 1e-15, Jacobian
+action to 1e-12 and the pure spatial residual to 1e-10 relative)."""
 from pathlib import Path
 
 import numpy as np
@@ -76,7 +76,9 @@ def test_gpu_matches_fixture(gpu, golden):
     c.set_stage(u, DT)
     r = torch.zeros_like(u)
     c.residual(u, r)
-    assert rel(r.cpu().numpy(), golden["r0"]) < 1e-12
+    # u = old: r is the spatial term alone, dt M^{-1}(volume + faces), whose
+    # volume and face parts are each ~1e2 times larger than their sum here.
+    assert rel(r.cpu().numpy(), golden["r0"]) < 1e-10
     c.call("eg_dg_refresh", gpu._ptr(u))
     v = torch.tensor(np.sin(0.37 * (np.arange(c.nu) + 1)), device="cuda")
     y = torch.zeros_like(v)

@@ -15,6 +15,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 SMALL = [f"--nx 3 --ny 2 --k {k}" for k in range(5)] + ["--nx 4 --ny 5 --k 2 --alpha-deg 30"]
+# cell blocks above 100 x 100 (k >= 5) take the large-block kernels
+BIG = ["--nx 3 --ny 2 --k 5", "--nx 2 --ny 3 --k 6", "--nx 2 --ny 2 --k 7", "--nx 2 --ny 2 --k 8"]
 CFG1 = "--nx 8 --ny 8 --k 2 --alpha-deg 45 --steps 1 --dt 1e-4 --newton-steps 1 --precond block-jacobi"
 CFG2 = "--nx 16 --ny 16 --k 3 --cfl 1 --steps 3 --newton-steps 3 --precond block-ilu0"
 
@@ -59,7 +61,7 @@ def test_tables_initial_freeze(gpu, oracle_lib, args):
     assert rel(fg, fo) < 1e-14
 
 
-@pytest.mark.parametrize("args", SMALL + [CFG1, CFG2])
+@pytest.mark.parametrize("args", SMALL + BIG + [CFG1, CFG2])
 def test_residual_and_trial(gpu, oracle_lib, args):
     import torch
     o, c, u, du = make(gpu, oracle_lib, args)
@@ -86,7 +88,7 @@ def test_residual_and_trial(gpu, oracle_lib, args):
     assert c.admissible(du) and not c.admissible(-du)
 
 
-@pytest.mark.parametrize("args", SMALL + [CFG2])
+@pytest.mark.parametrize("args", SMALL + BIG + [CFG2])
 def test_dg_apply(gpu, oracle_lib, args):
     import torch
     o, c, u, du = make(gpu, oracle_lib, args)
@@ -106,7 +108,7 @@ def test_dg_apply(gpu, oracle_lib, args):
 
 
 @pytest.mark.parametrize("precond", ["block-jacobi", "block-ilu0"])
-@pytest.mark.parametrize("args", SMALL + [CFG2.replace(" --precond block-ilu0", "")])
+@pytest.mark.parametrize("args", SMALL + BIG + [CFG2.replace(" --precond block-ilu0", "")])
 def test_dg_preconditioner(gpu, oracle_lib, args, precond):
     import torch
     args = f"{args} --precond {precond}"
@@ -139,7 +141,7 @@ def hdg_case(gpu, oracle_lib, args):
     return o, c, u, du, X, y, rhs, D
 
 
-@pytest.mark.parametrize("args", SMALL + [CFG2])
+@pytest.mark.parametrize("args", SMALL + BIG + [CFG2])
 def test_hdg_condense_schur_recover(gpu, oracle_lib, args):
     import torch
     o, c, u, du, X, y, rhs, D = hdg_case(gpu, oracle_lib, args)
@@ -172,7 +174,7 @@ def test_hdg_condense_schur_recover(gpu, oracle_lib, args):
 
 
 @pytest.mark.parametrize("precond", ["block-jacobi", "block-ilu0"])
-@pytest.mark.parametrize("args", SMALL + [CFG2.replace(" --precond block-ilu0", "")])
+@pytest.mark.parametrize("args", SMALL + BIG + [CFG2.replace(" --precond block-ilu0", "")])
 def test_hdg_preconditioner(gpu, oracle_lib, args, precond):
     import torch
     o, c, u, du, *_ = hdg_case(gpu, oracle_lib, f"{args} --precond {precond}")
@@ -224,7 +226,8 @@ def increment_ok(o, inc_gpu, inc_ref, u_scale, bound=1e-8):
 @pytest.mark.parametrize("form", ["dg", "hdg"])
 @pytest.mark.parametrize("args", [CFG1, CFG2.replace("--steps 3", "--steps 1"),
                                   "--nx 12 --ny 12 --k 1 --cfl .5 --steps 1 --newton-steps 2 --precond block-ilu0",
-                                  "--nx 6 --ny 6 --k 4 --cfl .5 --steps 1 --newton-steps 2"])
+                                  "--nx 6 --ny 6 --k 4 --cfl .5 --steps 1 --newton-steps 2",
+                                  "--nx 4 --ny 4 --k 6 --cfl .5 --steps 1 --newton-steps 1 --precond block-ilu0"])
 def test_run_parity(gpu, oracle_lib, args, form):
     o = oracle_lib.Oracle(args)
     c = gpu.Context(args)
