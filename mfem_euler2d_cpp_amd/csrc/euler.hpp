@@ -90,6 +90,7 @@ struct BlockPattern {
   std::vector<int> row, col, diag, upd_ptr, upd;
   std::vector<int> lower_level_ptr, lower_rows, upper_level_ptr, upper_rows;
   DeviceArray<int> d_row, d_col, d_diag, d_upd_ptr, d_upd, d_lower_rows, d_upper_rows;
+  DeviceArray<int> d_lower_diag; // diagonal block slot of each row of lower_rows
   BlockPattern(const std::vector<std::vector<int>> &graph, int size, hipStream_t s);
   int Find(int r, int c) const;
 };
@@ -183,11 +184,17 @@ public:
   int epoch_factor = 0;
   DeviceArray<double> sweep;
   DeviceArray<float> sweep32;
-  DeviceArray<double> factor_scratch; // large blocks: one block per row of the widest level
+  DeviceArray<double> factor_scratch; // large blocks: one block per row (or lower block) of the widest level
+  // Large-block split factor plan, per dependency level l: lower blocks
+  // [a_ptr[l], a_ptr[l+1]) and updated blocks [t_ptr[l], t_ptr[l+1]).
+  bool split_factor = false;
+  std::vector<int> a_ptr, t_ptr;
+  DeviceArray<int> d_a_ik, d_a_kk, d_t_slot, d_t_pair_ptr, d_pairs;
   bool dataflow = true, dataflow_factor = false;
   std::vector<double> h, cosine, sine, rotated, coefficients;
 
   LinearSolver(Problem &p, int form, Hybrid *hybrid = nullptr);
+  void PlanBigFactor();
   void Refresh(const double *dev_u) { state = dev_u; }
   void Setup();  // fill + factor the preconditioner at the current state
   void Apply(const double *x, double *y);

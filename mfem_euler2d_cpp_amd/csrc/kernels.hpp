@@ -45,10 +45,18 @@ int la_init(); // nonzero if the large-LDS attribute could not be set (clears th
 void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s);
 template <class T>
 void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s);
-// scratch: nrows b x b blocks, used for large blocks (big_block(b)) only.
-void launch_ilu_factor_level(double *bsr, int b, const int *rows, int nrows, const int *row_ptr,
-                             const int *diag_ptr, const int *col, const int *upd_ptr, const int *upd,
-                             double *scratch, int *flags, hipStream_t s);
+// rows: the level's rows; row_diag: their diagonal block slots; scratch:
+// nrows b x b blocks, used for large blocks (big_block(b)) only.
+void launch_ilu_factor_level(double *bsr, int b, const int *rows, const int *row_diag, int nrows,
+                             const int *row_ptr, const int *diag_ptr, const int *col, const int *upd_ptr,
+                             const int *upd, double *scratch, int *flags, hipStream_t s);
+// Split large-block form of one level: na lower blocks (slots a_ik, inverse
+// diagonal slots a_kk, scratch of na blocks), nt updated blocks t_slot with
+// pairs [t_pair_ptr[t], t_pair_ptr[t+1]) of (L slot, U slot), then the
+// inverses of the level's nrows diagonal blocks row_diag.
+void launch_ilu_factor_level_split(double *bsr, int b, int na, const int *a_ik, const int *a_kk, int nt,
+                                   const int *t_slot, const int *t_pair_ptr, const int *pairs, const int *row_diag,
+                                   int nrows, double *scratch, int *flags, hipStream_t s);
 // Blocks handled by the global-memory large-block kernels, and their limit.
 bool big_block(int b);
 int big_block_max();
@@ -61,7 +69,8 @@ void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, con
 
 // Register-resident / split-dot variants (kernels_ilu.hip) used by the
 // launchers above when the block size allows.
-bool fast_dense_ok(int b);
+bool fast_dense_ok(int b); // LDS-staged ILU factor rows and register sweeps
+bool gj_regs_ok(int b);    // register-resident Gauss-Jordan inverse
 int ilu_factor_lds_init();
 void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s);
 void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,

@@ -38,18 +38,20 @@ __device__ __forceinline__ void complete_memory_ops() {
 // consecutive lanes of one wavefront (R a power of two <= 16, so the group
 // is one aligned DPP row or a part of one), lane r holding rows r, r + R, ...
 template <int B> struct Shape {
-  static_assert(B >= 1 && B <= 127, "pivot keys hold the row index in 7 bits");
-  static constexpr int R = B <= 4 ? 4 : (B <= 8 ? 8 : (16 * B <= 1024 ? 16 : 8)); // lanes per column
+  static_assert(B >= 1 && B <= 511, "pivot keys hold the row index in 9 bits");
+  static constexpr int R = B <= 4 ? 4 : (B <= 8 ? 8 : (16 * B <= 1024 ? 16 : (8 * B <= 1024 ? 8 : 4))); // lanes per column
   static constexpr int NT = (B * R + 63) / 64 * 64;                                // threads per workgroup
   static constexpr int NE = (B + R - 1) / R;                                       // register entries per thread
-  static_assert(NE <= 32, "one bit per register entry");
+  static_assert(NE <= 64 && NT <= 1024, "one bit per register entry, one workgroup");
 };
 template <int B> struct GjLds {
   double col[2][B];              // pivot column, double buffered
   double pval[2];                // pivot value
   unsigned long long key[2];     // winning pivot key
+  unsigned long long wkey[2][16]; // per-wave keys of the workgroup-wide search
   double scale[B];               // row scale: true row = scale * stored row
   int piv[B], pinv[B];           // step -> pivot row, pivot row -> step
+  int used[B];                   // row already a pivot row (workgroup-wide search)
 };
 template <int B> constexpr int gj_lds() { return int((sizeof(GjLds<B>) + 7) / 8); } // in doubles
 
@@ -76,92 +78,149 @@ __device__ __forceinline__ double lane_value(double v, int src) {
   return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
 
+// Pivot key of row i: |value| bits with the low 9 replaced by 511 - i, so a
+// larger key is a larger |value| and near-ties go to the lowest row (a NaN
+// ranks high and is reported through its pivot value).
+__device__ __forceinline__ unsigned long long pivot_key(double v, int i) {
+  return (static_cast<unsigned long long>(__double_as_longlong(v)) & 0x7FFFFFFFFFFFFE00ull) | unsigned(511 - i);
+}
+// Max over the 64 lanes of a wavefront: DPP within rows of 16, then xor
+// shuffles across rows.
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long k) {
+  k = group_max<16>(k);
+  for (int m = 16; m < 64; m <<= 1) {
+    const unsigned lo = unsigned(__shfl_xor(int(unsigned(k)), m, 64));
+    const unsigned hi = unsigned(__shfl_xor(int(unsigned(k >> 32)), m, 64));
+    const unsigned long long o = (static_cast<unsigned long long>(hi) << 32) | lo;
+    k = o > k ? o : k;
+  }
+  return k;
+}
+
 // In-place inverse of the B x B column-major block A by one workgroup of
 // Shape<B>::NT threads; thread (tc, tr) = (tid / R, tid % R) keeps column tc
 // at rows tr + e R in registers. Step k eliminates column k with pivot row
-// p = an unused row of maximal |S(p,k)|: the column's lanes reduce the key
-// (|S(i,k)| bits with the low 7 replaced by 127 - i, used rows 0) in
-// registers, so near-ties go to the lowest row and p is always an unused row
-// (a NaN ranks high and is reported through its pivot value). The pivot
-// row's value for each column comes from a lane of the same group
-// (ds_bpermute), so a step needs one barrier.
-// Rows are stored unscaled: the pivot row keeps its values and records
-// scale[p] = 1 / S(p,k), and the broadcast column holds 0 at row p, so every
-// entry takes the same update S(i,j) -= S(i,k) S(p,j) / S(p,k); only the
-// column k itself is replaced (-S(i,k) / S(p,k), and 1 at row p). With the
-// scales applied at the end, A^{-1}(pinv[i], piv[j]) = scale[i] S(i,j).
+// p = an unused row of maximal |S(p,k)| (pivot_key). For up to 16 entries a
+// lane, the column's own lane group reduces the keys in registers (DPP), one
+// barrier a step; above that, the column is written to LDS and the whole
+// workgroup reduces one key a thread, two barriers a step. The pivot row's
+// value for each column comes from a lane of the same group (ds_bpermute).
+// Rows are stored unscaled: every entry, the pivot row's included, takes
+// S(i,j) -= S(i,k) S(p,j) / S(p,k), after which the pivot row's owners put
+// back its old values and record scale[p] = 1 / S(p,k); column k is replaced
+// by -S(i,k) / S(p,k), and 1 at row p. With the scales applied at the end,
+// A^{-1}(pinv[i], piv[j]) = scale[i] S(i,j). Register indices that depend on
+// the pivot are resolved with uniform branches, not per-entry selects.
 // Mathematically the inverse of the block whose LU the reference uses
 // (src/linear.cpp:198-202, src/block_ilu.cpp:83-84, src/condensation.cpp:137-140).
 // Returns true on a zero or non-finite pivot.
 template <int B, bool COH>
 __device__ __forceinline__ bool gj_inverse(double *__restrict__ A, double *lds) {
-  constexpr int R = Shape<B>::R, NE = Shape<B>::NE;
+  constexpr int R = Shape<B>::R, NE = Shape<B>::NE, NT = Shape<B>::NT;
+  constexpr bool WIDE = NE > 16;
+  static_assert(!WIDE || (NT / 64 <= 16 && B <= NT), "one key a thread, at most 16 waves");
   GjLds<B> &L = *reinterpret_cast<GjLds<B> *>(lds);
   const int tid = threadIdx.x, tc = tid / R, tr = tid % R;
   const bool active = tc < B;
   const int group = (tid & 63) & ~(R - 1); // first lane of this column's group
   double v[NE];
-  unsigned used = 0u; // bit e: row tr + e R is a pivot row already
+  unsigned long long used = 0ull; // bit e: row tr + e R is a pivot row already
 #pragma unroll
   for (int e = 0; e < NE; ++e) {
     const int i = tr + e * R;
     v[e] = (active && i < B) ? ld<COH>(A + i + tc * B) : 0.;
+  }
+  if (WIDE) {
+    for (int i = tid; i < B; i += blockDim.x) { L.used[i] = 0; }
   }
   __syncthreads();
   bool singular = false;
   for (int k = 0; k < B; ++k) {
     const int buf = k & 1;
     double *ck = L.col[buf];
-    if (tc == k) {
-      unsigned long long key = 0ull;
-      double bv = 0.;
+    int p;
+    double ap;
+    if (!WIDE) {
+      if (tc == k) {
+        unsigned long long key = 0ull;
+        double bv = 0.;
 #pragma unroll
-      for (int e = 0; e < NE; ++e) {
-        const int i = tr + e * R;
-        if (i < B) {
-          ck[i] = v[e];
-          const unsigned long long kk =
-              (static_cast<unsigned long long>(__double_as_longlong(v[e])) & 0x7FFFFFFFFFFFFF80ull) |
-              unsigned(127 - i);
-          if (!((used >> e) & 1u) && kk > key) { key = kk; bv = v[e]; }
+        for (int e = 0; e < NE; ++e) {
+          const int i = tr + e * R;
+          if (i < B) {
+            ck[i] = v[e];
+            const unsigned long long kk = pivot_key(v[e], i);
+            if (!((used >> e) & 1ull) && kk > key) { key = kk; bv = v[e]; }
+          }
+        }
+        const unsigned long long best = group_max<R>(key);
+        if (key == best) {
+          L.key[buf] = best;
+          L.pval[buf] = bv;
         }
       }
-      const unsigned long long best = group_max<R>(key);
-      if (key == best) {
-        L.key[buf] = best;
-        L.pval[buf] = bv;
-        ck[127 - int(best & 127ull)] = 0.;
+      __syncthreads();
+      p = 511 - int(L.key[buf] & 511ull);
+      ap = L.pval[buf];
+    } else {
+      if (tc == k) {
+#pragma unroll
+        for (int e = 0; e < NE; ++e) {
+          const int i = tr + e * R;
+          if (i < B) { ck[i] = v[e]; }
+        }
       }
+      __syncthreads();
+      const unsigned long long key = wave_max((tid < B && !L.used[tid]) ? pivot_key(ck[tid], tid) : 0ull);
+      if ((tid & 63) == 0) { L.wkey[buf][tid >> 6] = key; }
+      __syncthreads();
+      unsigned long long best = 0ull;
+#pragma unroll
+      for (int w = 0; w < NT / 64; ++w) { best = L.wkey[buf][w] > best ? L.wkey[buf][w] : best; }
+      p = 511 - int(best & 511ull);
+      ap = ck[p];
     }
-    __syncthreads();
-    const int p = 127 - int(L.key[buf] & 127ull);
-    const double ap = L.pval[buf];
     if (!(fabs(ap) > 0.) || !isfinite(ap)) { singular = true; }
     const int ep = p / R, lp = p % R;
     const bool mine = tr == lp;
     double sel = 0.;
 #pragma unroll
-    for (int e = 0; e < NE; ++e) { sel = e == ep ? v[e] : sel; }
-    if (mine) { used |= 1u << ep; }
+    for (int e = 0; e < NE; ++e) {
+      if (e == ep) {
+        asm volatile(""); // uniform branch, no per-entry select
+        sel = v[e];
+      }
+    }
+    if (mine) { used |= 1ull << ep; }
     const double rpv = lane_value(sel, group + lp);
     const double inv = 1. / ap;
     if (tid == 0) {
       L.piv[k] = p;
       L.pinv[p] = k;
       L.scale[p] = inv;
+      if (WIDE) { L.used[p] = 1; }
     }
     const double rpj = rpv * inv;
-    double c[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
       const int i = tr + e * R;
-      c[e] = ck[i < B ? i : B - 1];
-      v[e] = fma(-c[e], rpj, v[e]);
+      v[e] = fma(-ck[i < B ? i : B - 1], rpj, v[e]);
     }
     if (tc == k) {
       asm volatile(""); // keep this a branch taken by one lane group only
 #pragma unroll
-      for (int e = 0; e < NE; ++e) { v[e] = (mine && e == ep) ? 1. : -c[e] * inv; }
+      for (int e = 0; e < NE; ++e) {
+        const int i = tr + e * R;
+        v[e] = -ck[i < B ? i : B - 1] * inv;
+      }
+    }
+    const double keep = tc == k ? 1. : sel;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      if (e == ep) {
+        asm volatile(""); // uniform branch: restore the pivot row
+        v[e] = mine ? keep : v[e];
+      }
     }
   }
   __syncthreads();
@@ -269,31 +328,32 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_rows(double *__rest
 
 // ------------------------------------------------------------ sweeps
 // Dot products of block row r with x, split over P = blockDim/b threads
-// (columns c = part, part + P, ...).
+// (columns c = part, part + P, ...), four independent loads in flight.
 template <class T>
 __device__ __forceinline__ T row_dot(const T *__restrict__ a, const T *__restrict__ x, int b, int r, int part, int P) {
-  T acc0 = T(0), acc1 = T(0);
+  T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
   int c = part;
-  for (; c + P < b; c += 2 * P) {
-    acc0 += a[size_t(c) * b + r] * x[c];
-    acc1 += a[size_t(c + P) * b + r] * x[c + P];
+  for (; c + 3 * P < b; c += 4 * P) {
+    a0 += a[size_t(c) * b + r] * x[c];
+    a1 += a[size_t(c + P) * b + r] * x[c + P];
+    a2 += a[size_t(c + 2 * P) * b + r] * x[c + 2 * P];
+    a3 += a[size_t(c + 3 * P) * b + r] * x[c + 3 * P];
   }
-  if (c < b) { acc0 += a[size_t(c) * b + r] * x[c]; }
-  return acc0 + acc1;
+  for (; c < b; c += P) { a0 += a[size_t(c) * b + r] * x[c]; }
+  return (a0 + a1) + (a2 + a3);
 }
-constexpr int kSweepThreads = 256;
+constexpr int kSweepMaxBlock = 511;
 // Forward sweep of one level (src/block_ilu.cpp:87-99): w_i = b_i - sum_{j<i} L_ij w_j.
-template <class T>
-__global__ __launch_bounds__(kSweepThreads) void k_ilu_lower_split(const T *__restrict__ bsr, int b,
-                                                                   const int *__restrict__ rows,
-                                                                   const int *__restrict__ row_ptr,
-                                                                   const int *__restrict__ diag_ptr,
-                                                                   const int *__restrict__ col,
-                                                                   const T *__restrict__ rhs, T *__restrict__ w) {
-  __shared__ T red[kSweepThreads];
+// NT threads (256 up to 64 x 64 blocks, 1024 above: more loads in flight).
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_ilu_lower_split(const T *__restrict__ bsr, int b, const int *__restrict__ rows,
+                                                       const int *__restrict__ row_ptr,
+                                                       const int *__restrict__ diag_ptr, const int *__restrict__ col,
+                                                       const T *__restrict__ rhs, T *__restrict__ w) {
+  __shared__ T red[NT];
   const int i = rows[blockIdx.x];
   const size_t bb = size_t(b) * b;
-  const int P = blockDim.x / b, r = threadIdx.x % b, part = threadIdx.x / b;
+  const int P = NT / b, r = threadIdx.x % b, part = threadIdx.x / b;
   T acc = T(0);
   if (part < P) {
     for (int ij = row_ptr[i]; ij < diag_ptr[i]; ++ij) { acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P); }
@@ -308,17 +368,16 @@ __global__ __launch_bounds__(kSweepThreads) void k_ilu_lower_split(const T *__re
 }
 // Backward sweep of one level (src/block_ilu.cpp:100-108):
 // w_i = inv(U_ii) (w_i - sum_{j>i} U_ij w_j).
-template <class T>
-__global__ __launch_bounds__(kSweepThreads) void k_ilu_upper_split(const T *__restrict__ bsr, int b,
-                                                                   const int *__restrict__ rows,
-                                                                   const int *__restrict__ row_ptr,
-                                                                   const int *__restrict__ diag_ptr,
-                                                                   const int *__restrict__ col, T *__restrict__ w) {
-  __shared__ T red[kSweepThreads];
-  __shared__ T ts[kSweepThreads];
+template <class T, int NT>
+__global__ __launch_bounds__(NT) void k_ilu_upper_split(const T *__restrict__ bsr, int b, const int *__restrict__ rows,
+                                                       const int *__restrict__ row_ptr,
+                                                       const int *__restrict__ diag_ptr, const int *__restrict__ col,
+                                                       T *__restrict__ w) {
+  __shared__ T red[NT];
+  __shared__ T ts[kSweepMaxBlock];
   const int i = rows[blockIdx.x];
   const size_t bb = size_t(b) * b;
-  const int P = blockDim.x / b, r = threadIdx.x % b, part = threadIdx.x / b;
+  const int P = NT / b, r = threadIdx.x % b, part = threadIdx.x / b;
   T acc = T(0);
   if (part < P) {
     for (int ij = diag_ptr[i] + 1; ij < row_ptr[i + 1]; ++ij) { acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P); }
@@ -573,6 +632,14 @@ constexpr int kSpinLimit = 1 << 20;
   default: break;                                                                                  \
   }
 
+#define EG_DISPATCH_GJ(b, CALL)                                                                    \
+  switch (b) {                                                                                     \
+  case 144: { constexpr int B = 144; CALL; } break;                                                \
+  case 196: { constexpr int B = 196; CALL; } break;                                                \
+  default: EG_DISPATCH_B(b, CALL) break;                                                           \
+  }
+
+bool gj_regs_ok(int b) { return fast_dense_ok(b) || b == 144 || b == 196; }
 bool fast_dense_ok(int b) {
   return (b % 4 == 0 && b >= 4 && b <= 36) || b == 64 || b == 100;
 }
@@ -586,7 +653,7 @@ int ilu_factor_lds_init() {
   return 0;
 }
 void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
-  EG_DISPATCH_B(b, (k_block_inverse_b<B><<<count, Shape<B>::NT, 0, s>>>(blocks, index, flags)));
+  EG_DISPATCH_GJ(b, (k_block_inverse_b<B><<<count, Shape<B>::NT, 0, s>>>(blocks, index, flags)));
 }
 void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                             const int *col, const int *upd_ptr, const int *upd, int *flags, hipStream_t s) {
@@ -596,12 +663,14 @@ void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, cons
 template <class T>
 void launch_ilu_lower_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                            const int *col, const T *rhs, T *w, hipStream_t s) {
-  k_ilu_lower_split<T><<<nrows, kSweepThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w);
+  if (b <= 64) { k_ilu_lower_split<T, 256><<<nrows, 256, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w); }
+  else { k_ilu_lower_split<T, 1024><<<nrows, 1024, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w); }
 }
 template <class T>
 void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                            const int *col, T *w, hipStream_t s) {
-  k_ilu_upper_split<T><<<nrows, kSweepThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w);
+  if (b <= 64) { k_ilu_upper_split<T, 256><<<nrows, 256, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w); }
+  else { k_ilu_upper_split<T, 1024><<<nrows, 1024, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w); }
 }
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,

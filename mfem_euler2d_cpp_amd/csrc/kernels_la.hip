@@ -1,8 +1,10 @@
 // Dense-block and Krylov kernels: large-block (b > 100) Gauss-Jordan inverse
-// and ILU(0) factor in global memory, block GEMV for block-Jacobi, generic
-// level-scheduled BSR triangular sweeps, and the vector kernels of the
+// and ILU(0) block products in global memory, block GEMV for block-Jacobi,
+// and the vector kernels of the
 // weighted restarted GMRES (fused CGS2 passes with deterministic two-stage
 // reductions).
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace eg {
@@ -104,12 +106,14 @@ constexpr int kTile = 64, kPanel = 16;
 struct ProductLds {
   double x[4][kTile * kPanel], y[4][kPanel * kTile];
 };
+// The workgroup computes the tiles 4 slice + g, 4 (slice + nslices) + g, ...
+// so several workgroups can share one product.
 __device__ void tile_product_global(const double *__restrict__ X, const double *__restrict__ Y, double *C, int b,
-                                    int mode, ProductLds &S) {
+                                    int mode, ProductLds &S, int slice = 0, int nslices = 1) {
   const int tid = threadIdx.x, g = tid >> 8, lt = tid & 255, tr = lt % 16, tc = lt / 16;
   const int nt = (b + kTile - 1) / kTile, tiles = nt * nt;
   double *xs = S.x[g], *ys = S.y[g];
-  for (int t0 = 0; t0 < tiles; t0 += 4) {
+  for (int t0 = 4 * slice; t0 < tiles; t0 += 4 * nslices) {
     const int tile = t0 + g;
     const bool act = tile < tiles;
     const int r0 = (tile % nt) * kTile, c0 = (tile / nt) * kTile;
@@ -153,20 +157,19 @@ __device__ void tile_product_global(const double *__restrict__ X, const double *
   __syncthreads();
 }
 
-// One BSR row of ILU(0) (src/block_ilu.cpp:73-86) per workgroup for large
-// blocks, rows of one dependency level in parallel: L_ik = A_ik inv(A_kk)
-// through the workgroup's scratch block, A_ij -= L_ik A_kj for the cached
-// update pairs, then the updated diagonal block is replaced by its inverse.
-__global__ __launch_bounds__(kBigThreads) void k_ilu_factor_big(double *__restrict__ bsr, int b,
+// Block products of one BSR row of ILU(0) (src/block_ilu.cpp:73-86) per
+// workgroup for large blocks, rows of one dependency level in parallel:
+// L_ik = A_ik inv(A_kk) through the workgroup's scratch block, then
+// A_ij -= L_ik A_kj for the cached update pairs. The launcher then replaces
+// the level's updated diagonal blocks by their inverses.
+__global__ __launch_bounds__(kBigThreads) void k_ilu_products_big(double *__restrict__ bsr, int b,
                                                                 const int *__restrict__ rows,
                                                                 const int *__restrict__ row_ptr,
                                                                 const int *__restrict__ diag_ptr,
                                                                 const int *__restrict__ col,
                                                                 const int *__restrict__ upd_ptr,
-                                                                const int *__restrict__ upd, double *scratch,
-                                                                int *__restrict__ flags) {
+                                                                const int *__restrict__ upd, double *scratch) {
   __shared__ ProductLds S;
-  __shared__ BigLds L;
   const size_t bb = size_t(b) * b;
   const int i = rows[blockIdx.x];
   double *tmp = scratch + size_t(blockIdx.x) * bb;
@@ -179,7 +182,6 @@ __global__ __launch_bounds__(kBigThreads) void k_ilu_factor_big(double *__restri
       tile_product_global(aik, bsr + size_t(upd[2 * u + 1]) * bb, bsr + size_t(upd[2 * u]) * bb, b, 1, S);
     }
   }
-  if (gj_global(bsr + size_t(diag_ptr[i]) * bb, b, L) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
 }
 
 // y_blk = A_blk x_blk for count independent b x b blocks; one wave per block.
@@ -201,56 +203,6 @@ __global__ __launch_bounds__(256) void k_block_gemv(const T *__restrict__ blocks
       for (int c = 0; c < b; ++c) { s += a[size_t(c) * b + r] * xw[c]; }
       y[size_t(blk) * b + r] = s;
     }
-  }
-}
-
-// Forward sweep of one level (src/block_ilu.cpp:87-99): w_i = b_i - sum_{j<i} L_ij w_j.
-template <class T>
-__global__ __launch_bounds__(256) void k_ilu_lower(const T *__restrict__ bsr, int b, const int *__restrict__ rows,
-                                                   const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
-                                                   const int *__restrict__ col, const T *__restrict__ rhs,
-                                                   T *__restrict__ w) {
-  const int i = rows[blockIdx.x];
-  const size_t bb = size_t(b) * b;
-  for (int r = threadIdx.x; r < b; r += blockDim.x) {
-    T s = rhs[size_t(i) * b + r];
-    for (int ij = row_ptr[i]; ij < diag_ptr[i]; ++ij) {
-      const T *a = bsr + ij * bb;
-      const T *wj = w + size_t(col[ij]) * b;
-      T acc = T(0);
-      for (int c = 0; c < b; ++c) { acc += a[size_t(c) * b + r] * wj[c]; }
-      s -= acc;
-    }
-    w[size_t(i) * b + r] = s;
-  }
-}
-// Backward sweep of one level (src/block_ilu.cpp:100-108):
-// w_i = inv(U_ii) (w_i - sum_{j>i} U_ij w_j).
-template <class T>
-__global__ __launch_bounds__(256) void k_ilu_upper(const T *__restrict__ bsr, int b, const int *__restrict__ rows,
-                                                   const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
-                                                   const int *__restrict__ col, T *__restrict__ w) {
-  extern __shared__ unsigned char raw[];
-  T *ts = reinterpret_cast<T *>(raw);
-  const int i = rows[blockIdx.x];
-  const size_t bb = size_t(b) * b;
-  for (int r = threadIdx.x; r < b; r += blockDim.x) {
-    T s = w[size_t(i) * b + r];
-    for (int ij = diag_ptr[i] + 1; ij < row_ptr[i + 1]; ++ij) {
-      const T *a = bsr + ij * bb;
-      const T *wj = w + size_t(col[ij]) * b;
-      T acc = T(0);
-      for (int c = 0; c < b; ++c) { acc += a[size_t(c) * b + r] * wj[c]; }
-      s -= acc;
-    }
-    ts[r] = s;
-  }
-  __syncthreads();
-  const T *d = bsr + diag_ptr[i] * bb;
-  for (int r = threadIdx.x; r < b; r += blockDim.x) {
-    T s = T(0);
-    for (int c = 0; c < b; ++c) { s += d[size_t(c) * b + r] * ts[c]; }
-    w[size_t(i) * b + r] = s;
   }
 }
 
@@ -464,6 +416,43 @@ int grid_for(long n) {
 }
 } // namespace
 
+// Split form of the large-block ILU(0) products of one dependency level, for
+// patterns where no update targets a lower block of its own row (cell grids
+// of 3 or more cells a side): first every L_ik = A_ik inv(A_kk) of the level
+// (one workgroup each, through a scratch block), then every updated block
+// A_t -= sum L_ik A_kj over its pairs in ascending k (one workgroup each).
+// Same products and per-block order as k_ilu_products_big, on more CUs.
+// Products are shared by gridDim.y workgroups (tile slices).
+__global__ __launch_bounds__(kBigThreads) void k_ilu_lblocks_big(const double *__restrict__ bsr, int b,
+                                                                 const int *__restrict__ a_ik,
+                                                                 const int *__restrict__ a_kk, double *scratch) {
+  __shared__ ProductLds S;
+  const size_t bb = size_t(b) * b;
+  tile_product_global(bsr + size_t(a_ik[blockIdx.x]) * bb, bsr + size_t(a_kk[blockIdx.x]) * bb,
+                      scratch + size_t(blockIdx.x) * bb, b, 0, S, blockIdx.y, gridDim.y);
+}
+__global__ void k_copy_lblocks(double *__restrict__ bsr, int b, const int *__restrict__ a_ik,
+                               const double *__restrict__ scratch) {
+  const size_t bb = size_t(b) * b;
+  double *aik = bsr + size_t(a_ik[blockIdx.y]) * bb;
+  const double *tmp = scratch + size_t(blockIdx.y) * bb;
+  for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < bb; t += size_t(gridDim.x) * blockDim.x) {
+    aik[t] = tmp[t];
+  }
+}
+__global__ __launch_bounds__(kBigThreads) void k_ilu_targets_big(double *__restrict__ bsr, int b,
+                                                                 const int *__restrict__ t_slot,
+                                                                 const int *__restrict__ t_pair_ptr,
+                                                                 const int *__restrict__ pairs) {
+  __shared__ ProductLds S;
+  const size_t bb = size_t(b) * b;
+  double *tgt = bsr + size_t(t_slot[blockIdx.x]) * bb;
+  for (int q = t_pair_ptr[blockIdx.x]; q < t_pair_ptr[blockIdx.x + 1]; ++q) {
+    tile_product_global(bsr + size_t(pairs[2 * q]) * bb, bsr + size_t(pairs[2 * q + 1]) * bb, tgt, b, 1, S,
+                        blockIdx.y, gridDim.y);
+  }
+}
+
 // Clears the error state and lets the LDS-resident factor kernels use the
 // whole 160 KB workgroup budget.
 int la_init() {
@@ -471,7 +460,7 @@ int la_init() {
   return ilu_factor_lds_init();
 }
 void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
-  if (fast_dense_ok(b)) { launch_block_inverse_fast(blocks, b, count, index, flags, s); return; }
+  if (gj_regs_ok(b)) { launch_block_inverse_fast(blocks, b, count, index, flags, s); return; }
   k_block_inverse_big<<<count, kBigThreads, 0, s>>>(blocks, b, index, flags);
 }
 template <class T>
@@ -481,29 +470,39 @@ void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipS
 template void launch_block_gemv<double>(const double *, int, int, const double *, double *, hipStream_t);
 template void launch_block_gemv<float>(const float *, int, int, const float *, float *, hipStream_t);
 
-void launch_ilu_factor_level(double *bsr, int b, const int *rows, int nrows, const int *row_ptr,
-                             const int *diag_ptr, const int *col, const int *upd_ptr, const int *upd, double *scratch,
-                             int *flags, hipStream_t s) {
+void launch_ilu_factor_level(double *bsr, int b, const int *rows, const int *row_diag, int nrows,
+                             const int *row_ptr, const int *diag_ptr, const int *col, const int *upd_ptr,
+                             const int *upd, double *scratch, int *flags, hipStream_t s) {
   if (fast_dense_ok(b)) {
     launch_ilu_factor_fast(bsr, b, rows, nrows, row_ptr, diag_ptr, col, upd_ptr, upd, flags, s);
     return;
   }
-  k_ilu_factor_big<<<nrows, kBigThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, upd_ptr, upd, scratch, flags);
+  k_ilu_products_big<<<nrows, kBigThreads, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, upd_ptr, upd, scratch);
+  launch_block_inverse(bsr, b, nrows, row_diag, flags, s);
+}
+void launch_ilu_factor_level_split(double *bsr, int b, int na, const int *a_ik, const int *a_kk, int nt,
+                                   const int *t_slot, const int *t_pair_ptr, const int *pairs, const int *row_diag,
+                                   int nrows, double *scratch, int *flags, hipStream_t s) {
+  const int tiles = ((b + kTile - 1) / kTile) * ((b + kTile - 1) / kTile);
+  const int slices = std::min((tiles + 3) / 4, 16);
+  if (na > 0) {
+    k_ilu_lblocks_big<<<dim3(na, slices), kBigThreads, 0, s>>>(bsr, b, a_ik, a_kk, scratch);
+    k_copy_lblocks<<<dim3(16, na), 256, 0, s>>>(bsr, b, a_ik, scratch);
+  }
+  if (nt > 0) { k_ilu_targets_big<<<dim3(nt, slices), kBigThreads, 0, s>>>(bsr, b, t_slot, t_pair_ptr, pairs); }
+  launch_block_inverse(bsr, b, nrows, row_diag, flags, s);
 }
 bool big_block(int b) { return !fast_dense_ok(b); }
 int big_block_max() { return kBigMax; }
 template <class T>
 void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
                             const int *diag_ptr, const int *col, const T *rhs, T *w, hipStream_t s) {
-  if (b <= 256) { launch_ilu_lower_fast<T>(bsr, b, rows, nrows, row_ptr, diag_ptr, col, rhs, w, s); return; }
-  k_ilu_lower<T><<<nrows, b <= 64 ? 64 : (b <= 128 ? 128 : 256), 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w);
+  launch_ilu_lower_fast<T>(bsr, b, rows, nrows, row_ptr, diag_ptr, col, rhs, w, s);
 }
 template <class T>
 void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
                             const int *diag_ptr, const int *col, T *w, hipStream_t s) {
-  if (b <= 256) { launch_ilu_upper_fast<T>(bsr, b, rows, nrows, row_ptr, diag_ptr, col, w, s); return; }
-  k_ilu_upper<T><<<nrows, b <= 64 ? 64 : (b <= 128 ? 128 : 256), b * sizeof(T), s>>>(bsr, b, rows, row_ptr,
-                                                                                     diag_ptr, col, w);
+  launch_ilu_upper_fast<T>(bsr, b, rows, nrows, row_ptr, diag_ptr, col, w, s);
 }
 template void launch_ilu_lower_level<double>(const double *, int, const int *, int, const int *, const int *,
                                              const int *, const double *, double *, hipStream_t);

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <limits>
+#include <map>
 
 #include "euler.hpp"
 
@@ -49,13 +50,7 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     hip_check(hipMemsetAsync(done_factor, 0, sizeof(int) * pattern->nb, p.stream), "memset");
     sweep.resize(N);
     if (mixed) { sweep32.resize(N); }
-    if (eg::big_block(block)) {
-      int widest = 0;
-      for (size_t l = 0; l + 1 < pattern->lower_level_ptr.size(); ++l) {
-        widest = std::max(widest, pattern->lower_level_ptr[l + 1] - pattern->lower_level_ptr[l]);
-      }
-      factor_scratch.resize(size_t(widest) * block * block);
-    }
+    if (eg::big_block(block)) { PlanBigFactor(); }
     if (form == 0) {
       std::vector<int> ds(p.n), sl(4 * p.n);
       for (int e = 0; e < p.n; ++e) {
@@ -106,6 +101,66 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
   p.Sync();
 }
 
+// Large blocks: one workgroup per row is too few at a dependency level, so
+// the level's products are split per lower block and per updated block when
+// no update of a row targets one of its own lower blocks.
+void LinearSolver::PlanBigFactor() {
+  const auto &pt = *pattern;
+  const int nlev = int(pt.lower_level_ptr.size()) - 1;
+  bool independent = true;
+  for (int i = 0; i < pt.nb && independent; ++i) {
+    for (int ik = pt.row[i]; ik < pt.diag[i]; ++ik) {
+      for (int u = pt.upd_ptr[ik]; u < pt.upd_ptr[ik + 1]; ++u) {
+        if (pt.col[pt.upd[2 * u]] < i) { independent = false; }
+      }
+    }
+  }
+  int widest = 0;
+  std::vector<int> a_ik, a_kk, t_slot, t_pair_ptr{0}, pairs;
+  a_ptr.assign(1, 0);
+  t_ptr.assign(1, 0);
+  for (int l = 0; l < nlev; ++l) {
+    const int first = pt.lower_level_ptr[l], last = pt.lower_level_ptr[l + 1];
+    int na = 0;
+    for (int r = first; r < last; ++r) {
+      const int i = pt.lower_rows[r];
+      std::map<int, std::vector<int>> by_target; // target slot -> (L, U) pairs, ascending k
+      for (int ik = pt.row[i]; ik < pt.diag[i]; ++ik) {
+        a_ik.push_back(ik);
+        a_kk.push_back(pt.diag[pt.col[ik]]);
+        ++na;
+        for (int u = pt.upd_ptr[ik]; u < pt.upd_ptr[ik + 1]; ++u) {
+          auto &v = by_target[pt.upd[2 * u]];
+          v.push_back(ik);
+          v.push_back(pt.upd[2 * u + 1]);
+        }
+      }
+      for (const auto &[slot, v] : by_target) {
+        t_slot.push_back(slot);
+        pairs.insert(pairs.end(), v.begin(), v.end());
+        t_pair_ptr.push_back(int(pairs.size() / 2));
+      }
+    }
+    a_ptr.push_back(int(a_ik.size()));
+    t_ptr.push_back(int(t_slot.size()));
+    widest = std::max(widest, independent ? na : last - first);
+  }
+  split_factor = independent;
+  factor_scratch.resize(size_t(std::max(widest, 1)) * block * block);
+  if (!split_factor) { return; }
+  auto up = [&](DeviceArray<int> &d, const std::vector<int> &h) {
+    d.resize(std::max<size_t>(h.size(), 1));
+    if (!h.empty()) {
+      hip_check(hipMemcpyAsync(d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, p.stream), "upload");
+    }
+  };
+  up(d_a_ik, a_ik);
+  up(d_a_kk, a_kk);
+  up(d_t_slot, t_slot);
+  up(d_t_pair_ptr, t_pair_ptr);
+  up(d_pairs, pairs);
+}
+
 // Fill and factor the preconditioner at the current Newton state
 // (src/linear.cpp:182-203; src/partial.cpp:140-173,265-298;
 // src/operators.cpp:48-74,269-303; src/block_ilu.cpp:73-86).
@@ -133,8 +188,16 @@ void LinearSolver::Setup() {
     if (!df) {
       for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
         const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
-        eg::launch_ilu_factor_level(blocks, b, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col,
-                                    pt.d_upd_ptr, pt.d_upd, factor_scratch.p, p.flags, p.stream);
+        if (split_factor) {
+          const int a0 = a_ptr[l], t0 = t_ptr[l];
+          eg::launch_ilu_factor_level_split(blocks, b, a_ptr[l + 1] - a0, d_a_ik.p + a0, d_a_kk.p + a0,
+                                            t_ptr[l + 1] - t0, d_t_slot.p + t0, d_t_pair_ptr.p + t0, d_pairs.p,
+                                            pt.d_lower_diag.p + first, count, factor_scratch.p, p.flags, p.stream);
+          continue;
+        }
+        eg::launch_ilu_factor_level(blocks, b, pt.d_lower_rows.p + first, pt.d_lower_diag.p + first, count,
+                                    pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr, pt.d_upd, factor_scratch.p, p.flags,
+                                    p.stream);
       }
     }
   } else {

@@ -546,6 +546,9 @@ BlockPattern::BlockPattern(const std::vector<std::vector<int>> &graph, int size,
   up(d_upd, upd);
   up(d_lower_rows, lower_rows);
   up(d_upper_rows, upper_rows);
+  std::vector<int> lower_diag(lower_rows.size());
+  for (size_t t = 0; t < lower_rows.size(); ++t) { lower_diag[t] = diag[lower_rows[t]]; }
+  up(d_lower_diag, lower_diag);
   hip_check(hipStreamSynchronize(s), "sync");
 }
 int BlockPattern::Find(int r, int c) const {
