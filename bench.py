@@ -93,6 +93,103 @@ def measured_traffic(kernel: str, workload: str) -> dict | None:
             "fetch_scale": round(scale["FETCH_SIZE"], 4), "write_scale": round(scale["WRITE_SIZE"], 4)}
 
 
+SHOCK_TIME = 1.0 / 3.141592653589793  # 4/((gamma+1) pi) at gamma = 3 (src/stepper.cpp:94-95)
+
+
+def wave_dt(nx, ny, alpha_deg, cfl):
+    """dt = CFL h / a_max of the simple wave (v + c = 2, HLL safety 1.1) on
+    the rotated periodic rectangle [-1/cos a, 1/cos a] x [-1/sin a, 1/sin a]."""
+    import math
+    a = math.radians(alpha_deg)
+    h = min(2 / math.cos(a) / nx, 2 / math.sin(a) / ny)
+    return cfl * h / (1.1 * 2.0)
+
+
+def batched_cases(count, seed=0xE1E12D):
+    """Independent HDG mixed-precision instances: nx, ny ~ U{8..48}, k ~
+    U{1..6}, alpha ~ U(10, 80) deg, CFL ~ U[0.5, 2], 2 DIRK steps x 3 Newton
+    updates; draws whose two steps would pass the shock time are redrawn
+    (seeded)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    cases = []
+    while len(cases) < count:
+        nx, ny = (int(v) for v in rng.integers(8, 49, size=2))
+        k = int(rng.integers(1, 7))
+        alpha = float(rng.uniform(10, 80))
+        cfl = float(rng.uniform(0.5, 2.0))
+        if 2 * wave_dt(nx, ny, alpha, cfl) >= SHOCK_TIME:
+            continue
+        cases.append(f"--nx {nx} --ny {ny} --k {k} --alpha-deg {alpha:.6f} --cfl {cfl:.6f} --steps 2 "
+                     f"--newton-steps 3 --precond block-ilu0")
+    return cases
+
+
+def served_cases(count, seed=0x5E12ED, max_n=32, max_k=6):
+    """Solve requests: nx = ny log-uniform in [8, max_n], k ~ U{0..max_k},
+    alpha ~ U(5, 85) deg, final time T ~ U[0.01, 0.05], CFL 1, Newton to
+    convergence (tol 1e-12, at most 30), DG or HDG 50/50."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    cases = []
+    for _ in range(count):
+        n = int(round(float(np.exp(rng.uniform(np.log(8), np.log(max_n))))))
+        k = int(rng.integers(0, max_k + 1))
+        alpha = float(rng.uniform(5, 85))
+        T = float(rng.uniform(0.01, 0.05))
+        form = "dg" if rng.uniform() < 0.5 else "hdg"
+        cases.append((f"--nx {n} --ny {n} --k {k} --alpha-deg {alpha:.6f} --cfl 1 --T {T:.6f} "
+                      f"--precond block-ilu0", form))
+    return cases
+
+
+def run_batched(count, workers):
+    """The batched form (many independent HDG mixed-precision instances on
+    concurrent streams, ILU schedule per k from the autotuner) against the
+    same instances run one at a time in plain FP64."""
+    import numpy as np
+    import mfem_euler2d_cpp_amd as eg
+    cases = batched_cases(count)
+    ks = sorted({int(c.split("--k ")[1].split()[0]) for c in cases})
+    pick, times = eg.autotune_ilu(ks, form="hdg", precision="mixed")
+    mixed = [(c + f" --precision mixed --ilu-schedule {pick[int(c.split('--k ')[1].split()[0])]}", "hdg", 0.)
+             for c in cases]
+    out_b, rep_b, _ = eg.serve(mixed, workers=workers)
+    plain = [(c, "hdg", 0.) for c in cases]
+    out_p, rep_p, _ = eg.serve(plain, workers=1)
+    ok_b = [o for o in out_b if o["status"] == 0]
+    ok_p = [o for o in out_p if o["status"] == 0]
+    upd_b, upd_p = sum(o["updates"] for o in ok_b), sum(o["updates"] for o in ok_p)
+    return {"instances": count, "workers": workers, "completed": len(ok_b), "failed_plain": count - len(ok_p),
+            "updates": upd_b, "updates_per_s": upd_b / rep_b["wall"], "wall_s": rep_b["wall"],
+            "plain_fp64_sequential_updates_per_s": upd_p / rep_p["wall"],
+            "speedup_vs_plain": (upd_b / rep_b["wall"]) / max(upd_p / rep_p["wall"], 1e-30),
+            "max_concurrent": rep_b["max_concurrent"], "ilu_schedule_per_k": pick,
+            "autotune_ms": {k: [round(a, 3), round(b, 3)] for k, (a, b) in times.items()}}
+
+
+def run_served(count, workers, load=0.8, seed=0x5E12ED):
+    """The served form: the requests are first run all at once to measure the
+    capacity (requests/s); then a Poisson stream at `load` x capacity is
+    served with admission by HBM footprint; p50/p99 latency and throughput."""
+    import numpy as np
+    import mfem_euler2d_cpp_amd as eg
+    cases = served_cases(count)
+    burst, rep0, _ = eg.serve([(a, f, 0.) for a, f in cases], workers=workers)
+    capacity = sum(o["status"] == 0 for o in burst) / rep0["wall"]
+    rate = load * capacity
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    out, rep, _ = eg.serve([(a, f, float(t)) for (a, f), t in zip(cases, arrivals)], workers=workers)
+    ok = [o for o in out if o["status"] == 0]
+    lat = np.array([o["latency"] for o in ok])
+    return {"requests": count, "workers": workers, "completed": len(ok), "rate_per_s": rate,
+            "capacity_per_s": capacity, "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "throughput_per_s": len(ok) / rep["wall"],
+            "updates_per_s": sum(o["updates"] for o in ok) / rep["wall"], "max_concurrent": rep["max_concurrent"],
+            "max_reserved_GB": rep["max_reserved"] / 1e9}
+
+
 def only_kernel(args) -> int:
     """Time one operation alone (for profiler counter passes): `copy` is a
     device-to-device copy of --copy-mib MiB used to calibrate the HBM byte
@@ -127,6 +224,10 @@ def main() -> int:
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--copy-mib", type=float, default=256.0)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--batched", type=int, default=64, help="instances of the batched form (0: skip)")
+    ap.add_argument("--served", type=int, default=96, help="requests of the served form (0: skip)")
+    ap.add_argument("--workers", type=int, default=12, help="concurrent solves of the batched/served forms")
+    ap.add_argument("--extras-only", action="store_true", help="run only the batched/served forms")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: independent solves shard across GPUs without any exchange, "
@@ -142,6 +243,10 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.only_kernel:
         return only_kernel(args)
+    if args.extras_only:
+        print(json.dumps({"batched": run_batched(args.batched, args.workers) if args.batched else None,
+                          "served": run_served(args.served, args.workers) if args.served else None}))
+        return 0
     ctx = eg.Context(args.workload)
 
     # warm-up (also JIT/first-touch of every kernel)

@@ -127,6 +127,45 @@ int eg_run(eg_ctx *ctx, int form, const char *prefix, double *host_u_final, eg_r
  * (condensation and preconditioner setup, LU-based counts), else 0. */
 int eg_kernel_time(eg_ctx *ctx, const char *kernel, int reps, double *avg_ms, double *bytes, double *flops);
 
+/* ---------------------------------------------------------------- many solves
+ * No counterpart in the reference, which runs one case per process
+ * (src/main.cpp:17-29); these batch and serve independent whole solves
+ * (Run, src/stepper.cpp:28-292), each on its own stream and context. */
+typedef struct {
+  double arrival, start, finish; /* seconds from the start of eg_serve */
+  double newton_seconds;         /* timed Newton updates (src/stepper.cpp:117,226) */
+  double footprint;              /* device bytes reserved for admission (eg_solve_footprint) */
+  double measured_peak;          /* device bytes the solve actually held at its peak */
+  double errors[4];              /* L2 errors against the exact wave (Problem::Errors) */
+  int status;                    /* 0 done; 1 failed or rejected (message in errors buffer) */
+  int updates, time_steps, worker;
+} eg_solve_outcome;
+typedef struct {
+  double wall;         /* seconds from the first arrival window to the last result */
+  double max_reserved; /* largest sum of admitted footprints, bytes */
+  int max_concurrent;  /* most solves on the device at once */
+} eg_serve_report;
+/* Upper bound of the device bytes one solve (args, form 0 dg / 1 hdg) holds. */
+int eg_solve_footprint(const char *args, int form, double *bytes);
+/* Runs n independent solves: args[i] a reference command line, forms[i]
+ * 0 (dg) or 1 (hdg), arrivals[i] seconds after the call (NULL: all at once).
+ * Requests are admitted in arrival order when one of `workers` slots is free
+ * and their footprint fits in hbm_budget bytes (0: 80% of the free device
+ * memory) next to the solves already running; results overlap on the device.
+ * out[n] receives the outcomes; states (optional) n host pointers that
+ * receive each final state (nu doubles of that case, or NULL to skip);
+ * errors (optional) n * 256 chars for failure messages. Returns 0 when the
+ * loop ran; per-request failures are in out[i].status. */
+int eg_serve(const char *const *args, const int *forms, const double *arrivals, int n, int workers,
+             double hbm_budget, eg_solve_outcome *out, double *const *states, char *errors,
+             eg_serve_report *report);
+/* ILU(0) schedule autotuner: for each k in ks, times the preconditioner
+ * setup plus 16 applications (form 0 dg / 1 hdg, precision "fp64" or
+ * "mixed") on a 16x16 case of that k under the "levels" and the "dataflow"
+ * schedule and writes the faster name into out + 16*i (NUL-terminated);
+ * times (optional, 2*nk) receives the two measured times in ms. */
+int eg_autotune_ilu(const int *ks, int nk, int form, const char *precision, char *out, double *times);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,17 @@ class RunStats(ctypes.Structure):
                 ("errors", ctypes.c_double * 4)]
 
 
+class SolveOutcome(ctypes.Structure):
+    _fields_ = [("arrival", ctypes.c_double), ("start", ctypes.c_double), ("finish", ctypes.c_double),
+                ("newton_seconds", ctypes.c_double), ("footprint", ctypes.c_double),
+                ("measured_peak", ctypes.c_double), ("errors", ctypes.c_double * 4), ("status", ctypes.c_int),
+                ("updates", ctypes.c_int), ("time_steps", ctypes.c_int), ("worker", ctypes.c_int)]
+
+
+class ServeReport(ctypes.Structure):
+    _fields_ = [("wall", ctypes.c_double), ("max_reserved", ctypes.c_double), ("max_concurrent", ctypes.c_int)]
+
+
 _SIGNATURES = {
     "eg_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, _vp, ctypes.POINTER(_vp)]),
     "eg_destroy": (None, [_vp]),
@@ -70,6 +81,12 @@ _SIGNATURES = {
     "eg_kernel_time": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p]),
     "eg_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, _c_double_p, ctypes.POINTER(RunStats),
                               _c_double_p, ctypes.c_int, _c_int_p]),
+    "eg_solve_footprint": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _c_double_p]),
+    "eg_serve": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), _c_int_p, _c_double_p, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_double, ctypes.POINTER(SolveOutcome), ctypes.POINTER(_c_double_p),
+                                ctypes.c_char_p, ctypes.POINTER(ServeReport)]),
+    "eg_autotune_ilu": (ctypes.c_int, [_c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
+                                       _c_double_p]),
 }
 
 _lib = None
@@ -230,3 +247,75 @@ class Context:
                   ctypes.byref(st), inc.ctypes.data_as(_c_double_p) if inc is not None else None, increments,
                   kry if increments else None)
         return u, st, inc, list(kry)[:increments]
+
+
+def body_size(args: str) -> int:
+    """nu = nx ny 4 (k+1)^2 of a command line (reference defaults nx = ny =
+    100, k = 1, src/options.cpp)."""
+    tok = args.split()
+    val = {"--nx": 100, "--ny": 100, "--k": 1}
+    for i, t in enumerate(tok[:-1]):
+        if t in val:
+            val[t] = int(tok[i + 1])
+    return val["--nx"] * val["--ny"] * 4 * (val["--k"] + 1) ** 2
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(load().eg_last_error(None).decode())
+
+
+def solve_footprint(args: str, form) -> float:
+    """Device bytes one solve of `args` (form "dg"/"hdg") holds at most."""
+    out = ctypes.c_double()
+    _check(load().eg_solve_footprint(args.encode(), 1 if form in (1, "hdg") else 0, ctypes.byref(out)))
+    return out.value
+
+
+def serve(requests, workers=8, hbm_budget=0.0, keep_states=False):
+    """Run independent solves concurrently (eg_serve). requests: list of
+    (args, form, arrival_seconds). Returns (outcomes, report, states): one
+    dict per request in request order, the serve report, and the final states
+    (numpy arrays, or None) when keep_states."""
+    import numpy as np
+    lib = load()
+    n = len(requests)
+    args = (ctypes.c_char_p * n)(*[r[0].encode() for r in requests])
+    forms = (ctypes.c_int * n)(*[1 if r[1] in (1, "hdg") else 0 for r in requests])
+    arrivals = (ctypes.c_double * n)(*[float(r[2]) for r in requests])
+    out = (SolveOutcome * n)()
+    errors = ctypes.create_string_buffer(256 * max(n, 1))
+    rep = ServeReport()
+    states = None
+    holders = []
+    if keep_states:
+        ptrs = (_c_double_p * n)()
+        for i, r in enumerate(requests):
+            a = np.zeros(body_size(r[0]))
+            holders.append(a)
+            ptrs[i] = a.ctypes.data_as(_c_double_p)
+        states = ptrs
+    _check(lib.eg_serve(args, forms, arrivals, n, workers, hbm_budget, out, states, errors, ctypes.byref(rep)))
+    res = []
+    for i in range(n):
+        o = out[i]
+        res.append({"arrival": o.arrival, "start": o.start, "finish": o.finish, "latency": o.finish - o.arrival,
+                    "newton_seconds": o.newton_seconds, "footprint": o.footprint, "measured_peak": o.measured_peak,
+                    "errors": list(o.errors), "status": o.status, "updates": o.updates,
+                    "time_steps": o.time_steps, "worker": o.worker,
+                    "error": errors.raw[256 * i:256 * (i + 1)].split(b"\0", 1)[0].decode()})
+    report = {"wall": rep.wall, "max_reserved": rep.max_reserved, "max_concurrent": rep.max_concurrent}
+    return res, report, (holders if keep_states else None)
+
+
+def autotune_ilu(ks, form="hdg", precision="fp64"):
+    """Per-k ILU(0) schedule ("levels" or "dataflow") measured on the device
+    (eg_autotune_ilu): returns ({k: schedule}, {k: (ms levels, ms dataflow)})."""
+    lib = load()
+    ks = list(ks)
+    arr = (ctypes.c_int * len(ks))(*ks)
+    out = ctypes.create_string_buffer(16 * len(ks))
+    times = (ctypes.c_double * (2 * len(ks)))()
+    _check(lib.eg_autotune_ilu(arr, len(ks), 1 if form in (1, "hdg") else 0, precision.encode(), out, times))
+    pick = {k: out.raw[16 * i:16 * (i + 1)].split(b"\0", 1)[0].decode() for i, k in enumerate(ks)}
+    return pick, {k: (times[2 * i], times[2 * i + 1]) for i, k in enumerate(ks)}

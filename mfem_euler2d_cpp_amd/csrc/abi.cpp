@@ -308,4 +308,84 @@ int eg_run(eg_ctx *ctx, int form, const char *prefix, double *host_u_final, eg_r
     if (krylov) { for (int i = 0; i < nrec; ++i) { krylov[i] = r.records[i].krylov; } }
   });
 }
+
+int eg_solve_footprint(const char *args, int form, double *bytes) {
+  return guard(nullptr, [&] {
+    require(form == 0 || form == 1, "form must be 0 (dg) or 1 (hdg)");
+    *bytes = double(SolveFootprint(ParseOptions(std::string(args ? args : "")), form));
+  });
+}
+int eg_serve(const char *const *args, const int *forms, const double *arrivals, int n, int workers,
+             double hbm_budget, eg_solve_outcome *out, double *const *states, char *errors,
+             eg_serve_report *report) {
+  return guard(nullptr, [&] {
+    require(n >= 0 && args && forms && out, "eg_serve needs args, forms and out");
+    std::vector<SolveRequest> reqs(n);
+    for (int i = 0; i < n; ++i) {
+      reqs[i].args = args[i] ? args[i] : "";
+      reqs[i].form = forms[i];
+      reqs[i].arrival = arrivals ? arrivals[i] : 0.;
+    }
+    ServeConfig cfg;
+    cfg.workers = workers;
+    cfg.hbm_budget = hbm_budget;
+    cfg.keep_states = states != nullptr;
+    ServeReport rep;
+    auto res = Serve(reqs, cfg, &rep);
+    for (int i = 0; i < n; ++i) {
+      const auto &r = res[i];
+      eg_solve_outcome o{};
+      o.arrival = r.arrival;
+      o.start = r.start;
+      o.finish = r.finish;
+      o.newton_seconds = r.newton_seconds;
+      o.footprint = double(r.footprint);
+      o.measured_peak = double(r.measured_peak);
+      for (int c = 0; c < 4; ++c) { o.errors[c] = r.errors[c]; }
+      o.status = r.status;
+      o.updates = r.updates;
+      o.time_steps = r.time_steps;
+      o.worker = r.worker;
+      out[i] = o;
+      if (states && states[i] && !r.u.empty()) { std::memcpy(states[i], r.u.data(), sizeof(double) * r.u.size()); }
+      if (errors) {
+        std::strncpy(errors + size_t(i) * 256, r.error.c_str(), 255);
+        errors[size_t(i) * 256 + 255] = 0;
+      }
+    }
+    if (report) {
+      report->wall = rep.wall;
+      report->max_reserved = double(rep.max_reserved);
+      report->max_concurrent = rep.max_concurrent;
+    }
+  });
+}
+int eg_autotune_ilu(const int *ks, int nk, int form, const char *precision, char *out, double *times) {
+  return guard(nullptr, [&] {
+    require(form == 0 || form == 1, "form must be 0 (dg) or 1 (hdg)");
+    const std::string prec = precision ? precision : "fp64";
+    const char *names[2] = {"levels", "dataflow"};
+    for (int i = 0; i < nk; ++i) {
+      double best = 0.;
+      int pick = 0;
+      for (int c = 0; c < 2; ++c) {
+        const std::string args = "--nx 16 --ny 16 --k " + std::to_string(ks[i]) +
+                                 " --cfl 1 --precond block-ilu0 --precision " + prec + " --ilu-schedule " + names[c];
+        eg_ctx *ctx = nullptr;
+        if (eg_create(args.c_str(), 3., nullptr, &ctx) != 0) { throw std::runtime_error(create_error); }
+        std::unique_ptr<eg_ctx> hold(ctx);
+        double setup = 0., apply = 0., bytes = 0.;
+        if (eg_kernel_time(ctx, form ? "pc_setup_hdg" : "pc_setup_dg", 3, &setup, &bytes, nullptr) != 0 ||
+            eg_kernel_time(ctx, form ? "pc_hdg" : "pc_dg", 16, &apply, &bytes, nullptr) != 0) {
+          throw std::runtime_error(ctx->error);
+        }
+        const double t = setup + 16. * apply;
+        if (times) { times[2 * i + c] = t; }
+        if (c == 0 || t < best) { best = t; pick = c; }
+      }
+      std::strncpy(out + 16 * i, names[pick], 15);
+      out[16 * i + 15] = 0;
+    }
+  });
+}
 }

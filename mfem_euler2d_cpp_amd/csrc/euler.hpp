@@ -5,6 +5,7 @@
 // device pointers; all numerical work of the Newton update runs in the HIP
 // kernels of kernels_cell.hip / kernels_la.hip on the problem's stream.
 #pragma once
+#include <algorithm>
 #include <array>
 #include <memory>
 #include <stdexcept>
@@ -55,9 +56,31 @@ struct Options {
   std::string ilu_ordering = "none";
   std::string precision = "fp64";
   double inner_tol = 1e-4;
+  // ILU(0) launch schedule: "levels", "dataflow" or "dataflow-all"; empty
+  // takes EG_ILU_SCHEDULE from the environment, else "dataflow".
+  std::string ilu_schedule;
 };
 Options ParseOptions(int argc, char **argv);
 Options ParseOptions(const std::string &args);
+
+// Device memory made by this thread's DeviceArrays: live and peak bytes,
+// and, inside an AllocScope, a stream for stream-ordered allocation, so that
+// a solve that ends frees its memory without a device-wide synchronisation
+// (concurrent solves on other streams keep running).
+struct AllocStats {
+  size_t live = 0, peak = 0;
+  hipStream_t stream = nullptr;
+};
+AllocStats &thread_allocs();
+class AllocScope {
+public:
+  explicit AllocScope(hipStream_t s) : prev_(thread_allocs().stream) { thread_allocs().stream = s; }
+  ~AllocScope() { thread_allocs().stream = prev_; }
+  AllocScope(const AllocScope &) = delete;
+  AllocScope &operator=(const AllocScope &) = delete;
+private:
+  hipStream_t prev_;
+};
 
 template <class T> class DeviceArray {
 public:
@@ -71,15 +94,30 @@ public:
   void resize(size_t count) {
     if (count == n) { return; }
     release();
-    if (count) { hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc"); }
+    if (count) {
+      auto &a = thread_allocs();
+      stream_ = a.stream;
+      if (stream_) { hip_check(hipMallocAsync(reinterpret_cast<void **>(&p), count * sizeof(T), stream_), "hipMallocAsync"); }
+      else { hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc"); }
+      a.live += count * sizeof(T);
+      a.peak = std::max(a.peak, a.live);
+    }
     n = count;
   }
   void release() {
-    if (p) { (void)hipFree(p); }
+    if (p) {
+      if (stream_) { (void)hipFreeAsync(p, stream_); }
+      else { (void)hipFree(p); }
+      auto &a = thread_allocs();
+      a.live -= std::min(a.live, n * sizeof(T));
+    }
     p = nullptr;
     n = 0;
+    stream_ = nullptr;
   }
   operator T *() const { return p; }
+private:
+  hipStream_t stream_ = nullptr;
 };
 
 // Fixed BSR topology of a block ILU(0): pattern, update schedule and the
@@ -224,4 +262,40 @@ struct RunResult {
   std::vector<std::vector<double>> increments;
 };
 RunResult Run(Problem &p, const std::string &form, const std::string &prefix, bool keep_increments = false);
+
+// ------------------------------------------------------------ many solves
+// Batched and served execution of independent solves (no counterpart in the
+// reference, which runs one case per process): each solve is a whole Run on
+// its own stream and context, device-resident from set-up to result.
+size_t SolveFootprint(const Options &o, int form); // device bytes one solve may hold at its peak
+struct SolveRequest {
+  std::string args; // reference command line of the case
+  int form = 1;     // 0 dg, 1 hdg
+  double arrival = 0.; // seconds after the start of Serve
+  double gamma = 3.;
+};
+struct SolveOutcome {
+  int status = 0; // 0 done, 1 failed or rejected (error says why)
+  std::string error;
+  double arrival = 0., start = 0., finish = 0., newton_seconds = 0.;
+  int updates = 0, time_steps = 0, worker = -1;
+  size_t footprint = 0, measured_peak = 0;
+  std::array<double, 4> errors{};
+  std::vector<double> u; // final state when kept
+};
+struct ServeConfig {
+  int workers = 8;         // concurrent solves (one host thread and stream each)
+  double hbm_budget = 0.;  // bytes of device memory the solves may hold; 0: 80% of the free memory
+  bool keep_states = false;
+};
+struct ServeReport {
+  double wall = 0.;
+  int max_concurrent = 0;
+  size_t max_reserved = 0; // largest sum of admitted footprints
+};
+// Requests are admitted in arrival order when a worker is free and their
+// footprint fits in the budget next to the solves already running (a request
+// larger than the whole budget is rejected); results in request order.
+std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const ServeConfig &cfg,
+                                ServeReport *report = nullptr);
 } // namespace euler

@@ -42,7 +42,7 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     // factor and the sweeps), "dataflow" (default: single-launch sweeps,
     // per-level factor) or "dataflow-all" (single-launch factor as well).
     const char *sched = std::getenv("EG_ILU_SCHEDULE");
-    const std::string mode = sched ? sched : "dataflow";
+    const std::string mode = !p.opt.ilu_schedule.empty() ? p.opt.ilu_schedule : (sched ? sched : "dataflow");
     require(mode == "levels" || mode == "dataflow" || mode == "dataflow-all", "Invalid EG_ILU_SCHEDULE");
     dataflow = mode != "levels";
     dataflow_factor = mode == "dataflow-all";

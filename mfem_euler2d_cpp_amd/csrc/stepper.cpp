@@ -299,6 +299,7 @@ Options ParseOptions(int argc, char **argv) {
     else if (s == "--verify") { o.verify = true; }
     else if (s == "--precision") { o.precision = value(); }
     else if (s == "--inner-tol") { o.inner_tol = std::stod(value()); }
+    else if (s == "--ilu-schedule") { o.ilu_schedule = value(); }
     else if (s == "--help") {
       std::cout << "--nx N --ny N --k 0..8 --alpha-deg A (default 45)\n"
                    "--dt DT | --cfl C (CFL wins; h=min(hx,hy), speed includes safety)\n"
@@ -308,7 +309,7 @@ Options ParseOptions(int argc, char **argv) {
                    "--solver gmres --precond block-jacobi|block-ilu0 --ilu-ordering none\n"
                    "--assembly partial --matrix-free --safety 1.1\n"
                    "--linear-tol E --restart N --max-krylov N --output PREFIX\n"
-                   "--precision fp64|mixed --inner-tol E\n"
+                   "--precision fp64|mixed --inner-tol E --ilu-schedule levels|dataflow|dataflow-all\n"
                    "--check-update --timing-only --no-save-solution --hdg-first\n";
       std::exit(EXIT_SUCCESS);
     } else {
@@ -336,6 +337,9 @@ Options ParseOptions(int argc, char **argv) {
   require(o.assembly == "partial", "Invalid assembly");
   require(o.precision == "fp64" || o.precision == "mixed", "Invalid precision");
   require(o.inner_tol > 0 && o.inner_tol < 1, "Invalid inner tolerance");
+  require(o.ilu_schedule.empty() || o.ilu_schedule == "levels" || o.ilu_schedule == "dataflow" ||
+              o.ilu_schedule == "dataflow-all",
+          "Invalid ILU schedule");
   o.matrix_free = true;
   if (o.timing_only) { o.save_solution = false; }
   return o;

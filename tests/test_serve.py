@@ -1,0 +1,75 @@
+"""Batched / served execution of independent solves (eg_serve): results are
+the same alone and in the mix, admission keeps the reserved device memory
+within the budget, and each solve's measured device memory stays within its
+footprint estimate. The footprint estimate itself is host arithmetic and is
+checked on the CPU."""
+import numpy as np
+import pytest
+
+import mfem_euler2d_cpp_amd as eg
+
+MIX = [  # (args, form, arrival s)
+    ("--nx 8 --ny 6 --k 2 --alpha-deg 30 --cfl 1 --steps 1 --newton-steps 2 --precond block-ilu0", "hdg", 0.00),
+    ("--nx 6 --ny 9 --k 3 --alpha-deg 60 --cfl .8 --steps 1 --newton-steps 2 --precond block-jacobi", "dg", 0.01),
+    ("--nx 10 --ny 10 --k 2 --alpha-deg 45 --cfl .5 --steps 2 --newton-steps 2 --precond block-ilu0 "
+     "--precision mixed", "hdg", 0.02),
+    ("--nx 5 --ny 7 --k 4 --alpha-deg 20 --cfl .5 --steps 1 --newton-steps 1 --precond block-ilu0", "dg", 0.03),
+    ("--nx 12 --ny 8 --k 2 --alpha-deg 75 --cfl 1 --steps 1 --newton-steps 2 --precond block-ilu0 "
+     "--ilu-schedule levels", "dg", 0.05),
+    ("--nx 4 --ny 4 --k 5 --alpha-deg 40 --cfl .5 --steps 1 --newton-steps 1 --precond block-jacobi", "hdg", 0.05),
+    ("--nx 9 --ny 9 --k 3 --alpha-deg 50 --cfl 1 --steps 1 --newton-steps 2 --precond block-ilu0 "
+     "--precision mixed", "dg", 0.08),
+    ("--nx 7 --ny 5 --k 0 --alpha-deg 35 --cfl .5 --steps 2 --newton-steps 1 --precond block-ilu0", "hdg", 0.10),
+]
+
+
+def test_footprint_estimate_orders():
+    base = "--nx 16 --ny 16 --k 3 --cfl 1 --precond block-ilu0"
+    f = eg.solve_footprint
+    assert f(base, "dg") < f(base.replace("16 --ny 16", "32 --ny 32"), "dg")
+    assert f(base, "dg") < f(base.replace("--k 3", "--k 4"), "dg")
+    assert f(base, "hdg") < f(base + " --precision mixed", "hdg")
+    assert f(base.replace("block-ilu0", "block-jacobi"), "dg") < f(base, "dg")
+    # the DG ILU blocks alone: 5 cells blocks of 64x64 doubles per cell
+    assert f(base, "dg") > 5 * 256 * 64 * 64 * 8
+    with pytest.raises(RuntimeError, match="Invalid form"):
+        f(base + " --form cg", "dg")
+
+
+@pytest.mark.gpu
+def test_alone_equals_in_mix(gpu):
+    alone = []
+    for args, form, _ in MIX:
+        out, rep, states = eg.serve([(args, form, 0.)], workers=1, keep_states=True)
+        assert out[0]["status"] == 0, out[0]["error"]
+        alone.append((out[0], states[0]))
+    out, rep, states = eg.serve(MIX, workers=8, keep_states=True)
+    print(f"mix: wall {rep['wall']:.3f} s, at most {rep['max_concurrent']} solves at once")
+    assert rep["max_concurrent"] >= 2
+    for i, (o, u) in enumerate(zip(out, states)):
+        assert o["status"] == 0, (i, o["error"])
+        a, ua = alone[i]
+        assert o["updates"] == a["updates"] and o["time_steps"] == a["time_steps"]
+        assert np.array_equal(u, ua), (i, float(np.max(np.abs(u - ua))))
+        assert o["measured_peak"] <= o["footprint"], (i, o["measured_peak"], o["footprint"])
+
+
+@pytest.mark.gpu
+def test_served_matches_single_context_run(gpu):
+    args, form, _ = MIX[0]
+    out, _, states = eg.serve([(args, form, 0.)], workers=2, keep_states=True)
+    u, st, _, _ = gpu.Context(args).run(form)
+    assert np.array_equal(states[0], u) and out[0]["updates"] == st.updates
+
+
+@pytest.mark.gpu
+def test_admission_respects_budget(gpu):
+    fps = [eg.solve_footprint(a, f) for a, f, _ in MIX]
+    budget = 2.2 * max(fps)
+    big = ("--nx 64 --ny 64 --k 6 --cfl 1 --steps 1 --newton-steps 1 --precond block-ilu0", "dg", 0.)
+    out, rep, _ = eg.serve(MIX + [big], workers=6, hbm_budget=budget)
+    assert rep["max_reserved"] <= budget
+    assert rep["max_concurrent"] <= 6
+    assert out[-1]["status"] == 1 and "HBM budget" in out[-1]["error"]
+    for o in out[:-1]:
+        assert o["status"] == 0 and o["start"] >= o["arrival"] - 1e-6
