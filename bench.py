@@ -145,15 +145,17 @@ def served_cases(count, seed=0x5E12ED, max_n=32, max_k=6):
 
 def run_batched(count, workers):
     """The batched form (many independent HDG mixed-precision instances on
-    concurrent streams, ILU schedule per k from the autotuner) against the
-    same instances run one at a time in plain FP64."""
+    concurrent streams) against the same instances run one at a time in plain
+    FP64."""
     import numpy as np
     import mfem_euler2d_cpp_amd as eg
     cases = batched_cases(count)
-    ks = sorted({int(c.split("--k ")[1].split()[0]) for c in cases})
-    pick, times = eg.autotune_ilu(ks, form="hdg", precision="mixed")
-    mixed = [(c + f" --precision mixed --ilu-schedule {pick[int(c.split('--k ')[1].split()[0])]}", "hdg", 0.)
-             for c in cases]
+    kof = lambda c: int(c.split("--k ")[1].split()[0])
+    # concurrent solves run the ILU one launch per level; the autotuner picks
+    # the sweep's threads per block row for each k
+    _, times = eg.autotune_ilu(sorted({kof(c) for c in cases}), form="hdg", precision="mixed")
+    threads = {k: (256 if t["levels-256"] <= t["levels-1024"] else 1024) for k, t in times.items()}
+    mixed = [(c + f" --precision mixed --sweep-threads {threads[kof(c)]}", "hdg", 0.) for c in cases]
     out_b, rep_b, _ = eg.serve(mixed, workers=workers)
     plain = [(c, "hdg", 0.) for c in cases]
     out_p, rep_p, _ = eg.serve(plain, workers=1)
@@ -164,8 +166,8 @@ def run_batched(count, workers):
             "updates": upd_b, "updates_per_s": upd_b / rep_b["wall"], "wall_s": rep_b["wall"],
             "plain_fp64_sequential_updates_per_s": upd_p / rep_p["wall"],
             "speedup_vs_plain": (upd_b / rep_b["wall"]) / max(upd_p / rep_p["wall"], 1e-30),
-            "max_concurrent": rep_b["max_concurrent"], "ilu_schedule_per_k": pick,
-            "autotune_ms": {k: [round(a, 3), round(b, 3)] for k, (a, b) in times.items()}}
+            "max_concurrent": rep_b["max_concurrent"], "sweep_threads_per_k": threads,
+            "autotune_ms": {k: {n: round(v, 3) for n, v in t.items()} for k, t in times.items()}}
 
 
 def run_served(count, workers, load=0.8, seed=0x5E12ED):

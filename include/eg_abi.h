@@ -159,11 +159,14 @@ int eg_solve_footprint(const char *args, int form, double *bytes);
 int eg_serve(const char *const *args, const int *forms, const double *arrivals, int n, int workers,
              double hbm_budget, eg_solve_outcome *out, double *const *states, char *errors,
              eg_serve_report *report);
-/* ILU(0) schedule autotuner: for each k in ks, times the preconditioner
+/* ILU(0) launch-shape autotuner: for each k in ks, times the preconditioner
  * setup plus 16 applications (form 0 dg / 1 hdg, precision "fp64" or
- * "mixed") on a 16x16 case of that k under the "levels" and the "dataflow"
- * schedule and writes the faster name into out + 16*i (NUL-terminated);
- * times (optional, 2*nk) receives the two measured times in ms. */
+ * "mixed") on a 16x16 case of that k under three shapes: "levels-256" and
+ * "levels-1024" (one launch per dependency level, 256 or 1024 threads per
+ * block row of the sweeps) and "dataflow" (single cooperative launch per
+ * sweep, for a device not shared with other solves). out + 16*i receives the
+ * fastest name (NUL-terminated); times (optional, 3*nk) the three times in
+ * ms, in that order. */
 int eg_autotune_ilu(const int *ks, int nk, int form, const char *precision, char *out, double *times);
 
 #ifdef __cplusplus

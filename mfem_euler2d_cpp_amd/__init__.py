@@ -308,14 +308,17 @@ def serve(requests, workers=8, hbm_budget=0.0, keep_states=False):
     return res, report, (holders if keep_states else None)
 
 
+AUTOTUNE_SHAPES = ("levels-256", "levels-1024", "dataflow")
+
+
 def autotune_ilu(ks, form="hdg", precision="fp64"):
-    """Per-k ILU(0) schedule ("levels" or "dataflow") measured on the device
-    (eg_autotune_ilu): returns ({k: schedule}, {k: (ms levels, ms dataflow)})."""
+    """Per-k ILU(0) launch shape measured on the device (eg_autotune_ilu):
+    returns ({k: fastest shape}, {k: {shape: ms}})."""
     lib = load()
     ks = list(ks)
     arr = (ctypes.c_int * len(ks))(*ks)
     out = ctypes.create_string_buffer(16 * len(ks))
-    times = (ctypes.c_double * (2 * len(ks)))()
+    times = (ctypes.c_double * (3 * len(ks)))()
     _check(lib.eg_autotune_ilu(arr, len(ks), 1 if form in (1, "hdg") else 0, precision.encode(), out, times))
     pick = {k: out.raw[16 * i:16 * (i + 1)].split(b"\0", 1)[0].decode() for i, k in enumerate(ks)}
-    return pick, {k: (times[2 * i], times[2 * i + 1]) for i, k in enumerate(ks)}
+    return pick, {k: dict(zip(AUTOTUNE_SHAPES, times[3 * i:3 * i + 3])) for i, k in enumerate(ks)}

@@ -364,13 +364,15 @@ int eg_autotune_ilu(const int *ks, int nk, int form, const char *precision, char
   return guard(nullptr, [&] {
     require(form == 0 || form == 1, "form must be 0 (dg) or 1 (hdg)");
     const std::string prec = precision ? precision : "fp64";
-    const char *names[2] = {"levels", "dataflow"};
+    const char *names[3] = {"levels-256", "levels-1024", "dataflow"};
+    const char *flags[3] = {"--ilu-schedule levels --sweep-threads 256", "--ilu-schedule levels --sweep-threads 1024",
+                            "--ilu-schedule dataflow"};
     for (int i = 0; i < nk; ++i) {
       double best = 0.;
       int pick = 0;
-      for (int c = 0; c < 2; ++c) {
+      for (int c = 0; c < 3; ++c) {
         const std::string args = "--nx 16 --ny 16 --k " + std::to_string(ks[i]) +
-                                 " --cfl 1 --precond block-ilu0 --precision " + prec + " --ilu-schedule " + names[c];
+                                 " --cfl 1 --precond block-ilu0 --precision " + prec + " " + flags[c];
         eg_ctx *ctx = nullptr;
         if (eg_create(args.c_str(), 3., nullptr, &ctx) != 0) { throw std::runtime_error(create_error); }
         std::unique_ptr<eg_ctx> hold(ctx);
@@ -380,7 +382,7 @@ int eg_autotune_ilu(const int *ks, int nk, int form, const char *precision, char
           throw std::runtime_error(ctx->error);
         }
         const double t = setup + 16. * apply;
-        if (times) { times[2 * i + c] = t; }
+        if (times) { times[3 * i + c] = t; }
         if (c == 0 || t < best) { best = t; pick = c; }
       }
       std::strncpy(out + 16 * i, names[pick], 15);

@@ -59,6 +59,8 @@ struct Options {
   // ILU(0) launch schedule: "levels", "dataflow" or "dataflow-all"; empty
   // takes EG_ILU_SCHEDULE from the environment, else "dataflow".
   std::string ilu_schedule;
+  // Threads per row of the per-level ILU sweeps: 256 or 1024 (0: by block size).
+  int sweep_threads = 0;
 };
 Options ParseOptions(int argc, char **argv);
 Options ParseOptions(const std::string &args);

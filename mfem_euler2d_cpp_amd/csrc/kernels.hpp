@@ -60,12 +60,14 @@ void launch_ilu_factor_level_split(double *bsr, int b, int na, const int *a_ik, 
 // Blocks handled by the global-memory large-block kernels, and their limit.
 bool big_block(int b);
 int big_block_max();
+// Per-level sweeps; threads per row 256 or 1024 (0: 256 up to 64 x 64
+// blocks, else 1024).
 template <class T>
 void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
-                            const int *diag_ptr, const int *col, const T *rhs, T *w, hipStream_t s);
+                            const int *diag_ptr, const int *col, const T *rhs, T *w, int threads, hipStream_t s);
 template <class T>
 void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
-                            const int *diag_ptr, const int *col, T *w, hipStream_t s);
+                            const int *diag_ptr, const int *col, T *w, int threads, hipStream_t s);
 
 // Register-resident / split-dot variants (kernels_ilu.hip) used by the
 // launchers above when the block size allows.
@@ -75,13 +77,6 @@ int ilu_factor_lds_init();
 void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s);
 void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                             const int *col, const int *upd_ptr, const int *upd, int *flags, hipStream_t s);
-template <class T>
-void launch_ilu_lower_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
-                           const int *col, const T *rhs, T *w, hipStream_t s);
-template <class T>
-void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
-                           const int *col, T *w, hipStream_t s);
-
 // Single-launch dataflow forms (cooperative grid, per-row completion flags);
 // return false when a co-resident launch is not possible (caller falls back
 // to the per-level launches).

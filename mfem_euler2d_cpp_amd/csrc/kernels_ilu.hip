@@ -660,16 +660,19 @@ void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, cons
   EG_DISPATCH_B(b, (k_ilu_factor_rows<B><<<nrows, Shape<B>::NT, factor_lds<B>(), s>>>(bsr, rows, row_ptr, diag_ptr,
                                                                                        col, upd_ptr, upd, flags)));
 }
+// threads: 256 or 1024 per row (0: 256 up to 64 x 64 blocks, else 1024).
 template <class T>
-void launch_ilu_lower_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
-                           const int *col, const T *rhs, T *w, hipStream_t s) {
-  if (b <= 64) { k_ilu_lower_split<T, 256><<<nrows, 256, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w); }
+void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
+                            const int *diag_ptr, const int *col, const T *rhs, T *w, int threads, hipStream_t s) {
+  if (threads == 0) { threads = b <= 64 ? 256 : 1024; }
+  if (threads == 256 && b <= 256) { k_ilu_lower_split<T, 256><<<nrows, 256, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w); }
   else { k_ilu_lower_split<T, 1024><<<nrows, 1024, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, rhs, w); }
 }
 template <class T>
-void launch_ilu_upper_fast(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
-                           const int *col, T *w, hipStream_t s) {
-  if (b <= 64) { k_ilu_upper_split<T, 256><<<nrows, 256, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w); }
+void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
+                            const int *diag_ptr, const int *col, T *w, int threads, hipStream_t s) {
+  if (threads == 0) { threads = b <= 64 ? 256 : 1024; }
+  if (threads == 256 && b <= 256) { k_ilu_upper_split<T, 256><<<nrows, 256, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w); }
   else { k_ilu_upper_split<T, 1024><<<nrows, 1024, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w); }
 }
 template <class T>
@@ -720,14 +723,14 @@ bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const 
   return ok;
 }
 
-template void launch_ilu_lower_fast<double>(const double *, int, const int *, int, const int *, const int *,
-                                            const int *, const double *, double *, hipStream_t);
-template void launch_ilu_lower_fast<float>(const float *, int, const int *, int, const int *, const int *,
-                                           const int *, const float *, float *, hipStream_t);
-template void launch_ilu_upper_fast<double>(const double *, int, const int *, int, const int *, const int *,
-                                            const int *, double *, hipStream_t);
-template void launch_ilu_upper_fast<float>(const float *, int, const int *, int, const int *, const int *,
-                                           const int *, float *, hipStream_t);
+template void launch_ilu_lower_level<double>(const double *, int, const int *, int, const int *, const int *,
+                                             const int *, const double *, double *, int, hipStream_t);
+template void launch_ilu_lower_level<float>(const float *, int, const int *, int, const int *, const int *,
+                                            const int *, const float *, float *, int, hipStream_t);
+template void launch_ilu_upper_level<double>(const double *, int, const int *, int, const int *, const int *,
+                                             const int *, double *, int, hipStream_t);
+template void launch_ilu_upper_level<float>(const float *, int, const int *, int, const int *, const int *,
+                                            const int *, float *, int, hipStream_t);
 template bool launch_ilu_sweep_df<double>(bool, const double *, int, int, int, const int *, const int *, const int *,
                                           double *, double *, double *, int *, hipStream_t);
 template bool launch_ilu_sweep_df<float>(bool, const float *, int, int, int, const int *, const int *, const int *,

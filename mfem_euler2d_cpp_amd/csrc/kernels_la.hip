@@ -494,24 +494,6 @@ void launch_ilu_factor_level_split(double *bsr, int b, int na, const int *a_ik, 
 }
 bool big_block(int b) { return !fast_dense_ok(b); }
 int big_block_max() { return kBigMax; }
-template <class T>
-void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
-                            const int *diag_ptr, const int *col, const T *rhs, T *w, hipStream_t s) {
-  launch_ilu_lower_fast<T>(bsr, b, rows, nrows, row_ptr, diag_ptr, col, rhs, w, s);
-}
-template <class T>
-void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
-                            const int *diag_ptr, const int *col, T *w, hipStream_t s) {
-  launch_ilu_upper_fast<T>(bsr, b, rows, nrows, row_ptr, diag_ptr, col, w, s);
-}
-template void launch_ilu_lower_level<double>(const double *, int, const int *, int, const int *, const int *,
-                                             const int *, const double *, double *, hipStream_t);
-template void launch_ilu_lower_level<float>(const float *, int, const int *, int, const int *, const int *,
-                                            const int *, const float *, float *, hipStream_t);
-template void launch_ilu_upper_level<double>(const double *, int, const int *, int, const int *, const int *,
-                                             const int *, double *, hipStream_t);
-template void launch_ilu_upper_level<float>(const float *, int, const int *, int, const int *, const int *,
-                                            const int *, float *, hipStream_t);
 
 template <class T>
 void launch_multidot(const T *V, long ldv, int nvec, const T *w, const T *metric, long n, double *partials,

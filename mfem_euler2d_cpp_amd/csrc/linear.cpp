@@ -241,12 +241,12 @@ template <class T> static void precondition(LinearSolver &s, const T *bsr, const
   for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
     const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
     eg::launch_ilu_lower_level<T>(bsr, s.block, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col, b,
-                                  y, s.p.stream);
+                                  y, s.p.opt.sweep_threads, s.p.stream);
   }
   for (size_t l = 0; l + 1 < pt.upper_level_ptr.size(); ++l) {
     const int first = pt.upper_level_ptr[l], count = pt.upper_level_ptr[l + 1] - first;
     eg::launch_ilu_upper_level<T>(bsr, s.block, pt.d_upper_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col, y,
-                                  s.p.stream);
+                                  s.p.opt.sweep_threads, s.p.stream);
   }
 }
 void LinearSolver::Precondition(const double *b, double *y) { precondition<double>(*this, blocks, b, y); }
@@ -371,6 +371,7 @@ double LinearSolver::MixedSolve(const double *rhs, double tol, int limit, int &i
   if (rhs_norm == 0.) { return 0.; }
   double residual_norm = zero_guess ? rhs_norm : Norm(res);
   while (residual_norm > tol * rhs_norm && iterations < limit) {
+    const double before = residual_norm;
     eg::launch_convert<double, float>(res, b32, N, s);
     int inner = 0;
     double inner_norm = 0.;
@@ -385,6 +386,16 @@ double LinearSolver::MixedSolve(const double *rhs, double tol, int limit, int &i
     residual_norm = Norm(res);
     require(std::isfinite(residual_norm), "Nonfinite GMRES residual");
     if (inner == 0) { break; }
+    if (residual_norm > 0.5 * before && residual_norm > tol * rhs_norm) {
+      // The FP32 inner solves stall (the system is too ill-conditioned for
+      // single precision): finish in FP64 from the current iterate.
+      int rest = 0;
+      double true_norm = 0.;
+      GMRES<double>(rhs, x.p, tol, limit - iterations, rest, false, &true_norm);
+      iterations += rest;
+      ++outer;
+      return true_norm;
+    }
   }
   return residual_norm;
 }

@@ -229,11 +229,10 @@ Problem::Problem(const Options &o, const EulerGas &g, hipStream_t s)
   Sync();
 }
 Problem::~Problem() {
+  // Copies into the pinned buffer may still be queued on the stream.
+  if (stream) { (void)hipStreamSynchronize(stream); }
   if (pinned) { (void)hipHostFree(pinned); }
-  if (own_stream && stream) {
-    (void)hipStreamSynchronize(stream);
-    (void)hipStreamDestroy(stream);
-  }
+  if (own_stream && stream) { (void)hipStreamDestroy(stream); }
 }
 void Problem::Sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
 double Problem::ReduceScalar(const double *dev_partials, int count, int mode) {

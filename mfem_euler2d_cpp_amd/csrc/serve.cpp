@@ -69,6 +69,11 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
     try {
       require(requests[i].form == 0 || requests[i].form == 1, "form must be 0 (dg) or 1 (hdg)");
       opts[i] = ParseOptions(requests[i].args);
+      // The single-launch (cooperative) ILU schedules assume that their grid
+      // is resident on the whole device at once, which concurrent solves on
+      // other streams break: solves that share the device run the factor and
+      // the sweeps one launch per dependency level.
+      opts[i].ilu_schedule = "levels";
       out[i].footprint = SolveFootprint(opts[i], requests[i].form);
       if (double(out[i].footprint) > budget) {
         out[i].status = 1;
@@ -175,6 +180,14 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
   }
   wake_work.notify_all();
   for (auto &th : pool) { th.join(); }
+  // Give the memory the solves cached in the stream-ordered pool back.
+  int dev = 0;
+  hipMemPool_t mempool = nullptr;
+  if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&mempool, dev) == hipSuccess) {
+    (void)hipDeviceSynchronize();
+    (void)hipMemPoolTrimTo(mempool, 0);
+  }
+  (void)hipGetLastError();
   if (report) {
     report->wall = now();
     report->max_concurrent = max_running;

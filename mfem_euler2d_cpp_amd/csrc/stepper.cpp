@@ -300,6 +300,7 @@ Options ParseOptions(int argc, char **argv) {
     else if (s == "--precision") { o.precision = value(); }
     else if (s == "--inner-tol") { o.inner_tol = std::stod(value()); }
     else if (s == "--ilu-schedule") { o.ilu_schedule = value(); }
+    else if (s == "--sweep-threads") { o.sweep_threads = std::stoi(value()); }
     else if (s == "--help") {
       std::cout << "--nx N --ny N --k 0..8 --alpha-deg A (default 45)\n"
                    "--dt DT | --cfl C (CFL wins; h=min(hx,hy), speed includes safety)\n"
@@ -309,7 +310,7 @@ Options ParseOptions(int argc, char **argv) {
                    "--solver gmres --precond block-jacobi|block-ilu0 --ilu-ordering none\n"
                    "--assembly partial --matrix-free --safety 1.1\n"
                    "--linear-tol E --restart N --max-krylov N --output PREFIX\n"
-                   "--precision fp64|mixed --inner-tol E --ilu-schedule levels|dataflow|dataflow-all\n"
+                   "--precision fp64|mixed --inner-tol E --ilu-schedule levels|dataflow|dataflow-all --sweep-threads 0|256|1024\n"
                    "--check-update --timing-only --no-save-solution --hdg-first\n";
       std::exit(EXIT_SUCCESS);
     } else {
@@ -340,6 +341,7 @@ Options ParseOptions(int argc, char **argv) {
   require(o.ilu_schedule.empty() || o.ilu_schedule == "levels" || o.ilu_schedule == "dataflow" ||
               o.ilu_schedule == "dataflow-all",
           "Invalid ILU schedule");
+  require(o.sweep_threads == 0 || o.sweep_threads == 256 || o.sweep_threads == 1024, "Invalid sweep threads");
   o.matrix_free = true;
   if (o.timing_only) { o.save_solution = false; }
   return o;

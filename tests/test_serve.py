@@ -58,7 +58,8 @@ def test_alone_equals_in_mix(gpu):
 def test_served_matches_single_context_run(gpu):
     args, form, _ = MIX[0]
     out, _, states = eg.serve([(args, form, 0.)], workers=2, keep_states=True)
-    u, st, _, _ = gpu.Context(args).run(form)
+    # served solves run the ILU(0) one launch per dependency level
+    u, st, _, _ = gpu.Context(args + " --ilu-schedule levels").run(form)
     assert np.array_equal(states[0], u) and out[0]["updates"] == st.updates
 
 
@@ -73,3 +74,37 @@ def test_admission_respects_budget(gpu):
     assert out[-1]["status"] == 1 and "HBM budget" in out[-1]["error"]
     for o in out[:-1]:
         assert o["status"] == 0 and o["start"] >= o["arrival"] - 1e-6
+
+
+@pytest.mark.gpu
+def test_autotune_shapes(gpu):
+    pick, times = eg.autotune_ilu([1, 3], form="hdg", precision="mixed")
+    for k in (1, 3):
+        assert pick[k] in eg.AUTOTUNE_SHAPES
+        assert all(t > 0 for t in times[k].values())
+        assert times[k][pick[k]] == min(times[k].values())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("args", ["--nx 6 --ny 5 --k 3", "--nx 4 --ny 4 --k 5"])
+def test_sweep_threads_same_preconditioner(gpu, oracle_lib, args):
+    import torch
+    from test_gpu_parity import hdg_case, rel
+    res = {}
+    for threads in (256, 1024):
+        full = f"{args} --precond block-ilu0 --ilu-schedule levels --sweep-threads {threads}"
+        o, c, u, du, *_ = hdg_case(gpu, oracle_lib, full)
+        out = []
+        for form, size in ((1, o.nl), (0, o.nu)):
+            if form == 0:
+                c.call("eg_dg_refresh", gpu._ptr(du))
+                o.dg_refresh(u)
+            c.call("eg_pc_setup", form)
+            b = np.sin(0.13 * (np.arange(size) + 1))
+            y = torch.zeros(size, dtype=torch.float64, device="cuda")
+            c.call("eg_pc_apply", form, gpu._ptr(torch.tensor(b, device="cuda")), gpu._ptr(y))
+            out.append(y.cpu().numpy())
+            assert rel(out[-1], o.precondition(form, 1, b)) < 1e-10
+        res[threads] = out
+    for a, b in zip(res[256], res[1024]):
+        assert rel(a, b) < 1e-13
