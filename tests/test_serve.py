@@ -89,16 +89,27 @@ def test_autotune_shapes(gpu):
 @pytest.mark.parametrize("args", ["--nx 6 --ny 5 --k 3", "--nx 4 --ny 4 --k 5"])
 def test_sweep_threads_same_preconditioner(gpu, oracle_lib, args):
     import torch
-    from test_gpu_parity import hdg_case, rel
+    from test_gpu_parity import perturbed, rel
     res = {}
+    base = f"{args} --precond block-ilu0"
+    o = oracle_lib.Oracle(base)  # the oracle has no launch-shape options
+    u = perturbed(o)
+    r = 1e-3 * np.sin(0.29 * (np.arange(o.nu) + 1))
+    o.freeze(u, 1e-3)
+    o.set_stage(u, 0.05)
+    o.condense(u, r)
     for threads in (256, 1024):
-        full = f"{args} --precond block-ilu0 --ilu-schedule levels --sweep-threads {threads}"
-        o, c, u, du, *_ = hdg_case(gpu, oracle_lib, full)
+        c = gpu.Context(f"{base} --ilu-schedule levels --sweep-threads {threads}")
+        du = torch.tensor(u, device="cuda")
+        c.freeze(du, 1e-3)
+        c.set_stage(du, 0.05)
+        c.call("eg_hdg_condense", gpu._ptr(du), gpu._ptr(torch.tensor(r, device="cuda")))
         out = []
         for form, size in ((1, o.nl), (0, o.nu)):
             if form == 0:
                 c.call("eg_dg_refresh", gpu._ptr(du))
-                o.dg_refresh(u)
+                if threads == 256:
+                    o.dg_refresh(u)
             c.call("eg_pc_setup", form)
             b = np.sin(0.13 * (np.arange(size) + 1))
             y = torch.zeros(size, dtype=torch.float64, device="cuda")
