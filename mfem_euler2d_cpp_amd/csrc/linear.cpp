@@ -372,15 +372,21 @@ double LinearSolver::MixedSolve(const double *rhs, double tol, int limit, int &i
   double residual_norm = zero_guess ? rhs_norm : Norm(res);
   while (residual_norm > tol * rhs_norm && iterations < limit) {
     const double before = residual_norm;
-    eg::launch_convert<double, float>(res, b32, N, s);
+    // The correction equation is solved for res / ||res||, so that residuals
+    // near the rounding floor do not underflow in single precision.
+    eg::launch_scale_copy(res, 1. / residual_norm, corr, N, s);
+    eg::launch_convert<double, float>(corr, b32, N, s);
     int inner = 0;
     double inner_norm = 0.;
     const double inner_tol = std::max(p.opt.inner_tol, 0.5 * tol * rhs_norm / residual_norm);
-    GMRES<float>(b32.p, x32.p, inner_tol, limit - iterations, inner, true, &inner_norm);
+    // A few restart cycles per refinement step: an inner solve that needs
+    // more has stalled in single precision (caught below).
+    const int inner_limit = std::min(limit - iterations, 4 * restart);
+    GMRES<float>(b32.p, x32.p, inner_tol, inner_limit, inner, true, &inner_norm);
     iterations += inner;
     ++outer;
     eg::launch_convert<float, double>(x32, corr, N, s);
-    eg::launch_axpby<double>(x, 1., corr, 1., N, s);
+    eg::launch_axpby<double>(x, before, corr, 1., N, s);
     Apply(x.p, act.p);
     eg::launch_sub(rhs, act, res, N, s);
     residual_norm = Norm(res);
