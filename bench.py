@@ -13,6 +13,14 @@ value: mean wall seconds per accepted Newton update over both forms, timed by
 the library with the reference's update timer boundary (src/stepper.cpp:117,
 226). The per-update error against the CPU oracle of the same workload is
 measured in the same run (the oracle run is also the CPU baseline).
+
+The same JSON line also carries, as separate workloads: the mixed-precision
+form at 64x64 Q4 and 128x128 Q6 against plain FP64 ("mixed_precision"), a
+seeded sample of the batched form (independent HDG mixed-precision instances
+on concurrent streams, against the same instances in plain FP64 one at a
+time and concurrently: "batched"), and a seeded Poisson stream of solve
+requests with admission by HBM footprint ("served": p50/p99 latency,
+throughput).
 """
 from __future__ import annotations
 
@@ -158,14 +166,20 @@ def run_batched(count, workers):
     mixed = [(c + f" --precision mixed --sweep-threads {threads[kof(c)]}", "hdg", 0.) for c in cases]
     out_b, rep_b, _ = eg.serve(mixed, workers=workers)
     plain = [(c, "hdg", 0.) for c in cases]
+    out_c, rep_c, _ = eg.serve(plain, workers=workers)
     out_p, rep_p, _ = eg.serve(plain, workers=1)
-    ok_b = [o for o in out_b if o["status"] == 0]
-    ok_p = [o for o in out_p if o["status"] == 0]
-    upd_b, upd_p = sum(o["updates"] for o in ok_b), sum(o["updates"] for o in ok_p)
-    return {"instances": count, "workers": workers, "completed": len(ok_b), "failed_plain": count - len(ok_p),
-            "updates": upd_b, "updates_per_s": upd_b / rep_b["wall"], "wall_s": rep_b["wall"],
-            "plain_fp64_sequential_updates_per_s": upd_p / rep_p["wall"],
-            "speedup_vs_plain": (upd_b / rep_b["wall"]) / max(upd_p / rep_p["wall"], 1e-30),
+    both = [i for i in range(count) if out_b[i]["status"] == 0 and out_p[i]["status"] == 0]
+    upd = lambda out: sum(out[i]["updates"] for i in both)
+    rate = lambda out, rep: upd(out) / rep["wall"]
+    err = [max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(out_b[i]["errors"], out_p[i]["errors"]))
+           for i in both]
+    return {"instances": count, "workers": workers, "completed": len(both),
+            "failed": {"mixed": sum(o["status"] != 0 for o in out_b), "plain": sum(o["status"] != 0 for o in out_p)},
+            "updates": upd(out_b), "updates_per_s": rate(out_b, rep_b), "wall_s": rep_b["wall"],
+            "plain_fp64_sequential_updates_per_s": rate(out_p, rep_p),
+            "fp64_concurrent_updates_per_s": rate(out_c, rep_c),
+            "speedup_vs_plain": rate(out_b, rep_b) / max(rate(out_p, rep_p), 1e-30),
+            "max_rel_diff_L2_error_vs_fp64": max(err) if err else None,
             "max_concurrent": rep_b["max_concurrent"], "sweep_threads_per_k": threads,
             "autotune_ms": {k: {n: round(v, 3) for n, v in t.items()} for k, t in times.items()}}
 
@@ -190,6 +204,38 @@ def run_served(count, workers, load=0.8, seed=0x5E12ED):
             "p99_latency_s": float(np.percentile(lat, 99)), "throughput_per_s": len(ok) / rep["wall"],
             "updates_per_s": sum(o["updates"] for o in ok) / rep["wall"], "max_concurrent": rep["max_concurrent"],
             "max_reserved_GB": rep["max_reserved"] / 1e9}
+
+
+MIXED_CASES = {
+    "64x64 Q4": "--nx 64 --ny 64 --k 4 --alpha-deg 30 --cfl 1 --steps 1 --newton-steps 1 --restart 30 --precond block-ilu0",
+    "128x128 Q6": "--nx 128 --ny 128 --k 6 --alpha-deg 30 --cfl 1 --steps 1 --newton-steps 1 --restart 30 "
+                  "--precond block-ilu0",
+}
+
+
+def run_mixed_sizes():
+    """The mixed-precision form at 64x64 Q4 and 128x128 Q6 (one DIRK step x
+    one Newton update per stage, 4 updates per form) against the plain FP64
+    form on the same cases: seconds per update and the per-update relative
+    difference of the increments (2-norm)."""
+    import numpy as np
+    import mfem_euler2d_cpp_amd as eg
+    res = {}
+    for name, args in MIXED_CASES.items():
+        entry = {}
+        for form in ("dg", "hdg"):
+            runs = {}
+            for prec in ("fp64", "mixed"):
+                c = eg.Context(args + f" --precision {prec}")
+                u, st, inc, kr = c.run(form, increments=4)
+                runs[prec] = (st.newton_seconds / st.updates, inc[:st.updates], list(kr[:st.updates]))
+                del c
+            d = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(runs["mixed"][1], runs["fp64"][1])]
+            entry[form] = {"fp64_s_per_update": runs["fp64"][0], "mixed_s_per_update": runs["mixed"][0],
+                           "speedup": runs["fp64"][0] / runs["mixed"][0], "krylov_fp64": runs["fp64"][2],
+                           "krylov_mixed": runs["mixed"][2], "max_update_rel_diff_vs_fp64": max(d)}
+        res[name] = entry
+    return res
 
 
 def only_kernel(args) -> int:
@@ -226,10 +272,11 @@ def main() -> int:
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--copy-mib", type=float, default=256.0)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
-    ap.add_argument("--batched", type=int, default=64, help="instances of the batched form (0: skip)")
-    ap.add_argument("--served", type=int, default=96, help="requests of the served form (0: skip)")
+    ap.add_argument("--batched", type=int, default=32, help="instances of the batched form (0: skip)")
+    ap.add_argument("--served", type=int, default=64, help="requests of the served form (0: skip)")
     ap.add_argument("--workers", type=int, default=12, help="concurrent solves of the batched/served forms")
     ap.add_argument("--extras-only", action="store_true", help="run only the batched/served forms")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision 64x64 Q4 / 128x128 Q6 runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: independent solves shard across GPUs without any exchange, "
@@ -356,6 +403,9 @@ def main() -> int:
         "error_vs_oracle": err,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed_precision": run_mixed_sizes() if not args.no_mixed else None,
+        "batched": run_batched(args.batched, args.workers) if args.batched else None,
+        "served": run_served(args.served, args.workers) if args.served else None,
     }
     print(json.dumps(line))
     return 0
