@@ -231,10 +231,15 @@ public:
   std::vector<int> a_ptr, t_ptr;
   DeviceArray<int> d_a_ik, d_a_kk, d_t_slot, d_t_pair_ptr, d_pairs;
   bool dataflow = true, dataflow_factor = false;
+  // Row-resident factor of small blocks: per-row programs in level order.
+  bool resident = false;
+  size_t resident_lds = 0;
+  DeviceArray<int> d_prog, d_prog_off;
   std::vector<double> h, cosine, sine, rotated, coefficients;
 
   LinearSolver(Problem &p, int form, Hybrid *hybrid = nullptr);
   void PlanBigFactor();
+  void PlanResident();
   void Refresh(const double *dev_u) { state = dev_u; }
   void Setup();  // fill + factor the preconditioner at the current state
   void Apply(const double *x, double *y);

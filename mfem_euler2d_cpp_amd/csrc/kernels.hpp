@@ -77,7 +77,22 @@ int ilu_factor_lds_init();
 void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s);
 void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                             const int *col, const int *upd_ptr, const int *upd, int *flags, hipStream_t s);
-// Single-launch dataflow forms (cooperative grid, per-row completion flags);
+// Row-resident factor of small blocks (b <= 36): one workgroup per row of a
+// level, driven by a per-row program (prog_off[r] into prog, which is padded
+// by kIluProgMax entries); lds: the largest row's blocks and dependencies
+// (b * b doubles each) plus ilu_resident_gj_lds(b).
+constexpr int kIluProgMax = 128;
+bool ilu_resident_ok(int b);
+size_t ilu_resident_gj_lds(int b);
+void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const int *prog, int nrows, size_t lds,
+                                int *flags, hipStream_t s);
+// Row-slab block products of one level for b in {64, 100, 144, 196, 256,
+// 324} (the diagonal inverses follow with launch_block_inverse).
+bool ilu_slab_ok(int b);
+void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                     const int *col, const int *upd_ptr, const int *upd, hipStream_t s);
+// Single-launch dataflow forms (grid within the device's occupancy, per-row
+// completion flags or sentinel-armed values);
 // return false when a co-resident launch is not possible (caller falls back
 // to the per-level launches).
 // Sweeps: src/dst/arm as documented at k_ilu_sweep_sn (lower: src = rhs,
@@ -87,6 +102,7 @@ template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
                          const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, hipStream_t s);
 template <class T> void launch_arm(T *v, size_t n, hipStream_t s);
+
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
                           const int *upd_ptr, const int *upd, int *done, int epoch, int *flags, hipStream_t s);
 

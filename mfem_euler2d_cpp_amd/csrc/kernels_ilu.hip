@@ -235,6 +235,203 @@ __device__ __forceinline__ bool gj_inverse(double *__restrict__ A, double *lds) 
   return singular;
 }
 
+// Gauss-Jordan inverse for 64 <= B <= 196 with one wavefront per column
+// group: wave g keeps columns g + NCG j (j < NE) of every row, lane l rows
+// l + 64 i (i < RW), all in registers. Column k = s + NCG jb belongs to
+// wave s. Each step is one workgroup barrier: after it every wave reads the
+// pivot row p of step k, 1 / S(p,k) and column k from LDS. The wave that
+// owns column k + 1 first updates that column alone and searches its pivot
+// (a register max over its rows, then a wavefront max of float magnitudes,
+// lowest lane on ties), and publishes p, 1 / S(p,k+1) and the column for
+// the next step while the other waves are still updating; then every wave
+// finishes the rank-1 update S(i,j) -= S(i,k) S(p,j) / S(p,k) of its columns
+// with the pivot row's values taken from lane p % 64 (readlane). Rows are
+// kept unscaled: the pivot row keeps its values and records scale[p] =
+// 1 / S(p,k), column k becomes -S(i,k) / S(p,k) and 1 at row p, and
+// A^{-1}(pinv[i], piv[j]) = scale[i] S(i,j) at the end (piv[k] = p,
+// pinv[p] = k; no row interchanges). After each group of NCG steps every
+// wave rotates its register columns by one, so the column of the current
+// step is register 0 of its owner. Mathematically the inverse of the block
+// whose LU the reference uses (src/linear.cpp:198-202,
+// src/block_ilu.cpp:83-84, src/condensation.cpp:137-140).
+template <int B, int NCG> struct Gj2 {
+  static constexpr int RW = (B + 63) / 64;
+  static constexpr int NE = (B + NCG - 1) / NCG;
+  static constexpr int NT = 64 * NCG;
+  static_assert(RW <= 4 && NT <= 1024, "register-resident block");
+};
+template <int B, int NCG> struct Gj2Lds {
+  double col[2][64 * Gj2<B, NCG>::RW];
+  double inv[2];
+  int prow[2];
+  double scale[B];
+  int piv[B], pinv[B];
+};
+__device__ __forceinline__ double read_lane(double v, int lane) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(u)), lane));
+  const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(u >> 32)), lane));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+template <int CTRL> __device__ __forceinline__ float dpp_fmax(float x) {
+  const float o = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), CTRL, 0xF, 0xF, false));
+  return fmaxf(x, o);
+}
+// Max over the wavefront (the same value in every lane): DPP within rows of
+// 16 lanes, then the four row maxima.
+__device__ __forceinline__ float wave_fmax(float x) {
+  x = dpp_fmax<0xB1>(x);  // quad_perm [1,0,3,2]
+  x = dpp_fmax<0x4E>(x);  // quad_perm [2,3,0,1]
+  x = dpp_fmax<0x141>(x); // row_half_mirror
+  x = dpp_fmax<0x140>(x); // row_mirror
+  const float a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 0));
+  const float b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 16));
+  const float c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32));
+  const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 48));
+  return fmaxf(fmaxf(a, b), fmaxf(c, d));
+}
+// Pivot search of the owning wave over the register column x (rows lane +
+// 64 i not yet used); publishes the pivot row, its reciprocal and the column.
+template <int B, int NCG>
+__device__ __forceinline__ void gj2_publish(const double (&x)[Gj2<B, NCG>::RW], unsigned used, int lane,
+                                            Gj2Lds<B, NCG> &L, int buf) {
+  constexpr int RW = Gj2<B, NCG>::RW;
+  float key = -1.f;
+  int bi = 0;
+  double bv = 0.;
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+    const int r = lane + 64 * i;
+    const float a = float(x[i]);
+    const float kk = (r < B && !((used >> i) & 1u)) ? (a != a ? __int_as_float(0x7F800000) : fabsf(a)) : -1.f;
+    if (kk > key) { key = kk; bi = i; bv = x[i]; }
+    L.col[buf][r] = x[i];
+  }
+  const float best = wave_fmax(key);
+  const unsigned long long mask = __ballot(key == best);
+  const int pl = __ffsll(static_cast<long long>(mask)) - 1;
+  const int pi = __builtin_amdgcn_readlane(bi, pl);
+  const double ap = read_lane(bv, pl);
+  if (lane == 0) {
+    L.prow[buf] = pl + 64 * pi;
+    L.inv[buf] = 1. / ap;
+  }
+}
+template <int B, int NCG>
+__device__ __forceinline__ bool gj2_inverse(double *__restrict__ A, Gj2Lds<B, NCG> &L) {
+  using G = Gj2<B, NCG>;
+  constexpr int RW = G::RW, NE = G::NE;
+  const int t = threadIdx.x, lane = t & 63, cg = t >> 6;
+  double v[RW][NE];
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int r = lane + 64 * i, c = cg + NCG * j;
+      v[i][j] = (r < B && c < B) ? A[r + size_t(c) * B] : 0.;
+    }
+  }
+  unsigned used = 0u; // bit i: row lane + 64 i is a pivot row already
+  bool singular = false;
+  if (cg == 0) {
+    double x[RW];
+#pragma unroll
+    for (int i = 0; i < RW; ++i) { x[i] = v[i][0]; }
+    gj2_publish<B, NCG>(x, used, lane, L, 0);
+  }
+  for (int jb = 0; jb < NE; ++jb) {
+#pragma unroll
+    for (int s = 0; s < NCG; ++s) {
+      const int k = s + NCG * jb;
+      if (k >= B) { break; }
+      const int buf = k & 1;
+      const int s1 = (s + 1) % NCG;
+      const int j1 = s + 1 < NCG ? 0 : 1; // register of column k + 1 in its owner (unrolled: a constant)
+      __syncthreads();
+      const int p = __builtin_amdgcn_readfirstlane(L.prow[buf]);
+      const double inv = L.inv[buf];
+      if (!(isfinite(inv) && inv != 0.)) { singular = true; }
+      const int ip = p >> 6, lp = p & 63;
+      double m[RW];
+#pragma unroll
+      for (int i = 0; i < RW; ++i) { m[i] = (lane + 64 * i == p) ? 0. : L.col[buf][lane + 64 * i] * inv; }
+      const unsigned used_next = used | (lane == lp ? 1u << ip : 0u);
+      const bool ahead = cg == s1 && k + 1 < B;
+      if (ahead && j1 < NE) {
+        double pr = 0.;
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+          if (ip == i) { pr = read_lane(v[i][j1], lp); }
+        }
+        double x[RW];
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+          v[i][j1] = fma(-m[i], pr, v[i][j1]);
+          x[i] = v[i][j1];
+        }
+        gj2_publish<B, NCG>(x, used_next, lane, L, buf ^ 1);
+      }
+      double prow[NE];
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        if (ip == i) {
+#pragma unroll
+          for (int j = 0; j < NE; ++j) { prow[j] = read_lane(v[i][j], lp); }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NE; ++j) {
+        if ((ahead && j == j1) || (cg == s && j == 0)) { continue; }
+#pragma unroll
+        for (int i = 0; i < RW; ++i) { v[i][j] = fma(-m[i], prow[j], v[i][j]); }
+      }
+      if (cg == s) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i) { v[i][0] = (lane + 64 * i == p) ? 1. : -m[i]; }
+      }
+      used = used_next;
+      if (t == 0) {
+        L.piv[k] = p;
+        L.pinv[p] = k;
+        L.scale[p] = inv;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < RW; ++i) {
+      const double first = v[i][0];
+#pragma unroll
+      for (int j = 0; j + 1 < NE; ++j) { v[i][j] = v[i][j + 1]; }
+      v[i][NE - 1] = first;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < RW; ++i) {
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const int r = lane + 64 * i, c = cg + NCG * j;
+      if (r < B && c < B) { A[L.pinv[r] + size_t(L.piv[c]) * B] = L.scale[r] * v[i][j]; }
+    }
+  }
+  __syncthreads();
+  return singular;
+}
+template <int B> struct Gj2Pick;
+template <> struct Gj2Pick<64> { static constexpr int NCG = 4; };
+template <> struct Gj2Pick<100> { static constexpr int NCG = 4; };
+template <> struct Gj2Pick<144> { static constexpr int NCG = 8; };
+template <> struct Gj2Pick<196> { static constexpr int NCG = 8; };
+template <int B> constexpr int gj2_threads() { return 64 * Gj2Pick<B>::NCG; }
+template <int B>
+__global__ __launch_bounds__(gj2_threads<B>()) void k_block_inverse_w(double *__restrict__ blocks,
+                                                                                  const int *__restrict__ index,
+                                                                                  int *__restrict__ flags) {
+  constexpr int NCG = Gj2Pick<B>::NCG;
+  __shared__ Gj2Lds<B, NCG> L;
+  const int blk = index ? index[blockIdx.x] : blockIdx.x;
+  if (gj2_inverse<B, NCG>(blocks + size_t(blk) * B * B, L) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
+}
+
 template <int B>
 __global__ __launch_bounds__(Shape<B>::NT) void k_block_inverse_b(double *__restrict__ blocks,
                                                                   const int *__restrict__ index,
@@ -326,6 +523,195 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_rows(double *__rest
   }
 }
 
+// Row-resident ILU(0) factor of one row (small blocks): every block of the
+// row and every block it reads from earlier rows (inv(A_kk) of each lower
+// block, A_kj of each update pair) is loaded into LDS in one batch of
+// independent loads, the row is factored in LDS (src/block_ilu.cpp:73-86:
+// ascending k, L_ik = A_ik inv(A_kk), then A_ij -= L_ik A_kj over the update
+// pairs of ik in parallel), the diagonal block is inverted in place and the
+// row is written back as one contiguous run. The per-row program (built on
+// the host from the fixed pattern) is
+//   [len, first slot, blocks in row, diagonal position, dependencies,
+//    dependency slots..., per lower block: (inverse dependency, updates,
+//    (target position, dependency) x updates)].
+template <int B>
+__device__ __forceinline__ double block_dot(const double *X, const double *Y, int o) {
+  const int r = o % B, c = o / B;
+  double acc = 0.;
+#pragma unroll 4
+  for (int kk = 0; kk < B; ++kk) { acc += X[r + kk * B] * Y[kk + c * B]; }
+  return acc;
+}
+template <int B>
+__global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident(double *__restrict__ bsr,
+                                                                      const int *__restrict__ prog_off,
+                                                                      const int *__restrict__ prog,
+                                                                      int *__restrict__ flags) {
+  extern __shared__ double sm[];
+  __shared__ int P[kIluProgMax];
+  constexpr int bb = B * B, NT = Shape<B>::NT, NQ = (bb + NT - 1) / NT;
+  const int t = threadIdx.x;
+  const int *pg = prog + prog_off[blockIdx.x];
+  for (int e = t; e < kIluProgMax; e += NT) { P[e] = pg[e]; } // the program array is padded by kIluProgMax
+  __syncthreads();
+  const int j0 = P[1], nrow = P[2], dpos = P[3], ndep = P[4];
+  double *Arow = sm, *Dep = sm + size_t(nrow) * bb, *gjs = Dep + size_t(ndep) * bb;
+  for (int e = t; e < (nrow + ndep) * bb; e += NT) {
+    const int blk = e / bb, o = e - blk * bb;
+    const int slot = blk < nrow ? j0 + blk : P[5 + blk - nrow];
+    sm[e] = bsr[size_t(slot) * bb + o];
+  }
+  __syncthreads();
+  int pc = 5 + ndep;
+  for (int l = 0; l < dpos; ++l) {
+    const int inv = P[pc], nupd = P[pc + 1];
+    const int ops = pc + 2;
+    pc = ops + 2 * nupd;
+    double *X = Arow + size_t(l) * bb;
+    const double *Y = Dep + size_t(inv) * bb;
+    double res[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int o = t + q * NT;
+      res[q] = o < bb ? block_dot<B>(X, Y, o) : 0.;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int o = t + q * NT;
+      if (o < bb) { X[o] = res[q]; }
+    }
+    __syncthreads();
+    for (int e = t; e < nupd * bb; e += NT) {
+      const int u = e / bb, o = e - u * bb;
+      double *T = Arow + size_t(P[ops + 2 * u]) * bb;
+      T[o] -= block_dot<B>(X, Dep + size_t(P[ops + 2 * u + 1]) * bb, o);
+    }
+    __syncthreads();
+  }
+  const bool singular = gj_inverse<B, false>(Arow + size_t(dpos) * bb, gjs);
+  for (int e = t; e < nrow * bb; e += NT) { bsr[size_t(j0) * bb + e] = Arow[e]; }
+  if (singular && t == 0) { atomicOr(flags, kErrSingular); }
+}
+template <int B> constexpr size_t resident_gj_lds() { return size_t(gj_lds<B>()) * sizeof(double); }
+
+// ILU(0) block products of one dependency level by row slabs: rows
+// [r0, r0 + R) of the blocks of row i depend only on the same rows of row i's
+// blocks and on whole blocks of earlier rows, so every (row, slab) pair is an
+// independent workgroup (src/block_ilu.cpp:73-86, same order of products per
+// slab: ascending k, L_ik then its update pairs). The diagonal blocks are
+// inverted by a second launch. Thread (tc, tr) = (tid % 64, tid / 64) of the
+// 256 computes rows tr RT .. tr RT + RT - 1 of the slab at columns tc + 64 j;
+// the slab of the left operand sits in LDS transposed ([kk][r], broadcast
+// reads), the right operand is staged in panels of KP rows ([kk][c], padded
+// stride), prefetched into registers one panel ahead.
+template <int B> struct Slab {
+  static constexpr int R = B <= 100 ? 16 : 32;
+  static constexpr int RT = R / 4;
+  static constexpr int CJ = (B + 63) / 64;
+  static constexpr int KP = B <= 100 ? B : (B <= 256 ? 32 : 16);
+  static constexpr int LDY = B + 1;
+  static constexpr int NS = (B + R - 1) / R;
+  static constexpr int NP = (B + KP - 1) / KP;
+  static constexpr int PER = (KP * B + 255) / 256;
+  static constexpr size_t lds() { return (size_t(B) * R + size_t(KP) * LDY) * sizeof(double); }
+};
+template <int B>
+__device__ __forceinline__ void slab_product(const double *__restrict__ Y, const double *Xs, double *Ys,
+                                             double (&acc)[Slab<B>::RT][Slab<B>::CJ]) {
+  using S = Slab<B>;
+  const int t = threadIdx.x, tc = t & 63, tr = t >> 6;
+#pragma unroll
+  for (int a = 0; a < S::RT; ++a) {
+#pragma unroll
+    for (int j = 0; j < S::CJ; ++j) { acc[a][j] = 0.; }
+  }
+  double pre[S::PER];
+  auto fetch = [&](int k0) {
+    const int kp = min(S::KP, B - k0);
+#pragma unroll
+    for (int q = 0; q < S::PER; ++q) {
+      const int e = t + q * 256;
+      const int kk = e % kp, c = e / kp;
+      pre[q] = e < kp * B ? Y[k0 + kk + size_t(c) * B] : 0.;
+    }
+  };
+  fetch(0);
+  for (int p = 0; p < S::NP; ++p) {
+    const int k0 = p * S::KP, kp = min(S::KP, B - k0);
+    __syncthreads(); // the previous panel (and the caller's Xs stores) are done
+#pragma unroll
+    for (int q = 0; q < S::PER; ++q) {
+      const int e = t + q * 256;
+      const int kk = e % kp, c = e / kp;
+      if (e < kp * B) { Ys[kk * S::LDY + c] = pre[q]; }
+    }
+    __syncthreads();
+    if (p + 1 < S::NP) { fetch(k0 + S::KP); }
+#pragma unroll 2
+    for (int kk = 0; kk < kp; ++kk) {
+      double x[S::RT];
+#pragma unroll
+      for (int a = 0; a < S::RT; ++a) { x[a] = Xs[(k0 + kk) * S::R + tr * S::RT + a]; }
+#pragma unroll
+      for (int j = 0; j < S::CJ; ++j) {
+        const int c = tc + 64 * j;
+        const double y = (j + 1 < S::CJ || c < B) ? Ys[kk * S::LDY + c] : 0.;
+#pragma unroll
+        for (int a = 0; a < S::RT; ++a) { acc[a][j] += x[a] * y; }
+      }
+    }
+  }
+}
+template <int B>
+__global__ __launch_bounds__(256) void k_ilu_slab(double *__restrict__ bsr, const int *__restrict__ rows,
+                                                  const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
+                                                  const int *__restrict__ col, const int *__restrict__ upd_ptr,
+                                                  const int *__restrict__ upd) {
+  using S = Slab<B>;
+  extern __shared__ double sm[];
+  constexpr size_t bb = size_t(B) * B;
+  double *Xs = sm, *Ys = sm + B * S::R;
+  const int i = rows[blockIdx.x / S::NS], r0 = (blockIdx.x % S::NS) * S::R;
+  const int t = threadIdx.x, tc = t & 63, tr = t >> 6;
+  double acc[S::RT][S::CJ];
+  for (int ik = row_ptr[i]; ik < diag_ptr[i]; ++ik) {
+    double *aik = bsr + size_t(ik) * bb;
+    __syncthreads(); // Xs is free
+    for (int e = t; e < B * S::R; e += 256) {
+      const int r = e % S::R, kk = e / S::R;
+      Xs[e] = r0 + r < B ? aik[r0 + r + size_t(kk) * B] : 0.;
+    }
+    slab_product<B>(bsr + size_t(diag_ptr[col[ik]]) * bb, Xs, Ys, acc); // L_ik = A_ik inv(A_kk)
+    __syncthreads(); // every read of Xs is done
+#pragma unroll
+    for (int a = 0; a < S::RT; ++a) {
+      const int r = tr * S::RT + a;
+#pragma unroll
+      for (int j = 0; j < S::CJ; ++j) {
+        const int c = tc + 64 * j;
+        if (c < B) {
+          Xs[c * S::R + r] = acc[a][j];
+          if (r0 + r < B) { aik[r0 + r + size_t(c) * B] = acc[a][j]; }
+        }
+      }
+    }
+    for (int u = upd_ptr[ik]; u < upd_ptr[ik + 1]; ++u) {
+      slab_product<B>(bsr + size_t(upd[2 * u + 1]) * bb, Xs, Ys, acc); // L_ik A_kj
+      double *tgt = bsr + size_t(upd[2 * u]) * bb;
+#pragma unroll
+      for (int a = 0; a < S::RT; ++a) {
+        const int r = r0 + tr * S::RT + a;
+#pragma unroll
+        for (int j = 0; j < S::CJ; ++j) {
+          const int c = tc + 64 * j;
+          if (c < B && r < B) { tgt[r + size_t(c) * B] -= acc[a][j]; }
+        }
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------ sweeps
 // Dot products of block row r with x, split over P = blockDim/b threads
 // (columns c = part, part + P, ...), four independent loads in flight.
@@ -402,13 +788,16 @@ __global__ __launch_bounds__(NT) void k_ilu_upper_split(const T *__restrict__ bs
 
 // ------------------------------------------------------------ dataflow forms
 // One launch per sweep (or per factorisation) instead of one per level: the
-// grid is co-resident (cooperative launch) and each workgroup handles its
-// rows in dependency order (increasing for the lower sweep and the factor,
-// decreasing for the upper sweep). As the rows waited on precede the waiting
-// row in that order and all workgroups are resident, every wait ends; each
-// wait is nevertheless bounded by spin_limit and then raises kErrSpin
-// instead of hanging. Data crossing workgroups is read and written with
-// coherent (sc1) accesses only.
+// grid is no larger than the device holds at once (occupancy of the kernel
+// on every CU), so on the library's in-order stream all its workgroups are
+// resident together, and each workgroup handles its rows in dependency order
+// (increasing for the lower sweep and the factor, decreasing for the upper
+// sweep). As the rows waited on precede the waiting row in that order, every
+// wait ends; each wait is nevertheless bounded by spin_limit and then raises
+// kErrSpin instead of hanging. A plain launch is used: a cooperative launch
+// costs about 20 us more on this stack, as much as a whole sweep of a small
+// case. Data crossing workgroups is read and written with coherent (sc1)
+// accesses only.
 //
 // Factor: before a row the workgroup waits until every row it depends on has
 // published done[j] >= epoch; after the row it completes its stores and
@@ -563,6 +952,7 @@ template <class T> __global__ void k_arm(T *v, size_t n) {
     v[t] = sentinel<T>();
   }
 }
+
 // Dataflow ILU(0) factor: factor_row per row, rows in increasing order per
 // workgroup, waiting on the rows of the lower pattern.
 template <int B>
@@ -650,9 +1040,49 @@ int ilu_factor_lds_init() {
   EG_ALLOW(4) EG_ALLOW(8) EG_ALLOW(12) EG_ALLOW(16) EG_ALLOW(20) EG_ALLOW(24) EG_ALLOW(28) EG_ALLOW(32)
   EG_ALLOW(36) EG_ALLOW(64) EG_ALLOW(100)
 #undef EG_ALLOW
+#define EG_ALLOW_R(B) allow_lds(reinterpret_cast<const void *>(k_ilu_factor_resident<B>));
+  EG_ALLOW_R(4) EG_ALLOW_R(8) EG_ALLOW_R(12) EG_ALLOW_R(16) EG_ALLOW_R(20) EG_ALLOW_R(24) EG_ALLOW_R(28)
+  EG_ALLOW_R(32) EG_ALLOW_R(36)
+#undef EG_ALLOW_R
+#define EG_ALLOW_S(B) allow_lds(reinterpret_cast<const void *>(k_ilu_slab<B>));
+  EG_ALLOW_S(64) EG_ALLOW_S(100) EG_ALLOW_S(144) EG_ALLOW_S(196) EG_ALLOW_S(256) EG_ALLOW_S(324)
+#undef EG_ALLOW_S
   return 0;
 }
+bool ilu_resident_ok(int b) { return b % 4 == 0 && b >= 4 && b <= 36; }
+size_t ilu_resident_gj_lds(int b) {
+  size_t n = 0;
+  EG_DISPATCH_B_SMALL(b, (n = resident_gj_lds<B>()));
+  return n;
+}
+void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const int *prog, int nrows, size_t lds,
+                                int *flags, hipStream_t s) {
+  EG_DISPATCH_B_SMALL(b, (k_ilu_factor_resident<B><<<nrows, Shape<B>::NT, lds, s>>>(bsr, prog_off, prog, flags)));
+}
+#define EG_DISPATCH_SLAB(b, CALL)                                                                  \
+  switch (b) {                                                                                     \
+  case 64: { constexpr int B = 64; CALL; } break;                                                  \
+  case 100: { constexpr int B = 100; CALL; } break;                                                \
+  case 144: { constexpr int B = 144; CALL; } break;                                                \
+  case 196: { constexpr int B = 196; CALL; } break;                                                \
+  case 256: { constexpr int B = 256; CALL; } break;                                                \
+  case 324: { constexpr int B = 324; CALL; } break;                                                \
+  default: break;                                                                                  \
+  }
+bool ilu_slab_ok(int b) { return b == 64 || b == 100 || b == 144 || b == 196 || b == 256 || b == 324; }
+void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                     const int *col, const int *upd_ptr, const int *upd, hipStream_t s) {
+  EG_DISPATCH_SLAB(b, (k_ilu_slab<B><<<nrows * Slab<B>::NS, 256, Slab<B>::lds(), s>>>(bsr, rows, row_ptr, diag_ptr,
+                                                                                     col, upd_ptr, upd)));
+}
 void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
+  switch (b) {
+  case 64: k_block_inverse_w<64><<<count, gj2_threads<64>(), 0, s>>>(blocks, index, flags); return;
+  case 100: k_block_inverse_w<100><<<count, gj2_threads<100>(), 0, s>>>(blocks, index, flags); return;
+  case 144: k_block_inverse_w<144><<<count, gj2_threads<144>(), 0, s>>>(blocks, index, flags); return;
+  case 196: k_block_inverse_w<196><<<count, gj2_threads<196>(), 0, s>>>(blocks, index, flags); return;
+  default: break;
+  }
   EG_DISPATCH_GJ(b, (k_block_inverse_b<B><<<count, Shape<B>::NT, 0, s>>>(blocks, index, flags)));
 }
 void launch_ilu_factor_fast(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
@@ -688,7 +1118,7 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
     const int grid = std::min(nb, cap);
     int spin = kSpinLimit;
     void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin};
-    ok = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(threads), args, 0, s) == hipSuccess;
+    ok = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s) == hipSuccess;
     if (!ok) { (void)hipGetLastError(); }
   };
 #define EG_SWEEP(DISPATCH, KD)                                                                     \
@@ -716,7 +1146,7 @@ bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const 
       const int grid = std::min(nb, cap);
       int spin = kSpinLimit;
       void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &upd_ptr, &upd, &done, &epoch, &flags, &spin};
-      ok = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(Shape<B>::NT), args, lds, s) == hipSuccess;
+      ok = hipLaunchKernel(fn, dim3(grid), dim3(Shape<B>::NT), args, lds, s) == hipSuccess;
       if (!ok) { (void)hipGetLastError(); }
     }
   }));

@@ -50,7 +50,8 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     hip_check(hipMemsetAsync(done_factor, 0, sizeof(int) * pattern->nb, p.stream), "memset");
     sweep.resize(N);
     if (mixed) { sweep32.resize(N); }
-    if (eg::big_block(block)) { PlanBigFactor(); }
+    if (eg::ilu_resident_ok(block)) { PlanResident(); }
+    if (eg::big_block(block) && !eg::ilu_slab_ok(block)) { PlanBigFactor(); }
     if (form == 0) {
       std::vector<int> ds(p.n), sl(4 * p.n);
       for (int e = 0; e < p.n; ++e) {
@@ -161,6 +162,51 @@ void LinearSolver::PlanBigFactor() {
   up(d_pairs, pairs);
 }
 
+// Programs of the row-resident factor (small blocks), one per row in the
+// order of the dependency levels: [length, first slot, blocks in the row,
+// diagonal position, dependencies, dependency slots..., then per lower block
+// ik (ascending k): dependency of inv(A_kk), update count, (target position
+// in the row, dependency of A_kj) per update pair]. Rows whose blocks and
+// dependencies do not fit in LDS leave the factor on the LDS-staged kernel.
+void LinearSolver::PlanResident() {
+  const auto &pt = *pattern;
+  const size_t bb = size_t(block) * block;
+  std::vector<int> prog, off;
+  size_t most = 0;
+  for (int r = 0; r < pt.nb; ++r) {
+    const int i = pt.lower_rows[r];
+    std::vector<int> deps, ops;
+    for (int ik = pt.row[i]; ik < pt.diag[i]; ++ik) {
+      ops.push_back(int(deps.size()));
+      deps.push_back(pt.diag[pt.col[ik]]);
+      ops.push_back(pt.upd_ptr[ik + 1] - pt.upd_ptr[ik]);
+      for (int u = pt.upd_ptr[ik]; u < pt.upd_ptr[ik + 1]; ++u) {
+        ops.push_back(pt.upd[2 * u] - pt.row[i]);
+        ops.push_back(int(deps.size()));
+        deps.push_back(pt.upd[2 * u + 1]);
+      }
+    }
+    const int nrow = pt.row[i + 1] - pt.row[i];
+    const int len = 5 + int(deps.size() + ops.size());
+    if (len > eg::kIluProgMax) { return; }
+    off.push_back(int(prog.size()));
+    prog.insert(prog.end(), {len, pt.row[i], nrow, pt.diag[i] - pt.row[i], int(deps.size())});
+    prog.insert(prog.end(), deps.begin(), deps.end());
+    prog.insert(prog.end(), ops.begin(), ops.end());
+    most = std::max(most, (nrow + deps.size()) * bb * sizeof(double));
+  }
+  const size_t lds = most + eg::ilu_resident_gj_lds(block);
+  if (lds + sizeof(int) * eg::kIluProgMax > 163840) { return; }
+  prog.resize(prog.size() + eg::kIluProgMax, 0);
+  d_prog.resize(prog.size());
+  d_prog_off.resize(off.size());
+  hip_check(hipMemcpyAsync(d_prog, prog.data(), sizeof(int) * prog.size(), hipMemcpyHostToDevice, p.stream), "upload");
+  hip_check(hipMemcpyAsync(d_prog_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, p.stream),
+            "upload");
+  resident_lds = lds;
+  resident = true;
+}
+
 // Fill and factor the preconditioner at the current Newton state
 // (src/linear.cpp:182-203; src/partial.cpp:140-173,265-298;
 // src/operators.cpp:48-74,269-303; src/block_ilu.cpp:73-86).
@@ -188,6 +234,17 @@ void LinearSolver::Setup() {
     if (!df) {
       for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
         const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
+        if (resident) {
+          eg::launch_ilu_factor_resident(blocks, b, d_prog_off.p + first, d_prog, count, resident_lds, p.flags,
+                                         p.stream);
+          continue;
+        }
+        if (eg::ilu_slab_ok(b)) {
+          eg::launch_ilu_slab(blocks, b, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col,
+                              pt.d_upd_ptr, pt.d_upd, p.stream);
+          eg::launch_block_inverse(blocks, b, count, pt.d_lower_diag.p + first, p.flags, p.stream);
+          continue;
+        }
         if (split_factor) {
           const int a0 = a_ptr[l], t0 = t_ptr[l];
           eg::launch_ilu_factor_level_split(blocks, b, a_ptr[l + 1] - a0, d_a_ik.p + a0, d_a_kk.p + a0,

@@ -69,7 +69,7 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
     try {
       require(requests[i].form == 0 || requests[i].form == 1, "form must be 0 (dg) or 1 (hdg)");
       opts[i] = ParseOptions(requests[i].args);
-      // The single-launch (cooperative) ILU schedules assume that their grid
+      // The single-launch ILU schedules assume that their grid
       // is resident on the whole device at once, which concurrent solves on
       // other streams break: solves that share the device run the factor and
       // the sweeps one launch per dependency level.

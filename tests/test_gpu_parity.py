@@ -298,7 +298,9 @@ def test_cli_backward_euler_update_check(gpu, tmp_path):
 def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
     """The single-launch dataflow ILU(0) sweeps (sentinel-armed dependency
     polling) and factor (per-row completion flags) give the preconditioner of
-    the one-launch-per-level schedule."""
+    the one-launch-per-level schedule. The sweeps sum in the same order
+    (1e-13); the single-launch factor uses other dense kernels than the
+    per-level one (1e-12)."""
     import torch
     out = {}
     for mode in ("levels", "dataflow", "dataflow-all"):
@@ -316,6 +318,6 @@ def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
             res.append(y.cpu().numpy())
             assert rel(res[-1], o.precondition(form, 1, b)) < 1e-10
         out[mode] = res
-    for mode in ("dataflow", "dataflow-all"):
+    for mode, tol in (("dataflow", 1e-13), ("dataflow-all", 1e-12)):
         for a, b in zip(out["levels"], out[mode]):
-            assert rel(a, b) < 1e-13
+            assert rel(a, b) < tol, mode
