@@ -231,6 +231,10 @@ public:
   std::vector<int> a_ptr, t_ptr;
   DeviceArray<int> d_a_ik, d_a_kk, d_t_slot, d_t_pair_ptr, d_pairs;
   bool dataflow = true, dataflow_factor = false;
+  // Per-level factor launches recorded once into a graph (EG_FACTOR_GRAPH=0:
+  // launched directly).
+  bool use_graph = true;
+  hipGraphExec_t factor_exec = nullptr;
   // Row-resident factor of small blocks: per-row programs in level order.
   bool resident = false;
   size_t resident_lds = 0;
@@ -238,8 +242,12 @@ public:
   std::vector<double> h, cosine, sine, rotated, coefficients;
 
   LinearSolver(Problem &p, int form, Hybrid *hybrid = nullptr);
+  ~LinearSolver();
+  LinearSolver(const LinearSolver &) = delete;
+  LinearSolver &operator=(const LinearSolver &) = delete;
   void PlanBigFactor();
   void PlanResident();
+  void FactorLevels();
   void Refresh(const double *dev_u) { state = dev_u; }
   void Setup();  // fill + factor the preconditioner at the current state
   void Apply(const double *x, double *y);

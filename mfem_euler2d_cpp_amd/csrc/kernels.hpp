@@ -89,6 +89,7 @@ void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const i
 // Row-slab block products of one level for b in {64, 100, 144, 196, 256,
 // 324} (the diagonal inverses follow with launch_block_inverse).
 bool ilu_slab_ok(int b);
+
 void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                      const int *col, const int *upd_ptr, const int *upd, hipStream_t s);
 // Single-launch dataflow forms (grid within the device's occupancy, per-row

@@ -267,6 +267,16 @@ template <int B, int NCG> struct Gj2Lds {
   double scale[B];
   int piv[B], pinv[B];
 };
+// 1 / a from the hardware reciprocal and two Newton steps (about 1 ulp;
+// zero, infinite or NaN a give a NaN, which the caller reports as a
+// singular pivot).
+__device__ __forceinline__ double recip(double a) {
+  double x = __builtin_amdgcn_rcp(a);
+  double e = fma(-a, x, 1.);
+  x = fma(x, e, x);
+  e = fma(-a, x, 1.);
+  return fma(x, e, x);
+}
 __device__ __forceinline__ double read_lane(double v, int lane) {
   const unsigned long long u = __double_as_longlong(v);
   const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(u)), lane));
@@ -314,7 +324,7 @@ __device__ __forceinline__ void gj2_publish(const double (&x)[Gj2<B, NCG>::RW], 
   const double ap = read_lane(bv, pl);
   if (lane == 0) {
     L.prow[buf] = pl + 64 * pi;
-    L.inv[buf] = 1. / ap;
+    L.inv[buf] = recip(ap);
   }
 }
 template <int B, int NCG>
@@ -371,6 +381,7 @@ __device__ __forceinline__ bool gj2_inverse(double *__restrict__ A, Gj2Lds<B, NC
         }
         gj2_publish<B, NCG>(x, used_next, lane, L, buf ^ 1);
       }
+      // the pivot row's values of this wave's columns (scalar registers)
       double prow[NE];
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
@@ -606,19 +617,35 @@ template <int B> constexpr size_t resident_gj_lds() { return size_t(gj_lds<B>())
 // reads), the right operand is staged in panels of KP rows ([kk][c], padded
 // stride), prefetched into registers one panel ahead.
 template <int B> struct Slab {
-  static constexpr int R = B <= 100 ? 16 : 32;
+  static constexpr int R = 16;
+  static constexpr int LDX = R + 1; // padded slab stride (the column-wise stores of L do not conflict)
   static constexpr int RT = R / 4;
   static constexpr int CJ = (B + 63) / 64;
-  static constexpr int KP = B <= 100 ? B : (B <= 256 ? 32 : 16);
+  static constexpr int KP = B <= 64 ? B : 16; // larger blocks: panels of 16 rows (two workgroups a CU)
   static constexpr int LDY = B + 1;
   static constexpr int NS = (B + R - 1) / R;
   static constexpr int NP = (B + KP - 1) / KP;
   static constexpr int PER = (KP * B + 255) / 256;
-  static constexpr size_t lds() { return (size_t(B) * R + size_t(KP) * LDY) * sizeof(double); }
+  static constexpr size_t lds() { return (size_t(B) * LDX + size_t(KP) * LDY) * sizeof(double); }
 };
 template <int B>
-__device__ __forceinline__ void slab_product(const double *__restrict__ Y, const double *Xs, double *Ys,
-                                             double (&acc)[Slab<B>::RT][Slab<B>::CJ]) {
+__device__ __forceinline__ void slab_fetch(const double *__restrict__ Y, int k0, double (&pre)[Slab<B>::PER]) {
+  using S = Slab<B>;
+  const int t = threadIdx.x, kp = min(S::KP, B - k0);
+#pragma unroll
+  for (int q = 0; q < S::PER; ++q) {
+    const int e = t + q * 256;
+    const int kk = e % kp, c = e / kp;
+    pre[q] = e < kp * B ? Y[k0 + kk + size_t(c) * B] : 0.;
+  }
+}
+// acc = Xs Y: Xs the slab in LDS ([kk][r]), Y streamed from global memory in
+// panels through Ys; pre holds Y's first panel on entry and, on return, the
+// first panel of Ynext (when given).
+template <int B>
+__device__ __forceinline__ void slab_product(const double *__restrict__ Y, const double *__restrict__ Ynext,
+                                             const double *Xs, double *Ys, double (&acc)[Slab<B>::RT][Slab<B>::CJ],
+                                             double (&pre)[Slab<B>::PER]) {
   using S = Slab<B>;
   const int t = threadIdx.x, tc = t & 63, tr = t >> 6;
 #pragma unroll
@@ -626,20 +653,9 @@ __device__ __forceinline__ void slab_product(const double *__restrict__ Y, const
 #pragma unroll
     for (int j = 0; j < S::CJ; ++j) { acc[a][j] = 0.; }
   }
-  double pre[S::PER];
-  auto fetch = [&](int k0) {
-    const int kp = min(S::KP, B - k0);
-#pragma unroll
-    for (int q = 0; q < S::PER; ++q) {
-      const int e = t + q * 256;
-      const int kk = e % kp, c = e / kp;
-      pre[q] = e < kp * B ? Y[k0 + kk + size_t(c) * B] : 0.;
-    }
-  };
-  fetch(0);
   for (int p = 0; p < S::NP; ++p) {
     const int k0 = p * S::KP, kp = min(S::KP, B - k0);
-    __syncthreads(); // the previous panel (and the caller's Xs stores) are done
+    __syncthreads(); // the previous panel (and the caller's LDS stores) are done
 #pragma unroll
     for (int q = 0; q < S::PER; ++q) {
       const int e = t + q * 256;
@@ -647,12 +663,13 @@ __device__ __forceinline__ void slab_product(const double *__restrict__ Y, const
       if (e < kp * B) { Ys[kk * S::LDY + c] = pre[q]; }
     }
     __syncthreads();
-    if (p + 1 < S::NP) { fetch(k0 + S::KP); }
-#pragma unroll 2
+    if (p + 1 < S::NP) { slab_fetch<B>(Y, k0 + S::KP, pre); }
+    else if (Ynext) { slab_fetch<B>(Ynext, 0, pre); }
+#pragma unroll 4
     for (int kk = 0; kk < kp; ++kk) {
       double x[S::RT];
 #pragma unroll
-      for (int a = 0; a < S::RT; ++a) { x[a] = Xs[(k0 + kk) * S::R + tr * S::RT + a]; }
+      for (int a = 0; a < S::RT; ++a) { x[a] = Xs[(k0 + kk) * S::LDX + tr * S::RT + a]; }
 #pragma unroll
       for (int j = 0; j < S::CJ; ++j) {
         const int c = tc + 64 * j;
@@ -671,7 +688,7 @@ __global__ __launch_bounds__(256) void k_ilu_slab(double *__restrict__ bsr, cons
   using S = Slab<B>;
   extern __shared__ double sm[];
   constexpr size_t bb = size_t(B) * B;
-  double *Xs = sm, *Ys = sm + B * S::R;
+  double *Xs = sm, *Ys = sm + B * S::LDX;
   const int i = rows[blockIdx.x / S::NS], r0 = (blockIdx.x % S::NS) * S::R;
   const int t = threadIdx.x, tc = t & 63, tr = t >> 6;
   double acc[S::RT][S::CJ];
@@ -680,9 +697,11 @@ __global__ __launch_bounds__(256) void k_ilu_slab(double *__restrict__ bsr, cons
     __syncthreads(); // Xs is free
     for (int e = t; e < B * S::R; e += 256) {
       const int r = e % S::R, kk = e / S::R;
-      Xs[e] = r0 + r < B ? aik[r0 + r + size_t(kk) * B] : 0.;
+      Xs[kk * S::LDX + r] = r0 + r < B ? aik[r0 + r + size_t(kk) * B] : 0.;
     }
-    slab_product<B>(bsr + size_t(diag_ptr[col[ik]]) * bb, Xs, Ys, acc); // L_ik = A_ik inv(A_kk)
+    double pre[S::PER];
+    slab_fetch<B>(bsr + size_t(diag_ptr[col[ik]]) * bb, 0, pre);
+    slab_product<B>(bsr + size_t(diag_ptr[col[ik]]) * bb, nullptr, Xs, Ys, acc, pre); // L_ik = A_ik inv(A_kk)
     __syncthreads(); // every read of Xs is done
 #pragma unroll
     for (int a = 0; a < S::RT; ++a) {
@@ -691,13 +710,14 @@ __global__ __launch_bounds__(256) void k_ilu_slab(double *__restrict__ bsr, cons
       for (int j = 0; j < S::CJ; ++j) {
         const int c = tc + 64 * j;
         if (c < B) {
-          Xs[c * S::R + r] = acc[a][j];
+          Xs[c * S::LDX + r] = acc[a][j];
           if (r0 + r < B) { aik[r0 + r + size_t(c) * B] = acc[a][j]; }
         }
       }
     }
     for (int u = upd_ptr[ik]; u < upd_ptr[ik + 1]; ++u) {
-      slab_product<B>(bsr + size_t(upd[2 * u + 1]) * bb, Xs, Ys, acc); // L_ik A_kj
+      slab_fetch<B>(bsr + size_t(upd[2 * u + 1]) * bb, 0, pre);
+      slab_product<B>(bsr + size_t(upd[2 * u + 1]) * bb, nullptr, Xs, Ys, acc, pre); // L_ik A_kj
       double *tgt = bsr + size_t(upd[2 * u]) * bb;
 #pragma unroll
       for (int a = 0; a < S::RT; ++a) {
@@ -953,6 +973,128 @@ template <class T> __global__ void k_arm(T *v, size_t n) {
   }
 }
 
+// Dataflow sweeps for large blocks (b > 100, whose rows do not fit in
+// registers): same row order, sentinel polling and coherent exchange as
+// k_ilu_sweep_sn, with the block GEMVs streamed from memory after the wait by
+// 1024 threads (thread (r, part) = (tid % b, tid / b), columns part + v P).
+// The first PRE columns of each block are loaded before the wait.
+// sum over columns c = c0, c0 + P, ... < b of a[c b] x[c], in column order,
+// with 16 loads in flight (the memory-level parallelism a CU needs to stream
+// at its share of the HBM bandwidth).
+template <class T>
+__device__ __forceinline__ T stream_dot(const T *__restrict__ a, const T *x, int b, int c0, int P) {
+  constexpr int U = 16;
+  T acc = T(0);
+  int c = c0;
+  for (; c + (U - 1) * P < b; c += U * P) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { v[u] = a[size_t(c + u * P) * b]; }
+#pragma unroll
+    for (int u = 0; u < U; ++u) { acc += v[u] * x[c + u * P]; }
+  }
+  for (; c < b; c += P) { acc += a[size_t(c) * b] * x[c]; }
+  return acc;
+}
+template <class T, bool LOWER>
+__global__ __launch_bounds__(1024) void k_ilu_sweep_stream(const T *__restrict__ bsr, int b, int nb, int kd,
+                                                           const int *__restrict__ row_ptr,
+                                                           const int *__restrict__ diag_ptr,
+                                                           const int *__restrict__ col, T *src, T *dst, T *arm,
+                                                           int *flags, int spin_limit) {
+  constexpr int NT = 1024, PRE = 4, KDM = 4; // cell blocks: at most 4 neighbours a triangle
+  extern __shared__ unsigned char stream_lds[];
+  T *xs = reinterpret_cast<T *>(stream_lds); // kd x b dependency values
+  T *red = xs + size_t(kd) * b;               // NT partial sums
+  T *ts = red + NT;                           // b
+  const int tid = threadIdx.x, P = NT / b, r = tid % b, part = tid / b;
+  const bool worker = part < P;
+  const size_t bb = size_t(b) * b;
+  if (LOWER) {
+    for (size_t t = size_t(blockIdx.x) * NT + tid; t < size_t(nb) * b; t += size_t(gridDim.x) * NT) {
+      arm[t] = sentinel<T>();
+    }
+  }
+  for (int idx = blockIdx.x; idx < nb; idx += gridDim.x) {
+    const int i = LOWER ? idx : nb - 1 - idx;
+    const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
+    const int nd = j1 - j0;
+    T pre[KDM][PRE];
+#pragma unroll
+    for (int d = 0; d < KDM; ++d) {
+#pragma unroll
+      for (int v = 0; v < PRE; ++v) {
+        const int c = part + v * P;
+        pre[d][v] = (d < nd && worker && c < b) ? bsr[size_t(j0 + d) * bb + size_t(c) * b + r] : T(0);
+      }
+    }
+    T own = T(0);
+    if (part == 0) {
+      own = src[size_t(i) * b + r];
+      if (!LOWER) { src[size_t(i) * b + r] = sentinel<T>(); }
+    }
+    for (int t = tid; t < nd * b; t += NT) {
+      const T *p = dst + size_t(col[j0 + t / b]) * b + t % b;
+      T x = ld_coherent(p);
+      int spins = 0;
+      while (armed(x)) {
+        if (++spins > spin_limit) {
+          atomicOr(flags, kErrSpin);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        x = ld_coherent(p);
+      }
+      xs[t] = x;
+    }
+    __syncthreads();
+    T acc = T(0);
+    if (worker) {
+#pragma unroll
+      for (int d = 0; d < KDM; ++d) {
+        if (d < nd) {
+          const T *a = bsr + size_t(j0 + d) * bb + r;
+          const T *x = xs + d * b;
+#pragma unroll
+          for (int v = 0; v < PRE; ++v) {
+            const int c = part + v * P;
+            if (c < b) { acc += pre[d][v] * x[c]; }
+          }
+          acc += stream_dot<T>(a, x, b, part + PRE * P, P);
+        }
+      }
+    }
+    red[tid] = acc;
+    __syncthreads();
+    if (LOWER) {
+      if (part == 0) {
+        T sum = T(0);
+        for (int q = 0; q < P; ++q) { sum += red[q * b + r]; }
+        st_coherent(dst + size_t(i) * b + r, own - sum);
+      }
+    } else {
+      if (part == 0) {
+        T sum = T(0);
+        for (int q = 0; q < P; ++q) { sum += red[q * b + r]; }
+        ts[r] = own - sum;
+      }
+      __syncthreads();
+      acc = T(0);
+      if (worker) {
+        acc = stream_dot<T>(bsr + size_t(diag_ptr[i]) * bb + r, ts, b, part, P);
+      }
+      red[tid] = acc;
+      __syncthreads();
+      if (part == 0) {
+        T sum = T(0);
+        for (int q = 0; q < P; ++q) { sum += red[q * b + r]; }
+        st_coherent(dst + size_t(i) * b + r, sum);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Dataflow ILU(0) factor: factor_row per row, rows in increasing order per
 // workgroup, waiting on the rows of the lower pattern.
 template <int B>
@@ -1129,6 +1271,21 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
   // (HDG, at most 7).
   if (kd == 4) { EG_SWEEP(EG_DISPATCH_B, 4) } else { EG_SWEEP(EG_DISPATCH_B_SMALL, 8) }
 #undef EG_SWEEP
+  if (!known && b > 100 && b <= 1024 && kd == 4) {
+    // large cell blocks: streamed GEMVs
+    known = true;
+    const void *fn = lower ? reinterpret_cast<const void *>(k_ilu_sweep_stream<T, true>)
+                           : reinterpret_cast<const void *>(k_ilu_sweep_stream<T, false>);
+    const size_t lds = (size_t(kd) * b + 1024 + b) * sizeof(T);
+    const int cap = coresident(fn, 1024, lds);
+    if (cap > 0) {
+      const int grid = std::min(nb, cap);
+      int spin = kSpinLimit, bv = b, kdv = kd;
+      void *args[] = {&bsr, &bv, &nb, &kdv, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin};
+      ok = hipLaunchKernel(fn, dim3(grid), dim3(1024), args, lds, s) == hipSuccess;
+      if (!ok) { (void)hipGetLastError(); }
+    }
+  }
   return known && ok;
 }
 template <class T> void launch_arm(T *v, size_t n, hipStream_t s) {

@@ -46,6 +46,8 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     require(mode == "levels" || mode == "dataflow" || mode == "dataflow-all", "Invalid EG_ILU_SCHEDULE");
     dataflow = mode != "levels";
     dataflow_factor = mode == "dataflow-all";
+    const char *graph = std::getenv("EG_FACTOR_GRAPH");
+    use_graph = !(graph && std::string(graph) == "0");
     done_factor.resize(pattern->nb);
     hip_check(hipMemsetAsync(done_factor, 0, sizeof(int) * pattern->nb, p.stream), "memset");
     sweep.resize(N);
@@ -232,6 +234,44 @@ void LinearSolver::Setup() {
     const bool df = dataflow_factor && eg::launch_ilu_factor_df(blocks, b, pt.nb, pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr,
                                                          pt.d_upd, done_factor, ++epoch_factor, p.flags, p.stream);
     if (!df) {
+      // The per-level launches are recorded once into a graph and replayed
+      // (their arguments are fixed for the solver's lifetime), which removes
+      // most of the launch gaps between the levels.
+      if (factor_exec == nullptr && use_graph) {
+        hipGraph_t graph = nullptr;
+        bool ok = hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+          FactorLevels();
+          ok = hipStreamEndCapture(p.stream, &graph) == hipSuccess && graph != nullptr;
+        }
+        ok = ok && hipGraphInstantiate(&factor_exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) { (void)hipGraphDestroy(graph); }
+        if (!ok) {
+          (void)hipGetLastError();
+          factor_exec = nullptr;
+          use_graph = false;
+        }
+      }
+      if (factor_exec) { hip_check(hipGraphLaunch(factor_exec, p.stream), "hipGraphLaunch"); }
+      else { FactorLevels(); }
+    }
+  } else {
+    eg::launch_block_inverse(blocks, b, N / b, nullptr, p.flags, p.stream);
+  }
+  if (mixed) { eg::launch_convert<double, float>(blocks, blocks32, blocks.n, p.stream); }
+  if (ilu && dataflow) {
+    eg::launch_arm<double>(sweep, N, p.stream);
+    if (mixed) { eg::launch_arm<float>(sweep32, N, p.stream); }
+  }
+  p.CheckFlags(ilu ? "Singular BSR ILU diagonal" : "Singular block Jacobi factor");
+}
+
+// ILU(0) factor, one launch (or two) per dependency level (src/block_ilu.cpp:73-86).
+void LinearSolver::FactorLevels() {
+  const int b = block;
+  const auto &pt = *pattern;
+  {
+    {
       for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
         const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
         if (resident) {
@@ -257,15 +297,10 @@ void LinearSolver::Setup() {
                                     p.stream);
       }
     }
-  } else {
-    eg::launch_block_inverse(blocks, b, N / b, nullptr, p.flags, p.stream);
   }
-  if (mixed) { eg::launch_convert<double, float>(blocks, blocks32, blocks.n, p.stream); }
-  if (ilu && dataflow) {
-    eg::launch_arm<double>(sweep, N, p.stream);
-    if (mixed) { eg::launch_arm<float>(sweep32, N, p.stream); }
-  }
-  p.CheckFlags(ilu ? "Singular BSR ILU diagonal" : "Singular block Jacobi factor");
+}
+LinearSolver::~LinearSolver() {
+  if (factor_exec) { (void)hipGraphExecDestroy(factor_exec); }
 }
 
 void LinearSolver::Apply(const double *xv, double *yv) {

@@ -294,7 +294,8 @@ def test_cli_backward_euler_update_check(gpu, tmp_path):
     assert float(comp["max_nodal_difference"]) < 1e-8 and float(comp["solution_L2_difference"]) < 1e-8
 
 
-@pytest.mark.parametrize("args", ["--nx 16 --ny 16 --k 3", "--nx 5 --ny 3 --k 2", "--nx 4 --ny 6 --k 0"])
+@pytest.mark.parametrize("args", ["--nx 16 --ny 16 --k 3", "--nx 5 --ny 3 --k 2", "--nx 4 --ny 6 --k 0",
+                                  "--nx 3 --ny 3 --k 5", "--nx 4 --ny 3 --k 6"])
 def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
     """The single-launch dataflow ILU(0) sweeps (sentinel-armed dependency
     polling) and factor (per-row completion flags) give the preconditioner of
