@@ -8,7 +8,14 @@
 namespace eg {
 constexpr int kMaxQ = 9;           // k <= 8
 constexpr int kBlock = 256;        // threads per block of the cell kernels
-constexpr int kReduceBlocks = 512; // fixed first-stage width of vector reductions
+constexpr int kReduceBlocks = 512; // largest first-stage width of vector reductions
+// First-stage width of a reduction over n entries: one 256-thread block per
+// 512 entries, at most kReduceBlocks. Fixed for a given n, so every
+// reduction of a vector has the same summation tree.
+__host__ __device__ inline int reduce_blocks(long n) {
+  const long b = (n + 511) / 512;
+  return int(b < 1 ? 1 : (b > kReduceBlocks ? kReduceBlocks : b));
+}
 
 // Geometry and 1-D tables of one problem. Node i of a cell is (ix, iy) =
 // (i % q, i / q), x fastest; cell e = iy*nx + ix; face 2e is the x-face with

@@ -207,8 +207,8 @@ __global__ __launch_bounds__(256) void k_block_gemv(const T *__restrict__ blocks
 }
 
 // ------------------------------------------------------------ Krylov vectors
-// Partial sums over a fixed grid of kReduceBlocks blocks (grid-stride), so
-// every reduction has the same tree regardless of launch timing.
+// Partial sums over a fixed grid of reduce_blocks(n) blocks (grid-stride),
+// so every reduction has the same tree regardless of launch timing.
 constexpr int kMaxVec = 64;
 
 template <class T>
@@ -238,12 +238,12 @@ __global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ V, long 
   for (int i = threadIdx.x; i < nvec; i += blockDim.x) { partials[size_t(i) * gridDim.x + blockIdx.x] = acc_s[i]; }
 }
 
-// Reduce partials[nvec][kReduceBlocks] -> proj (LDS) redundantly in every block.
-__device__ void reduce_partials(const double *__restrict__ partials, int nvec, double *proj) {
+// Reduce partials[nvec][nb] -> proj (LDS) redundantly in every block.
+__device__ void reduce_partials(const double *__restrict__ partials, int nvec, int nb, double *proj) {
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   for (int i = wid; i < nvec; i += nw) {
     double s = 0.;
-    for (int k = lane; k < kReduceBlocks; k += 64) { s += partials[size_t(i) * kReduceBlocks + k]; }
+    for (int k = lane; k < nb; k += 64) { s += partials[size_t(i) * nb + k]; }
     s = wave_sum(s);
     if (lane == 0) { proj[i] = s; }
   }
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void k_cgs_update(const T *__restrict__ V, lon
   __shared__ double proj[kMaxVec];
   __shared__ double red[16];
   __shared__ double acc_s[kMaxVec];
-  reduce_partials(partials_in, nvec, proj);
+  reduce_partials(partials_in, nvec, reduce_blocks(n), proj);
   if (blockIdx.x == 0) {
     for (int i = threadIdx.x; i < nvec; i += blockDim.x) { proj_out[i] = proj[i]; }
   }
@@ -307,7 +307,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_normalize(const double *__restrict__ partials, T *__restrict__ w,
                                                    T *__restrict__ vnext, long n, double *__restrict__ norm2_out) {
   __shared__ double proj[1];
-  reduce_partials(partials, 1, proj);
+  reduce_partials(partials, 1, reduce_blocks(n), proj);
   const double norm2 = proj[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) { norm2_out[0] = norm2; }
   const double next = sqrt(norm2);
@@ -498,12 +498,12 @@ int big_block_max() { return kBigMax; }
 template <class T>
 void launch_multidot(const T *V, long ldv, int nvec, const T *w, const T *metric, long n, double *partials,
                      hipStream_t s) {
-  k_multidot<T><<<kReduceBlocks, 256, 0, s>>>(V, ldv, nvec, w, metric, n, partials);
+  k_multidot<T><<<reduce_blocks(n), 256, 0, s>>>(V, ldv, nvec, w, metric, n, partials);
 }
 template <class T>
 void launch_cgs_update(const T *V, long ldv, int nvec, const double *partials_in, T *w, const T *metric, long n,
                        double *proj_out, double *partials_out, int norm_mode, hipStream_t s) {
-  k_cgs_update<T><<<kReduceBlocks, 256, 0, s>>>(V, ldv, nvec, partials_in, w, metric, n, proj_out, partials_out,
+  k_cgs_update<T><<<reduce_blocks(n), 256, 0, s>>>(V, ldv, nvec, partials_in, w, metric, n, proj_out, partials_out,
                                                 norm_mode);
 }
 template <class T>
@@ -516,7 +516,7 @@ void launch_lincomb(const T *V, long ldv, int count, const Coef &coef, T *out, l
 }
 template <class T>
 void launch_dot_partials(const T *a, const T *b, const T *metric, long n, double *partials, hipStream_t s) {
-  k_dot_partials<T><<<kReduceBlocks, 256, 0, s>>>(a, b, metric, n, partials);
+  k_dot_partials<T><<<reduce_blocks(n), 256, 0, s>>>(a, b, metric, n, partials);
 }
 void launch_sum_partials(const double *partials, int count, double *out, int mode, hipStream_t s, int rows) {
   k_sum_partials<<<rows, 256, 0, s>>>(partials, count, out, mode);

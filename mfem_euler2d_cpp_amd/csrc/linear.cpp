@@ -347,7 +347,7 @@ void LinearSolver::Precondition(const float *b, float *y) { precondition<float>(
 double LinearSolver::Norm(const double *v) {
   const double *metric = form == 0 ? p.d_cell_metric.p : p.d_trace_metric.p;
   eg::launch_dot_partials<double>(v, v, metric, N, p.partial, p.stream);
-  return std::sqrt(p.ReduceScalar(p.partial, eg::kReduceBlocks, 0));
+  return std::sqrt(p.ReduceScalar(p.partial, eg::reduce_blocks(N), 0));
 }
 // One restarted GMRES solve in arithmetic T on the buffers of that type.
 // On return x holds the solution and res = rhs - A x (its norm returned).
@@ -368,7 +368,7 @@ double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iter
   double *scal = p.scalars.p;
   auto norm = [&](const T *v) {
     eg::launch_dot_partials<T>(v, v, metric, N, p.partial, s);
-    return std::sqrt(p.ReduceScalar(p.partial, eg::kReduceBlocks, 0));
+    return std::sqrt(p.ReduceScalar(p.partial, eg::reduce_blocks(N), 0));
   };
   auto H = [&](int i, int j) -> double & { return h[i + size_t(j) * (restart + 1)]; };
   if (zero_guess) {

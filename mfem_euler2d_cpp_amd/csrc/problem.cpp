@@ -314,7 +314,7 @@ void Problem::SetStage(const double *dev_old, double step) {
 // src/problem.cpp:385-391
 double Problem::CellNorm(const double *dev_r) {
   eg::launch_dot_partials<double>(dev_r, dev_r, d_cell_metric, nu, partial, stream);
-  return std::sqrt(ReduceScalar(partial, eg::kReduceBlocks, 0));
+  return std::sqrt(ReduceScalar(partial, eg::reduce_blocks(nu), 0));
 }
 // src/problem.cpp:259-275
 bool Problem::Admissible(const double *dev_u) {
