@@ -427,6 +427,51 @@ __device__ __forceinline__ bool gj2_inverse(double *__restrict__ A, Gj2Lds<B, NC
   __syncthreads();
   return singular;
 }
+// Gauss-Jordan inverse of a B x B block (B <= 64) by one wavefront, lane i
+// holding row i in registers: no workgroup barrier and no LDS traffic, each
+// step a wavefront max of float pivot magnitudes (lowest lane on ties), the
+// pivot row's values by readlane and a rank-1 update. Rows stay unscaled and
+// pivoting is implicit, as in gj2_inverse: A^{-1}(step of row i, piv[j]) =
+// scale_i S(i, j). A may point to LDS or global memory.
+template <int B>
+__device__ __forceinline__ bool gj_wave(double *A) {
+  static_assert(B <= 64, "one row a lane");
+  const int lane = threadIdx.x & 63;
+  double v[B];
+#pragma unroll
+  for (int j = 0; j < B; ++j) { v[j] = lane < B ? A[lane + j * B] : 0.; }
+  bool used = lane >= B;
+  int step = 0, piv[B];
+  double scale = 1.;
+  bool singular = false;
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const float a = float(v[k]);
+    const float key = used ? -1.f : (a != a ? __int_as_float(0x7F800000) : fabsf(a));
+    const float best = wave_fmax(key);
+    const int p = __ffsll(static_cast<long long>(__ballot(key == best))) - 1;
+    const double inv = recip(read_lane(v[k], p));
+    if (!(isfinite(inv) && inv != 0.)) { singular = true; }
+    const bool pivot = lane == p;
+    const double m = pivot ? 0. : v[k] * inv;
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      if (j != k) { v[j] = fma(-m, read_lane(v[j], p), v[j]); }
+    }
+    v[k] = pivot ? 1. : -m;
+    if (pivot) {
+      used = true;
+      step = k;
+      scale = inv;
+    }
+    piv[k] = p;
+  }
+  if (lane < B) {
+#pragma unroll
+    for (int j = 0; j < B; ++j) { A[step + piv[j] * B] = scale * v[j]; }
+  }
+  return singular;
+}
 template <int B> struct Gj2Pick;
 template <> struct Gj2Pick<64> { static constexpr int NCG = 4; };
 template <> struct Gj2Pick<100> { static constexpr int NCG = 4; };
@@ -566,7 +611,7 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident(double *__
   for (int e = t; e < kIluProgMax; e += NT) { P[e] = pg[e]; } // the program array is padded by kIluProgMax
   __syncthreads();
   const int j0 = P[1], nrow = P[2], dpos = P[3], ndep = P[4];
-  double *Arow = sm, *Dep = sm + size_t(nrow) * bb, *gjs = Dep + size_t(ndep) * bb;
+  double *Arow = sm, *Dep = sm + size_t(nrow) * bb;
   for (int e = t; e < (nrow + ndep) * bb; e += NT) {
     const int blk = e / bb, o = e - blk * bb;
     const int slot = blk < nrow ? j0 + blk : P[5 + blk - nrow];
@@ -600,11 +645,13 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident(double *__
     }
     __syncthreads();
   }
-  const bool singular = gj_inverse<B, false>(Arow + size_t(dpos) * bb, gjs);
+  bool singular = false;
+  if (t < 64) { singular = gj_wave<B>(Arow + size_t(dpos) * bb); }
+  __syncthreads();
   for (int e = t; e < nrow * bb; e += NT) { bsr[size_t(j0) * bb + e] = Arow[e]; }
   if (singular && t == 0) { atomicOr(flags, kErrSingular); }
 }
-template <int B> constexpr size_t resident_gj_lds() { return size_t(gj_lds<B>()) * sizeof(double); }
+template <int B> constexpr size_t resident_gj_lds() { return 0; }
 
 // ILU(0) block products of one dependency level by row slabs: rows
 // [r0, r0 + R) of the blocks of row i depend only on the same rows of row i's
