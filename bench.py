@@ -116,8 +116,8 @@ def wave_dt(nx, ny, alpha_deg, cfl):
 def batched_cases(count, seed=0xE1E12D):
     """Independent HDG mixed-precision instances: nx, ny ~ U{8..48}, k ~
     U{1..6}, alpha ~ U(10, 80) deg, CFL ~ U[0.5, 2], 2 DIRK steps x 3 Newton
-    updates; draws whose two steps would pass the shock time are redrawn
-    (seeded)."""
+    updates; draws whose two steps would come within 10% of the shock time
+    are redrawn (seeded)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     cases = []
@@ -126,7 +126,10 @@ def batched_cases(count, seed=0xE1E12D):
         k = int(rng.integers(1, 7))
         alpha = float(rng.uniform(10, 80))
         cfl = float(rng.uniform(0.5, 2.0))
-        if 2 * wave_dt(nx, ny, alpha, cfl) >= SHOCK_TIME:
+        # the step is set from the frozen wave rate of the current state, which
+        # can be below the initial one: a 10% margin keeps both steps short
+        # of the shock time
+        if 2.2 * wave_dt(nx, ny, alpha, cfl) >= SHOCK_TIME:
             continue
         cases.append(f"--nx {nx} --ny {ny} --k {k} --alpha-deg {alpha:.6f} --cfl {cfl:.6f} --steps 2 "
                      f"--newton-steps 3 --precond block-ilu0")
@@ -175,6 +178,7 @@ def run_batched(count, workers):
            for i in both]
     return {"instances": count, "workers": workers, "completed": len(both),
             "failed": {"mixed": sum(o["status"] != 0 for o in out_b), "plain": sum(o["status"] != 0 for o in out_p)},
+            "failure_reasons": sorted({o["error"] for o in out_b + out_p if o["status"] != 0}),
             "updates": upd(out_b), "updates_per_s": rate(out_b, rep_b), "wall_s": rep_b["wall"],
             "plain_fp64_sequential_updates_per_s": rate(out_p, rep_p),
             "fp64_concurrent_updates_per_s": rate(out_c, rep_c),
