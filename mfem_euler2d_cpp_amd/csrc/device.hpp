@@ -128,10 +128,28 @@ __device__ __forceinline__ void trace(const Geo &g, const V *cell, int side, int
 }
 
 // ------------------------------------------------------------ reductions
-template <class T> __device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) { v += __shfl_down(v, off, 64); }
-  return v;
+// Sum over the 64 lanes of a wavefront, the same value in every lane, in a
+// fixed order: DPP butterflies within rows of 16 lanes (xor 1, xor 2, half-row
+// mirror, row mirror; each step adds two equal-order partial sums, so every
+// lane of a row holds the same bits), then the four row sums (a + b) + (c + d).
+template <int CTRL> __device__ __forceinline__ double dpp_add(double v) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(u)), CTRL, 0xF, 0xF, false));
+  const unsigned hi = unsigned(__builtin_amdgcn_update_dpp(0, int(unsigned(u >> 32)), CTRL, 0xF, 0xF, false));
+  return v + __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+__device__ __forceinline__ double lane_read(double v, int lane) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(u)), lane));
+  const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(u >> 32)), lane));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  v = dpp_add<0xB1>(v);  // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E>(v);  // quad_perm [2,3,0,1]
+  v = dpp_add<0x141>(v); // row_half_mirror
+  v = dpp_add<0x140>(v); // row_mirror
+  return (lane_read(v, 0) + lane_read(v, 16)) + (lane_read(v, 32) + lane_read(v, 48));
 }
 // Deterministic block sum (fixed tree); result valid in thread 0.
 template <class T> __device__ __forceinline__ T block_sum(T v, T *lds /* >= 16 */) {
@@ -143,6 +161,24 @@ template <class T> __device__ __forceinline__ T block_sum(T v, T *lds /* >= 16 *
   if (threadIdx.x == 0) {
     const int nw = (blockDim.x + 63) >> 6;
     for (int i = 0; i < nw; ++i) { s += lds[i]; }
+  }
+  __syncthreads();
+  return s;
+}
+// K block sums at once (one pair of barriers): on return thread k < K holds
+// the sum of v[k] over the block, in the same fixed tree as block_sum.
+template <int K> __device__ __forceinline__ double block_sum_k(double (&v)[K], double *lds /* >= 16 K */) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double w = wave_sum(v[k]);
+    if (lane == k) { lds[wid * K + k] = w; }
+  }
+  __syncthreads();
+  double s = 0.;
+  if (threadIdx.x < K) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) { s += lds[i * K + threadIdx.x]; }
   }
   __syncthreads();
   return s;

@@ -215,7 +215,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ V, long ldv, int nvec,
                                                   const T *__restrict__ w, const T *__restrict__ metric, long n,
                                                   double *__restrict__ partials) {
-  __shared__ double red[16];
+  __shared__ double red[16 * 8];
   __shared__ double acc_s[kMaxVec];
   for (int i0 = 0; i0 < nvec; i0 += 8) {
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -226,13 +226,8 @@ __global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ V, long 
         if (i0 + t < nvec) { acc[t] += double(V[(i0 + t) * ldv + k]) * wk; }
       }
     }
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      if (i0 + t < nvec) {
-        const double s = block_sum(acc[t], red);
-        if (threadIdx.x == 0) { acc_s[i0 + t] = s; }
-      }
-    }
+    const double s = block_sum_k<8>(acc, red);
+    if (threadIdx.x < 8 && i0 + threadIdx.x < nvec) { acc_s[i0 + threadIdx.x] = s; }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nvec; i += blockDim.x) { partials[size_t(i) * gridDim.x + blockIdx.x] = acc_s[i]; }
@@ -259,7 +254,7 @@ __global__ __launch_bounds__(256) void k_cgs_update(const T *__restrict__ V, lon
                                                     const T *__restrict__ metric, long n, double *__restrict__ proj_out,
                                                     double *__restrict__ partials_out, int norm_mode) {
   __shared__ double proj[kMaxVec];
-  __shared__ double red[16];
+  __shared__ double red[16 * 8];
   __shared__ double acc_s[kMaxVec];
   reduce_partials(partials_in, nvec, reduce_blocks(n), proj);
   if (blockIdx.x == 0) {
@@ -289,13 +284,8 @@ __global__ __launch_bounds__(256) void k_cgs_update(const T *__restrict__ V, lon
           if (i0 + t < nvec) { acc[t] += double(V[(i0 + t) * ldv + k]) * mw; }
         }
       }
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        if (i0 + t < nvec) {
-          const double sd = block_sum(acc[t], red);
-          if (threadIdx.x == 0) { acc_s[i0 + t] = sd; }
-        }
-      }
+      const double sd = block_sum_k<8>(acc, red);
+      if (threadIdx.x < 8 && i0 + threadIdx.x < nvec) { acc_s[i0 + threadIdx.x] = sd; }
     }
   }
   __syncthreads();
