@@ -151,7 +151,7 @@ public:
   DeviceArray<double> old, d_cell_metric, d_trace_metric, partial, scalars;
   DeviceArray<float> d_cell_metric32, d_trace_metric32;
   DeviceArray<int> flags;
-  double *pinned = nullptr;
+  double *pinned = nullptr; // 512 doubles (two GMRES Hessenberg slots), then the flags
   int *pinned_flags = nullptr;
   int freeze_generation = 0;
   double rate = 0;
@@ -240,6 +240,9 @@ public:
   size_t resident_lds = 0;
   DeviceArray<int> d_prog, d_prog_off;
   std::vector<double> h, cosine, sine, rotated, coefficients;
+  // Completion of the Arnoldi step whose Hessenberg column sits in pinned
+  // slot j % 2 (pinned + 256 (j % 2)).
+  hipEvent_t step_done[2] = {nullptr, nullptr};
 
   LinearSolver(Problem &p, int form, Hybrid *hybrid = nullptr);
   ~LinearSolver();
@@ -258,6 +261,7 @@ public:
   void Solve(const double *dev_rhs, LinearResult &out, double tol, bool refresh);
   double Norm(const double *v);
 private:
+  template <class T> void EnqueueArnoldi(T *Vb, T *zb, T *wb, const T *metric, int j);
   template <class T> double GMRES(const T *rhs, T *xv, double tol, int limit, int &iterations, bool zero_guess,
                                   double *true_norm);
   double MixedSolve(const double *rhs, double tol, int limit, int &iterations, int &outer, bool zero_guess);

@@ -118,7 +118,8 @@ template <class T>
 void launch_cgs_update(const T *V, long ldv, int nvec, const double *partials_in, T *w, const T *metric, long n,
                        double *proj_out, double *partials_out, int norm_mode, hipStream_t s);
 template <class T>
-void launch_normalize(const double *partials, T *w, T *vnext, long n, double *norm2_out, hipStream_t s);
+void launch_normalize(const double *partials, T *w, T *vnext, long n, double *norm2_out, const double *proj_in,
+                      int nproj, double *host_out, hipStream_t s);
 template <class T>
 void launch_lincomb(const T *V, long ldv, int count, const Coef &coef, T *out, long n, hipStream_t s);
 template <class T>

@@ -295,11 +295,21 @@ __global__ __launch_bounds__(256) void k_cgs_update(const T *__restrict__ V, lon
 // next = sqrt(sum partials); vnext = w * (1/next) if next > 0 (src/linear.cpp:117-122).
 template <class T>
 __global__ __launch_bounds__(256) void k_normalize(const double *__restrict__ partials, T *__restrict__ w,
-                                                   T *__restrict__ vnext, long n, double *__restrict__ norm2_out) {
+                                                   T *__restrict__ vnext, long n, double *__restrict__ norm2_out,
+                                                   const double *__restrict__ proj_in, int nproj,
+                                                   double *__restrict__ host_out) {
   __shared__ double proj[1];
   reduce_partials(partials, 1, reduce_blocks(n), proj);
   const double norm2 = proj[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) { norm2_out[0] = norm2; }
+  if (blockIdx.x == 0 && host_out) {
+    // the Hessenberg column: both projection passes and the squared norm
+    for (int i = threadIdx.x; i < nproj; i += blockDim.x) {
+      host_out[i] = proj_in[i];
+      host_out[64 + i] = proj_in[64 + i];
+    }
+    if (threadIdx.x == 0) { host_out[128] = norm2; }
+  }
   const double next = sqrt(norm2);
   if (next > 0) {
     const T inv = T(1 / next);
@@ -497,8 +507,9 @@ void launch_cgs_update(const T *V, long ldv, int nvec, const double *partials_in
                                                 norm_mode);
 }
 template <class T>
-void launch_normalize(const double *partials, T *w, T *vnext, long n, double *norm2_out, hipStream_t s) {
-  k_normalize<T><<<grid_for(n), 256, 0, s>>>(partials, w, vnext, n, norm2_out);
+void launch_normalize(const double *partials, T *w, T *vnext, long n, double *norm2_out, const double *proj_in,
+                      int nproj, double *host_out, hipStream_t s) {
+  k_normalize<T><<<grid_for(n), 256, 0, s>>>(partials, w, vnext, n, norm2_out, proj_in, nproj, host_out);
 }
 template <class T>
 void launch_lincomb(const T *V, long ldv, int count, const Coef &coef, T *out, long n, hipStream_t s) {
@@ -537,7 +548,8 @@ void launch_slope(const double *u, const double *base, double inv_dt, double *sl
   template void launch_multidot<T>(const T *, long, int, const T *, const T *, long, double *, hipStream_t);   \
   template void launch_cgs_update<T>(const T *, long, int, const double *, T *, const T *, long, double *,     \
                                      double *, int, hipStream_t);                                             \
-  template void launch_normalize<T>(const double *, T *, T *, long, double *, hipStream_t);                   \
+  template void launch_normalize<T>(const double *, T *, T *, long, double *, const double *, int, double *,  \
+                                     hipStream_t);                   \
   template void launch_lincomb<T>(const T *, long, int, const Coef &, T *, long, hipStream_t);                \
   template void launch_dot_partials<T>(const T *, const T *, const T *, long, double *, hipStream_t);         \
   template void launch_axpby<T>(T *, double, const T *, double, long, hipStream_t);

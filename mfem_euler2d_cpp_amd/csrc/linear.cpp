@@ -27,6 +27,7 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
   require(restart + 1 <= 64, "GMRES restart above 63 is not supported by the device solver");
   require(form == 0 || hybrid != nullptr, "Schur solver needs its condensed state");
   V.resize(size_t(N) * (restart + 1));
+  for (hipEvent_t &e : step_done) { hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"); }
   for (auto *v : {&w, &z, &comb, &corr, &res, &x, &act}) { v->resize(N); }
   partA.resize(size_t(64) * eg::kReduceBlocks);
   partB.resize(size_t(64) * eg::kReduceBlocks);
@@ -301,6 +302,9 @@ void LinearSolver::FactorLevels() {
 }
 LinearSolver::~LinearSolver() {
   if (factor_exec) { (void)hipGraphExecDestroy(factor_exec); }
+  for (hipEvent_t e : step_done) {
+    if (e) { (void)hipEventDestroy(e); }
+  }
 }
 
 void LinearSolver::Apply(const double *xv, double *yv) {
@@ -349,6 +353,24 @@ double LinearSolver::Norm(const double *v) {
   eg::launch_dot_partials<double>(v, v, metric, N, p.partial, p.stream);
   return std::sqrt(p.ReduceScalar(p.partial, eg::reduce_blocks(N), 0));
 }
+// Arnoldi step j of GMRES (src/linear.cpp:100-131): z = M^-1 v_j, w = A z,
+// CGS2 against v_0..v_j (two fused projection passes), the norm and
+// v_{j+1} = w / |w|. The Hessenberg column (both passes and |w|^2) is written
+// by the last kernel straight into pinned slot j % 2; step_done[j % 2] marks
+// its completion.
+template <class T>
+void LinearSolver::EnqueueArnoldi(T *Vb, T *zb, T *wb, const T *metric, int j) {
+  hipStream_t s = p.stream;
+  double *scal = p.scalars.p;
+  Precondition(Vb + size_t(j) * N, zb);
+  Apply(zb, wb);
+  eg::launch_multidot<T>(Vb, N, j + 1, wb, metric, N, partA, s);
+  eg::launch_cgs_update<T>(Vb, N, j + 1, partA, wb, metric, N, scal, partB, 0, s);
+  eg::launch_cgs_update<T>(Vb, N, j + 1, partB, wb, metric, N, scal + 64, partA, 1, s);
+  eg::launch_normalize<T>(partA, wb, Vb + size_t(j + 1) * N, N, scal + 128, scal, j + 1, p.pinned + 256 * (j & 1), s);
+  hip_check(hipEventRecord(step_done[j & 1], s), "hipEventRecord");
+}
+
 // One restarted GMRES solve in arithmetic T on the buffers of that type.
 // On return x holds the solution and res = rhs - A x (its norm returned).
 template <class T>
@@ -365,7 +387,6 @@ double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iter
   const T *metric = f64 ? reinterpret_cast<const T *>(form == 0 ? p.d_cell_metric.p : p.d_trace_metric.p)
                         : reinterpret_cast<const T *>(form == 0 ? p.d_cell_metric32.p : p.d_trace_metric32.p);
   hipStream_t s = p.stream;
-  double *scal = p.scalars.p;
   auto norm = [&](const T *v) {
     eg::launch_dot_partials<T>(v, v, metric, N, p.partial, s);
     return std::sqrt(p.ReduceScalar(p.partial, eg::reduce_blocks(N), 0));
@@ -394,19 +415,31 @@ double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iter
     std::fill(rotated.begin(), rotated.end(), 0.);
     rotated[0] = beta;
     int count = 0;
+    bool queued = false; // step j already enqueued (ahead of the host)
     for (int j = 0; j < restart && iterations < limit; ++j) {
-      T *vj = Vb + size_t(j) * N;
-      Precondition(vj, zb);
-      Apply(zb, wb);
-      // CGS2: two fused projection passes, then the norm and normalisation.
-      eg::launch_multidot<T>(Vb, N, j + 1, wb, metric, N, partA, s);
-      eg::launch_cgs_update<T>(Vb, N, j + 1, partA, wb, metric, N, scal, partB, 0, s);
-      eg::launch_cgs_update<T>(Vb, N, j + 1, partB, wb, metric, N, scal + 64, partA, 1, s);
-      eg::launch_normalize<T>(partA, wb, Vb + size_t(j + 1) * N, N, scal + 128, s);
-      hip_check(hipMemcpyAsync(p.pinned, scal, sizeof(double) * 129, hipMemcpyDeviceToHost, s), "readback");
-      p.Sync();
-      for (int i = 0; i <= j; ++i) { H(i, j) = (0. + p.pinned[i]) + p.pinned[64 + i]; }
-      const double next = std::sqrt(p.pinned[128]);
+      if (!queued) { EnqueueArnoldi(Vb, zb, wb, metric, j); }
+      // Enqueue step j + 1 before waiting for step j's column when the
+      // residual history says step j will not end the cycle: the device then
+      // runs on while the host applies the rotations. Step j + 1 depends only
+      // on v_{j+1}, which step j computes on the device; if step j does end
+      // the cycle, its results are never read.
+      queued = false;
+      if (j + 1 < restart && iterations + 1 < limit) {
+        bool ahead = true;
+        if (j > 0) {
+          const double gj = std::abs(rotated[j]), gp = std::abs(rotated[j - 1]);
+          const double rho = gp > 0. ? std::min(1., gj / gp) : 1.;
+          ahead = gj * rho > 3. * tol * rhs_norm;
+        }
+        if (ahead) {
+          EnqueueArnoldi(Vb, zb, wb, metric, j + 1);
+          queued = true;
+        }
+      }
+      hip_check(hipEventSynchronize(step_done[j & 1]), "hipEventSynchronize");
+      const double *col = p.pinned + 256 * (j & 1);
+      for (int i = 0; i <= j; ++i) { H(i, j) = (0. + col[i]) + col[64 + i]; }
+      const double next = std::sqrt(col[128]);
       H(j + 1, j) = next;
       for (int i = 0; i < j; ++i) {
         const double u = H(i, j), vr = H(i + 1, j);
@@ -529,6 +562,8 @@ void LinearSolver::Solve(const double *dev_rhs, LinearResult &out, double tol, b
   require(std::isfinite(out.residual) && out.residual <= std::max(10 * tol, 1e-12),
           "True inverse-mass weighted linear residual too large");
 }
+template void LinearSolver::EnqueueArnoldi<double>(double *, double *, double *, const double *, int);
+template void LinearSolver::EnqueueArnoldi<float>(float *, float *, float *, const float *, int);
 template double LinearSolver::GMRES<double>(const double *, double *, double, int, int &, bool, double *);
 template double LinearSolver::GMRES<float>(const float *, float *, double, int, int &, bool, double *);
 } // namespace euler

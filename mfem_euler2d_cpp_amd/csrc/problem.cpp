@@ -214,9 +214,9 @@ Problem::Problem(const Options &o, const EulerGas &g, hipStream_t s)
   partial.resize(2 * size_t(np));
   scalars.resize(256);
   flags.resize(1);
-  hip_check(hipHostMalloc(reinterpret_cast<void **>(&pinned), 256 * sizeof(double) + 64, hipHostMallocDefault),
+  hip_check(hipHostMalloc(reinterpret_cast<void **>(&pinned), 512 * sizeof(double) + 64, hipHostMallocDefault),
             "hipHostMalloc");
-  pinned_flags = reinterpret_cast<int *>(pinned + 256);
+  pinned_flags = reinterpret_cast<int *>(pinned + 512);
   hip_check(hipMemcpyAsync(d_cell_metric, cell_metric.data(), sizeof(double) * nu, hipMemcpyHostToDevice, stream),
             "upload metric");
   hip_check(hipMemcpyAsync(d_trace_metric, trace_metric.data(), sizeof(double) * nl, hipMemcpyHostToDevice,
