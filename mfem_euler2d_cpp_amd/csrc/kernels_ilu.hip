@@ -472,6 +472,73 @@ __device__ __forceinline__ bool gj_wave(double *A) {
   }
   return singular;
 }
+// The same Gauss-Jordan inverse for B <= 16 with a quad of lanes per row:
+// lane 4 r + q holds columns 4 q .. 4 q + 3 of row r, so a step updates four
+// values a lane. The pivot search is gj_wave's (float magnitudes of column k
+// over the unused rows, lowest row on ties), S(r, k) reaches the lanes of row
+// r by a quad broadcast and the pivot row by a lane permute; every value is
+// computed by the same operations as in gj_wave, so the results are the same.
+template <int KQ> __device__ __forceinline__ double quad_bcast(double v) {
+  constexpr int ctrl = KQ | (KQ << 2) | (KQ << 4) | (KQ << 6);
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(u)), ctrl, 0xF, 0xF, false));
+  const unsigned hi = unsigned(__builtin_amdgcn_mov_dpp(int(unsigned(u >> 32)), ctrl, 0xF, 0xF, false));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+template <int B>
+__device__ __forceinline__ bool gj_quad(double *A) {
+  static_assert(B <= 16, "one row a quad of lanes");
+  const int lane = threadIdx.x & 63, q = lane & 3, r = lane >> 2;
+  double v[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) { v[c] = (r < B && 4 * q + c < B) ? A[r + (4 * q + c) * B] : 0.; }
+  bool used = r >= B;
+  int step = 0, piv[B];
+  double scale = 1.;
+  bool singular = false;
+#pragma unroll
+  for (int k = 0; k < B; ++k) {
+    const int kq = k >> 2, kc = k & 3;
+    const float a = float(v[kc]);
+    const float key = (used || q != kq) ? -1.f : (a != a ? __int_as_float(0x7F800000) : fabsf(a));
+    const float best = wave_fmax(key);
+    const int pl = __ffsll(static_cast<long long>(__ballot(key == best))) - 1;
+    const int p = pl >> 2;
+    const double inv = recip(read_lane(v[kc], pl));
+    if (!(isfinite(inv) && inv != 0.)) { singular = true; }
+    double srk;
+    switch (kq) {
+    case 0: srk = quad_bcast<0>(v[kc]); break;
+    case 1: srk = quad_bcast<1>(v[kc]); break;
+    case 2: srk = quad_bcast<2>(v[kc]); break;
+    default: srk = quad_bcast<3>(v[kc]); break;
+    }
+    const bool pivot = r == p;
+    const double m = pivot ? 0. : srk * inv;
+    const int src = 4 * p + q;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double pr = __shfl(v[c], src, 64);
+      v[c] = (c == kc && q == kq) ? (pivot ? 1. : -m) : fma(-m, pr, v[c]);
+    }
+    if (pivot) {
+      used = true;
+      step = k;
+      scale = inv;
+    }
+    piv[k] = p;
+  }
+  if (r < B) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      int pc = 0;
+#pragma unroll
+      for (int j = c; j < B; j += 4) { pc = (j == 4 * q + c) ? piv[j] : pc; }
+      if (4 * q + c < B) { A[step + pc * B] = scale * v[c]; }
+    }
+  }
+  return singular;
+}
 template <int B> struct Gj2Pick;
 template <> struct Gj2Pick<64> { static constexpr int NCG = 4; };
 template <> struct Gj2Pick<100> { static constexpr int NCG = 4; };
@@ -599,6 +666,19 @@ __device__ __forceinline__ double block_dot(const double *X, const double *Y, in
   return acc;
 }
 template <int B>
+__device__ __forceinline__ void block_dot2(const double *X, const double *Y0, const double *Y1, int o, double &a0,
+                                           double &a1) {
+  const int r = o % B, c = o / B;
+  a0 = 0.;
+  a1 = 0.;
+#pragma unroll 4
+  for (int kk = 0; kk < B; ++kk) {
+    const double x = X[r + kk * B];
+    a0 += x * Y0[kk + c * B];
+    a1 += x * Y1[kk + c * B];
+  }
+}
+template <int B>
 __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident(double *__restrict__ bsr,
                                                                       const int *__restrict__ prog_off,
                                                                       const int *__restrict__ prog,
@@ -612,10 +692,28 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident(double *__
   __syncthreads();
   const int j0 = P[1], nrow = P[2], dpos = P[3], ndep = P[4];
   double *Arow = sm, *Dep = sm + size_t(nrow) * bb;
-  for (int e = t; e < (nrow + ndep) * bb; e += NT) {
-    const int blk = e / bb, o = e - blk * bb;
-    const int slot = blk < nrow ? j0 + blk : P[5 + blk - nrow];
-    sm[e] = bsr[size_t(slot) * bb + o];
+  {
+    // all loads of a batch in flight before the first LDS store
+    constexpr int U = 16;
+    const int total = (nrow + ndep) * bb;
+    for (int e0 = 0; e0 < total; e0 += U * NT) {
+      double tmp[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * NT + t;
+        tmp[u] = 0.;
+        if (e < total) {
+          const int blk = e / bb, o = e - blk * bb;
+          const int slot = blk < nrow ? j0 + blk : P[5 + blk - nrow];
+          tmp[u] = bsr[size_t(slot) * bb + o];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * NT + t;
+        if (e < total) { sm[e] = tmp[u]; }
+      }
+    }
   }
   __syncthreads();
   int pc = 5 + ndep;
@@ -638,15 +736,29 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident(double *__
       if (o < bb) { X[o] = res[q]; }
     }
     __syncthreads();
-    for (int e = t; e < nupd * bb; e += NT) {
-      const int u = e / bb, o = e - u * bb;
-      double *T = Arow + size_t(P[ops + 2 * u]) * bb;
-      T[o] -= block_dot<B>(X, Dep + size_t(P[ops + 2 * u + 1]) * bb, o);
+    // the update pairs of this lower block two at a time (independent
+    // accumulators, one read of L_ik for both)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int o = t + q * NT;
+      if (o < bb) {
+        int u = 0;
+        for (; u + 1 < nupd; u += 2) {
+          double a0, a1;
+          block_dot2<B>(X, Dep + size_t(P[ops + 2 * u + 1]) * bb, Dep + size_t(P[ops + 2 * u + 3]) * bb, o, a0, a1);
+          Arow[size_t(P[ops + 2 * u]) * bb + o] -= a0;
+          Arow[size_t(P[ops + 2 * u + 2]) * bb + o] -= a1;
+        }
+        if (u < nupd) { Arow[size_t(P[ops + 2 * u]) * bb + o] -= block_dot<B>(X, Dep + size_t(P[ops + 2 * u + 1]) * bb, o); }
+      }
     }
     __syncthreads();
   }
   bool singular = false;
-  if (t < 64) { singular = gj_wave<B>(Arow + size_t(dpos) * bb); }
+  if (t < 64) {
+    if constexpr (B <= 16) { singular = gj_quad<B>(Arow + size_t(dpos) * bb); }
+    else { singular = gj_wave<B>(Arow + size_t(dpos) * bb); }
+  }
   __syncthreads();
   for (int e = t; e < nrow * bb; e += NT) { bsr[size_t(j0) * bb + e] = Arow[e]; }
   if (singular && t == 0) { atomicOr(flags, kErrSingular); }
