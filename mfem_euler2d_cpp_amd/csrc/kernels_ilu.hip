@@ -8,6 +8,8 @@
 //  * ILU(0) sweeps whose block GEMVs split each row's dot product across
 //    threads, per dependency level or as one dataflow launch.
 #include <algorithm>
+#include <cstdlib>
+#include <string>
 
 #include "kernels.hpp"
 
@@ -775,6 +777,13 @@ template <int B> constexpr size_t resident_gj_lds() { return 0; }
 // the slab of the left operand sits in LDS transposed ([kk][r], broadcast
 // reads), the right operand is staged in panels of KP rows ([kk][c], padded
 // stride), prefetched into registers one panel ahead.
+// Workgroup barrier that orders LDS accesses only: outstanding global loads
+// (operand prefetches) and stores stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 template <int B> struct Slab {
   static constexpr int R = 16;
   static constexpr int LDX = R + 1; // padded slab stride (the column-wise stores of L do not conflict)
@@ -885,6 +894,152 @@ __global__ __launch_bounds__(256) void k_ilu_slab(double *__restrict__ bsr, cons
         for (int j = 0; j < S::CJ; ++j) {
           const int c = tc + 64 * j;
           if (c < B && r < B) { tgt[r + size_t(c) * B] -= acc[a][j]; }
+        }
+      }
+    }
+  }
+}
+
+// The same slab products on the FP64 matrix cores (v_mfma_f64_16x16x4f64):
+// the 16-row slab is the M dimension of the tile, each wavefront owns the
+// 16-column output tiles w, w + 4, ... Lane (g, j) = (lane / 16, lane % 16)
+// supplies k = g KS + s at step s (s < KS, 4 KS >= B, k >= B zero), so the
+// right operand of a lane is the contiguous run Y[g KS .. g KS + KS) of
+// column 16 ct + j, read straight from global memory in chunks of S steps,
+// one chunk ahead; the left operand (the slab, k-major) sits in LDS. The
+// accumulator of lane (g, j) holds rows g + 4 r (r < 4) of column 16 ct + j.
+constexpr int slab_chunk(int k4) {
+  int best = 16, waste = 1 << 30;
+  for (int c = 16; c >= 8; --c) {
+    const int w = (k4 + c - 1) / c * c - k4;
+    if (w < waste) { waste = w; best = c; }
+  }
+  return k4 < 16 ? k4 : best;
+}
+template <int B> struct SlabM {
+  static constexpr int NS = (B + 15) / 16;           // slabs of a block row
+  static constexpr int CT = (B + 15) / 16;           // 16-column output tiles
+  static constexpr int TW = (CT + 3) / 4;            // tiles per wavefront (at most)
+  static constexpr int K4 = (B + 3) / 4;
+  static constexpr int S = slab_chunk(K4);           // steps per operand chunk
+  static constexpr int KS = (K4 + S - 1) / S * S;    // steps per tile
+  static constexpr int NH = KS / S;                  // chunks per tile
+  static constexpr size_t lds() { return size_t(4) * KS * 16 * sizeof(double); }
+};
+typedef double slab_d4 __attribute__((ext_vector_type(4)));
+template <int B>
+__device__ __forceinline__ void slab_chunk_load(const double *__restrict__ Y, int ct, int h, double (&y)[SlabM<B>::S]) {
+  using M = SlabM<B>;
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = 16 * ct + (lane & 15);
+#pragma unroll
+  for (int s = 0; s < M::S; ++s) {
+    const int k = g * M::KS + h * M::S + s;
+    y[s] = (k < B && c < B) ? Y[k + size_t(c) * B] : 0.;
+  }
+}
+// acc[tw] = Xs Y for the tiles of this wavefront; pre holds the first chunk
+// of Y on entry and the first chunk of Ynext (when given) on return.
+template <int B>
+__device__ __forceinline__ void slab_mfma(const double *__restrict__ Y, const double *__restrict__ Ynext,
+                                          const double *Xs, slab_d4 (&acc)[SlabM<B>::TW],
+                                          double (&pre)[SlabM<B>::S]) {
+  using M = SlabM<B>;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
+  double cur[M::S];
+#pragma unroll
+  for (int s = 0; s < M::S; ++s) { cur[s] = pre[s]; }
+#pragma unroll
+  for (int tw = 0; tw < M::TW; ++tw) {
+    const int ct = w + 4 * tw;
+    acc[tw] = slab_d4{0., 0., 0., 0.};
+    if (ct < M::CT) {
+#pragma unroll
+      for (int h = 0; h < M::NH; ++h) {
+        double nxt[M::S];
+        if (h + 1 < M::NH) { slab_chunk_load<B>(Y, ct, h + 1, nxt); }
+        else if (tw + 1 < M::TW && ct + 4 < M::CT) { slab_chunk_load<B>(Y, ct + 4, 0, nxt); }
+        else if (Ynext) { slab_chunk_load<B>(Ynext, w, 0, nxt); }
+        else {
+#pragma unroll
+          for (int s = 0; s < M::S; ++s) { nxt[s] = 0.; }
+        }
+        slab_d4 a4 = acc[tw];
+#pragma unroll
+        for (int s = 0; s < M::S; ++s) {
+          const double a = Xs[(g * M::KS + h * M::S + s) * 16 + j];
+          a4 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, cur[s], a4, 0, 0, 0);
+        }
+        acc[tw] = a4;
+#pragma unroll
+        for (int s = 0; s < M::S; ++s) { cur[s] = nxt[s]; }
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < M::S; ++s) { pre[s] = cur[s]; }
+}
+template <int B>
+__global__ __launch_bounds__(256) void k_ilu_slab_m(double *__restrict__ bsr, const int *__restrict__ rows,
+                                                    const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
+                                                    const int *__restrict__ col, const int *__restrict__ upd_ptr,
+                                                    const int *__restrict__ upd) {
+  using M = SlabM<B>;
+  extern __shared__ double Xs[]; // [k][16], k < 4 KS
+  constexpr size_t bb = size_t(B) * B;
+  const int i = rows[blockIdx.x / M::NS], r0 = (blockIdx.x % M::NS) * 16;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, g = lane >> 4, j = lane & 15;
+  for (int e = B * 16 + t; e < 4 * M::KS * 16; e += 256) { Xs[e] = 0.; } // k >= B
+  const int ik0 = row_ptr[i], ik1 = diag_ptr[i];
+  slab_d4 acc[M::TW];
+  double pre[M::S];
+  if (ik0 < ik1) { slab_chunk_load<B>(bsr + size_t(diag_ptr[col[ik0]]) * bb, w, 0, pre); }
+  for (int ik = ik0; ik < ik1; ++ik) {
+    double *aik = bsr + size_t(ik) * bb;
+    if (ik > ik0) { __syncthreads(); } // earlier updates of A_ik are visible; Xs is free
+    for (int e = t; e < B * 16; e += 256) {
+      const int r = e & 15, k = e >> 4;
+      Xs[e] = r0 + r < B ? aik[r0 + r + size_t(k) * B] : 0.;
+    }
+    lds_barrier();
+    const int u0 = upd_ptr[ik], u1 = upd_ptr[ik + 1];
+    const double *dnext = ik + 1 < ik1 ? bsr + size_t(diag_ptr[col[ik + 1]]) * bb : nullptr;
+    slab_mfma<B>(bsr + size_t(diag_ptr[col[ik]]) * bb, u0 < u1 ? bsr + size_t(upd[2 * u0 + 1]) * bb : dnext, Xs,
+                 acc, pre); // L_ik = A_ik inv(A_kk)
+    lds_barrier(); // every read of Xs is done
+#pragma unroll
+    for (int tw = 0; tw < M::TW; ++tw) {
+      const int c = 16 * (w + 4 * tw) + j;
+      if (c < B) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = g + 4 * q;
+          Xs[c * 16 + r] = acc[tw][q];
+          if (r0 + r < B) { aik[r0 + r + size_t(c) * B] = acc[tw][q]; }
+        }
+      }
+    }
+    lds_barrier();
+    for (int u = u0; u < u1; ++u) {
+      double *tgt = bsr + size_t(upd[2 * u]) * bb;
+      double tv[M::TW][4];
+#pragma unroll
+      for (int tw = 0; tw < M::TW; ++tw) {
+        const int c = 16 * (w + 4 * tw) + j;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = r0 + g + 4 * q;
+          tv[tw][q] = (c < B && r < B) ? tgt[r + size_t(c) * B] : 0.;
+        }
+      }
+      slab_mfma<B>(bsr + size_t(upd[2 * u + 1]) * bb, u + 1 < u1 ? bsr + size_t(upd[2 * u + 3]) * bb : dnext, Xs,
+                   acc, pre); // L_ik A_kj
+#pragma unroll
+      for (int tw = 0; tw < M::TW; ++tw) {
+        const int c = 16 * (w + 4 * tw) + j;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = r0 + g + 4 * q;
+          if (c < B && r < B) { tgt[r + size_t(c) * B] = tv[tw][q] - acc[tw][q]; }
         }
       }
     }
@@ -1345,7 +1500,9 @@ int ilu_factor_lds_init() {
   EG_ALLOW_R(4) EG_ALLOW_R(8) EG_ALLOW_R(12) EG_ALLOW_R(16) EG_ALLOW_R(20) EG_ALLOW_R(24) EG_ALLOW_R(28)
   EG_ALLOW_R(32) EG_ALLOW_R(36)
 #undef EG_ALLOW_R
-#define EG_ALLOW_S(B) allow_lds(reinterpret_cast<const void *>(k_ilu_slab<B>));
+#define EG_ALLOW_S(B)                                                                              \
+  allow_lds(reinterpret_cast<const void *>(k_ilu_slab<B>));                                        \
+  allow_lds(reinterpret_cast<const void *>(k_ilu_slab_m<B>));
   EG_ALLOW_S(64) EG_ALLOW_S(100) EG_ALLOW_S(144) EG_ALLOW_S(196) EG_ALLOW_S(256) EG_ALLOW_S(324)
 #undef EG_ALLOW_S
   return 0;
@@ -1373,8 +1530,17 @@ void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const i
 bool ilu_slab_ok(int b) { return b == 64 || b == 100 || b == 144 || b == 196 || b == 256 || b == 324; }
 void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                      const int *col, const int *upd_ptr, const int *upd, hipStream_t s) {
-  EG_DISPATCH_SLAB(b, (k_ilu_slab<B><<<nrows * Slab<B>::NS, 256, Slab<B>::lds(), s>>>(bsr, rows, row_ptr, diag_ptr,
-                                                                                     col, upd_ptr, upd)));
+  static const bool vector = [] {
+    const char *e = std::getenv("EG_SLAB");
+    return e && std::string(e) == "vector";
+  }();
+  if (vector) {
+    EG_DISPATCH_SLAB(b, (k_ilu_slab<B><<<nrows * Slab<B>::NS, 256, Slab<B>::lds(), s>>>(bsr, rows, row_ptr, diag_ptr,
+                                                                                       col, upd_ptr, upd)));
+  } else {
+    EG_DISPATCH_SLAB(b, (k_ilu_slab_m<B><<<nrows * SlabM<B>::NS, 256, SlabM<B>::lds(), s>>>(
+                            bsr, rows, row_ptr, diag_ptr, col, upd_ptr, upd)));
+  }
 }
 void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
   switch (b) {
