@@ -541,6 +541,98 @@ __device__ __forceinline__ bool gj_quad(double *A) {
   }
   return singular;
 }
+// Gauss-Jordan inverse of a 64 x 64 block by B / PW wavefronts with panel
+// ownership: wave w keeps columns PW w .. PW w + PW - 1 of every row (lane =
+// row) and owns steps k of its panel. The owner searches the pivot of column
+// k (float magnitudes, lowest row on ties, as in gj2_inverse), publishes
+// the multipliers of step k (with 1 / S(p,k) in the pivot row's slot) and p
+// in LDS and raises the step counter; every other wave applies step k to its
+// columns as soon as the counter passes k. No workgroup barrier in the step
+// loop: the owner's pivot recurrence never waits for the other waves, and a
+// wave that becomes owner has applied every earlier step to its panel. Each
+// value is computed by the same operations as in gj2_inverse.
+template <int B> struct GjPanelLds {
+  double m[B][64];
+  int p[B];
+  int step;
+};
+template <int B, int PW>
+__device__ __forceinline__ bool gj_panel(double *__restrict__ A, GjPanelLds<B> &L, int *flags) {
+  static_assert(B == 64 && B % PW == 0, "one row a lane");
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  double v[PW];
+#pragma unroll
+  for (int c = 0; c < PW; ++c) { v[c] = A[lane + (PW * w + c) * B]; }
+  if (t == 0) { L.step = 0; }
+  __syncthreads();
+  bool used = false, singular = false;
+  int step = 0;
+  double scale = 1.;
+  for (int q = 0; q < B / PW; ++q) {
+    if (w == q) {
+#pragma unroll
+      for (int kc = 0; kc < PW; ++kc) {
+        const int k = PW * q + kc;
+        const float a = float(v[kc]);
+        const float key = used ? -1.f : (a != a ? __int_as_float(0x7F800000) : fabsf(a));
+        const float best = wave_fmax(key);
+        const int p = __ffsll(static_cast<long long>(__ballot(key == best))) - 1;
+        const double inv = recip(read_lane(v[kc], p));
+        if (!(isfinite(inv) && inv != 0.)) { singular = true; }
+        const bool pivot = lane == p;
+        const double m = pivot ? 0. : v[kc] * inv;
+        if (kc + 1 < PW) { v[kc + 1] = fma(-m, read_lane(v[kc + 1], p), v[kc + 1]); }
+        L.m[k][lane] = pivot ? inv : m;
+        L.p[k] = p;
+#pragma unroll
+        for (int c = 0; c < PW; ++c) {
+          if (c != kc && c != kc + 1) { v[c] = fma(-m, read_lane(v[c], p), v[c]); }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __hip_atomic_store(&L.step, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        v[kc] = pivot ? 1. : -m;
+        used = used || pivot;
+        step = pivot ? k : step;
+        scale = pivot ? inv : scale;
+      }
+    } else {
+      for (int kc = 0; kc < PW; ++kc) {
+        const int k = PW * q + kc;
+        int spins = 0;
+        while (__hip_atomic_load(&L.step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) <= k) {
+          if (++spins > (1 << 22)) {
+            atomicOr(flags, kErrSpin);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        const int p = __builtin_amdgcn_readfirstlane(L.p[k]);
+        const double mv = L.m[k][lane];
+        const bool pivot = lane == p;
+        const double m = pivot ? 0. : mv;
+#pragma unroll
+        for (int c = 0; c < PW; ++c) { v[c] = fma(-m, read_lane(v[c], p), v[c]); }
+        used = used || pivot;
+        step = pivot ? k : step;
+        scale = pivot ? mv : scale;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < PW; ++c) { A[step + L.p[PW * w + c] * B] = scale * v[c]; }
+  __syncthreads();
+  return singular;
+}
+template <int B, int PW>
+__global__ __launch_bounds__(64 * B / PW) void k_block_inverse_p(double *__restrict__ blocks,
+                                                                 const int *__restrict__ index,
+                                                                 int *__restrict__ flags) {
+  __shared__ GjPanelLds<B> L;
+  const int blk = index ? index[blockIdx.x] : blockIdx.x;
+  if (gj_panel<B, PW>(blocks + size_t(blk) * B * B, L, flags) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
+}
 template <int B> struct Gj2Pick;
 template <> struct Gj2Pick<64> { static constexpr int NCG = 4; };
 template <> struct Gj2Pick<100> { static constexpr int NCG = 4; };
@@ -1544,7 +1636,7 @@ void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *
 }
 void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
   switch (b) {
-  case 64: k_block_inverse_w<64><<<count, gj2_threads<64>(), 0, s>>>(blocks, index, flags); return;
+  case 64: k_block_inverse_p<64, 8><<<count, 512, 0, s>>>(blocks, index, flags); return;
   case 100: k_block_inverse_w<100><<<count, gj2_threads<100>(), 0, s>>>(blocks, index, flags); return;
   case 144: k_block_inverse_w<144><<<count, gj2_threads<144>(), 0, s>>>(blocks, index, flags); return;
   case 196: k_block_inverse_w<196><<<count, gj2_threads<196>(), 0, s>>>(blocks, index, flags); return;
