@@ -178,6 +178,9 @@ public:
   void Sync();
   int TakeFlags();
   void CheckFlags(const char *context);
+  // The same checks on the flags last copied to pinned memory (by
+  // ReduceScalar), without a readback of their own.
+  void CheckPinnedFlags(const char *context);
   double ReduceScalar(const double *dev_partials, int count, int mode);
 };
 
@@ -252,7 +255,11 @@ public:
   void PlanResident();
   void FactorLevels();
   void Refresh(const double *dev_u) { state = dev_u; }
-  void Setup();  // fill + factor the preconditioner at the current state
+  // Fill + factor the preconditioner at the current state. check_flags
+  // false leaves the singular-factor check to the next readback of Solve
+  // (pending_check).
+  void Setup(bool check_flags = true);
+  const char *pending_check = nullptr;
   void Apply(const double *x, double *y);
   void Apply(const float *x, float *y);
   void Precondition(const double *b, double *y);
@@ -263,8 +270,10 @@ public:
 private:
   template <class T> void EnqueueArnoldi(T *Vb, T *zb, T *wb, const T *metric, int j);
   template <class T> double GMRES(const T *rhs, T *xv, double tol, int limit, int &iterations, bool zero_guess,
-                                  double *true_norm);
-  double MixedSolve(const double *rhs, double tol, int limit, int &iterations, int &outer, bool zero_guess);
+                                  double *true_norm,
+                                    double known_rhs_norm = -1.);
+  double MixedSolve(const double *rhs, double tol, int limit, int &iterations, int &outer, bool zero_guess,
+                    double rhs_norm);
 };
 
 // include/euler.hpp:250-256

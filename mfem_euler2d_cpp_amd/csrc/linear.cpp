@@ -213,7 +213,7 @@ void LinearSolver::PlanResident() {
 // Fill and factor the preconditioner at the current Newton state
 // (src/linear.cpp:182-203; src/partial.cpp:140-173,265-298;
 // src/operators.cpp:48-74,269-303; src/block_ilu.cpp:73-86).
-void LinearSolver::Setup() {
+void LinearSolver::Setup(bool check_flags) {
   const int b = block;
   if (form == 0) {
     require(state != nullptr, "DG solver has no refreshed state");
@@ -264,7 +264,9 @@ void LinearSolver::Setup() {
     eg::launch_arm<double>(sweep, N, p.stream);
     if (mixed) { eg::launch_arm<float>(sweep32, N, p.stream); }
   }
-  p.CheckFlags(ilu ? "Singular BSR ILU diagonal" : "Singular block Jacobi factor");
+  const char *context = ilu ? "Singular BSR ILU diagonal" : "Singular block Jacobi factor";
+  if (check_flags) { p.CheckFlags(context); }
+  else { pending_check = context; }
 }
 
 // ILU(0) factor, one launch (or two) per dependency level (src/block_ilu.cpp:73-86).
@@ -375,7 +377,7 @@ void LinearSolver::EnqueueArnoldi(T *Vb, T *zb, T *wb, const T *metric, int j) {
 // On return x holds the solution and res = rhs - A x (its norm returned).
 template <class T>
 double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iterations, bool zero_guess,
-                           double *true_norm) {
+                           double *true_norm, double known_rhs_norm) {
   const bool f64 = sizeof(T) == 8;
   T *Vb = f64 ? reinterpret_cast<T *>(V.p) : reinterpret_cast<T *>(V32.p);
   T *wb = f64 ? reinterpret_cast<T *>(w.p) : reinterpret_cast<T *>(w32.p);
@@ -400,7 +402,7 @@ double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iter
     hip_check(hipMemcpyAsync(rb, rhs, sizeof(T) * N, hipMemcpyDeviceToDevice, s), "copy");
     eg::launch_axpby<T>(rb, -1., ab, 1., N, s);
   }
-  const double rhs_norm = norm(rhs);
+  const double rhs_norm = known_rhs_norm >= 0. ? known_rhs_norm : norm(rhs);
   iterations = 0;
   if (rhs_norm == 0.) {
     if (true_norm) { *true_norm = 0.; }
@@ -481,7 +483,7 @@ double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iter
 
 // FP64 refinement around the FP32 inner GMRES (precision "mixed").
 double LinearSolver::MixedSolve(const double *rhs, double tol, int limit, int &iterations, int &outer,
-                                bool zero_guess) {
+                                bool zero_guess, double rhs_norm) {
   hipStream_t s = p.stream;
   if (zero_guess) {
     hip_check(hipMemsetAsync(x, 0, sizeof(double) * N, s), "memset");
@@ -490,7 +492,6 @@ double LinearSolver::MixedSolve(const double *rhs, double tol, int limit, int &i
     Apply(x.p, act.p);
     eg::launch_sub(rhs, act, res, N, s);
   }
-  const double rhs_norm = Norm(rhs);
   iterations = 0;
   outer = 0;
   if (rhs_norm == 0.) { return 0.; }
@@ -522,7 +523,7 @@ double LinearSolver::MixedSolve(const double *rhs, double tol, int limit, int &i
       // single precision): finish in FP64 from the current iterate.
       int rest = 0;
       double true_norm = 0.;
-      GMRES<double>(rhs, x.p, tol, limit - iterations, rest, false, &true_norm);
+      GMRES<double>(rhs, x.p, tol, limit - iterations, rest, false, &true_norm, rhs_norm);
       iterations += rest;
       ++outer;
       return true_norm;
@@ -536,12 +537,17 @@ double LinearSolver::MixedSolve(const double *rhs, double tol, int limit, int &i
 void LinearSolver::Solve(const double *dev_rhs, LinearResult &out, double tol, bool refresh) {
   out = LinearResult{};
   double start = clock_s();
-  if (refresh) { Setup(); }
+  if (refresh) { Setup(false); }
   out.setup = clock_s() - start;
   start = clock_s();
   int its = 0, outer = 0;
   double true_norm = 0.;
-  const double rhs_norm = Norm(dev_rhs);
+  const double rhs_norm = Norm(dev_rhs); // its readback also brings the factor's flags
+  if (pending_check) {
+    const char *context = pending_check;
+    pending_check = nullptr;
+    p.CheckPinnedFlags(context);
+  }
   if (rhs_norm == 0.) {
     hip_check(hipMemsetAsync(x, 0, sizeof(double) * N, p.stream), "memset");
     hip_check(hipMemsetAsync(res, 0, sizeof(double) * N, p.stream), "memset");
@@ -549,9 +555,9 @@ void LinearSolver::Solve(const double *dev_rhs, LinearResult &out, double tol, b
     return;
   }
   if (mixed) {
-    true_norm = MixedSolve(dev_rhs, tol, p.opt.max_krylov, its, outer, refresh);
+    true_norm = MixedSolve(dev_rhs, tol, p.opt.max_krylov, its, outer, refresh, rhs_norm);
   } else {
-    GMRES<double>(dev_rhs, x.p, tol, p.opt.max_krylov, its, refresh, &true_norm);
+    GMRES<double>(dev_rhs, x.p, tol, p.opt.max_krylov, its, refresh, &true_norm, rhs_norm);
   }
   out.solve = clock_s() - start;
   out.count = its;
@@ -564,6 +570,6 @@ void LinearSolver::Solve(const double *dev_rhs, LinearResult &out, double tol, b
 }
 template void LinearSolver::EnqueueArnoldi<double>(double *, double *, double *, const double *, int);
 template void LinearSolver::EnqueueArnoldi<float>(float *, float *, float *, const float *, int);
-template double LinearSolver::GMRES<double>(const double *, double *, double, int, int &, bool, double *);
-template double LinearSolver::GMRES<float>(const float *, float *, double, int, int &, bool, double *);
+template double LinearSolver::GMRES<double>(const double *, double *, double, int, int &, bool, double *, double);
+template double LinearSolver::GMRES<float>(const float *, float *, double, int, int &, bool, double *, double);
 } // namespace euler

@@ -253,6 +253,9 @@ int Problem::TakeFlags() {
 void Problem::CheckFlags(const char *context) {
   hip_check(hipMemcpyAsync(pinned_flags, flags, sizeof(int), hipMemcpyDeviceToHost, stream), "readback");
   Sync();
+  CheckPinnedFlags(context);
+}
+void Problem::CheckPinnedFlags(const char *context) {
   const int f = TakeFlags();
   if (f & eg::kErrSpin) { throw std::runtime_error("ILU dataflow dependency wait exceeded its bound"); }
   if (f & eg::kErrSingular) { throw std::runtime_error(context); }
