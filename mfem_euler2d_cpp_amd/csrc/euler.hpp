@@ -246,6 +246,7 @@ public:
   // Completion of the Arnoldi step whose Hessenberg column sits in pinned
   // slot j % 2 (pinned + 256 (j % 2)).
   hipEvent_t step_done[2] = {nullptr, nullptr};
+  hipEvent_t setup_mark[2] = {nullptr, nullptr}; // around the set-up on the stream (its device time)
 
   LinearSolver(Problem &p, int form, Hybrid *hybrid = nullptr);
   ~LinearSolver();

@@ -588,7 +588,10 @@ __device__ __forceinline__ bool gj_panel(double *__restrict__ A, GjPanelLds<B> &
         for (int c = 0; c < PW; ++c) {
           if (c != kc && c != kc + 1) { v[c] = fma(-m, read_lane(v[c], p), v[c]); }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        // The LDS operations of one wavefront complete in issue order, so the
+        // multipliers are visible before the counter that announces them; a
+        // compiler fence keeps that order without waiting for the stores.
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __hip_atomic_store(&L.step, k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         v[kc] = pivot ? 1. : -m;
         used = used || pivot;
