@@ -179,6 +179,7 @@ def run_batched(count, workers):
     return {"instances": count, "workers": workers, "completed": len(both),
             "failed": {"mixed": sum(o["status"] != 0 for o in out_b), "plain": sum(o["status"] != 0 for o in out_p)},
             "failure_reasons": sorted({o["error"] for o in out_b + out_p if o["status"] != 0}),
+            "failed_cases": [cases[i] for i in range(count) if out_b[i]["status"] != 0 or out_p[i]["status"] != 0],
             "updates": upd(out_b), "updates_per_s": rate(out_b, rep_b), "wall_s": rep_b["wall"],
             "plain_fp64_sequential_updates_per_s": rate(out_p, rep_p),
             "fp64_concurrent_updates_per_s": rate(out_c, rep_c),
