@@ -242,6 +242,11 @@ public:
   bool resident = false;
   size_t resident_lds = 0;
   DeviceArray<int> d_prog, d_prog_off;
+  // Blocks up to 64: the factor's upper blocks are scaled in place to
+  // inv(U_ii) U_ij after each factorisation (one product a row in the
+  // backward sweep); the upper block slots and their rows' diagonal slots.
+  bool scaled_upper = false;
+  DeviceArray<int> d_up_slot, d_up_diag;
   std::vector<double> h, cosine, sine, rotated, coefficients;
   // Completion of the Arnoldi step whose Hessenberg column sits in pinned
   // slot j % 2 (pinned + 256 (j % 2)).

@@ -67,7 +67,7 @@ void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, con
                             const int *diag_ptr, const int *col, const T *rhs, T *w, int threads, hipStream_t s);
 template <class T>
 void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
-                            const int *diag_ptr, const int *col, T *w, int threads, hipStream_t s);
+                            const int *diag_ptr, const int *col, T *w, int threads, bool scaled, hipStream_t s);
 
 // Register-resident / split-dot variants (kernels_ilu.hip) used by the
 // launchers above when the block size allows.
@@ -89,6 +89,10 @@ void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const i
 // Row-slab block products of one level for b in {64, 100, 144, 196, 256,
 // 324} (the diagonal inverses follow with launch_block_inverse).
 bool ilu_slab_ok(int b);
+// In place M_ij = inv(U_ii) U_ij for the upper blocks (slot[i], diagonal
+// dslot[i]) after the factor, for block sizes where ilu_scale_upper_ok.
+bool ilu_scale_upper_ok(int b);
+void launch_ilu_scale_upper(double *bsr, int b, const int *slot, const int *dslot, int count, hipStream_t s);
 
 void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                      const int *col, const int *upd_ptr, const int *upd, hipStream_t s);
@@ -101,7 +105,8 @@ void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *
 // final output). The intermediate must be armed before the first lower sweep.
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
-                         const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, hipStream_t s);
+                         const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, bool scaled,
+                         hipStream_t s);
 template <class T> void launch_arm(T *v, size_t n, hipStream_t s);
 
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,

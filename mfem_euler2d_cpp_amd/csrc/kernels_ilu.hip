@@ -8,6 +8,7 @@
 //  * ILU(0) sweeps whose block GEMVs split each row's dot product across
 //    threads, per dependency level or as one dataflow launch.
 #include <algorithm>
+#include <stdexcept>
 #include <cstdlib>
 #include <string>
 
@@ -1141,6 +1142,59 @@ __global__ __launch_bounds__(256) void k_ilu_slab_m(double *__restrict__ bsr, co
   }
 }
 
+// M_ij = inv(U_ii) U_ij in place for every block right of the diagonal
+// (blocks up to 64, after the factor), one workgroup a block: both blocks in
+// LDS, 4 x 4 output tiles a thread, the k sum in ascending order.
+template <int B>
+__global__ __launch_bounds__(256) void k_ilu_scale_upper(double *__restrict__ bsr, const int *__restrict__ slot,
+                                                         const int *__restrict__ dslot) {
+  static_assert(B % 4 == 0 && B <= 64, "4 x 4 tiles of a block up to 64");
+  constexpr int bb = B * B, NQ = (bb + 255) / 256, T4 = B / 4;
+  extern __shared__ double sh[];
+  double *Ds = sh, *Us = sh + bb;
+  const int t = threadIdx.x;
+  const double *D = bsr + size_t(dslot[blockIdx.x]) * bb;
+  double *U = bsr + size_t(slot[blockIdx.x]) * bb;
+  double dv[NQ], uv[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int e = t + 256 * q;
+    dv[q] = e < bb ? D[e] : 0.;
+    uv[q] = e < bb ? U[e] : 0.;
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int e = t + 256 * q;
+    if (e < bb) {
+      Ds[e] = dv[q];
+      Us[e] = uv[q];
+    }
+  }
+  __syncthreads();
+  for (int tile = t; tile < T4 * T4; tile += 256) {
+    const int r0 = 4 * (tile % T4), c0 = 4 * (tile / T4);
+    double acc[4][4] = {};
+    for (int k = 0; k < B; ++k) {
+      double d[4], u[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        d[a] = Ds[r0 + a + k * B];
+        u[a] = Us[k + (c0 + a) * B];
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { acc[a][c] = fma(d[a], u[c], acc[a][c]); }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) { U[r0 + a + (c0 + c) * B] = acc[a][c]; }
+    }
+  }
+}
+
 // ------------------------------------------------------------ sweeps
 // Dot products of block row r with x, split over P = blockDim/b threads
 // (columns c = part, part + P, ...), four independent loads in flight.
@@ -1183,16 +1237,41 @@ __global__ __launch_bounds__(NT) void k_ilu_lower_split(const T *__restrict__ bs
 }
 // Backward sweep of one level (src/block_ilu.cpp:100-108):
 // w_i = inv(U_ii) (w_i - sum_{j>i} U_ij w_j).
+// With scaled upper blocks (M_ij = inv(U_ii) U_ij, stored in place of U_ij
+// after the factor): w_i = inv(U_ii) w_i - sum_{j>i} M_ij w_j.
 template <class T, int NT>
 __global__ __launch_bounds__(NT) void k_ilu_upper_split(const T *__restrict__ bsr, int b, const int *__restrict__ rows,
                                                        const int *__restrict__ row_ptr,
                                                        const int *__restrict__ diag_ptr, const int *__restrict__ col,
-                                                       T *__restrict__ w) {
+                                                       T *__restrict__ w, int scaled) {
   __shared__ T red[NT];
   __shared__ T ts[kSweepMaxBlock];
   const int i = rows[blockIdx.x];
   const size_t bb = size_t(b) * b;
   const int P = NT / b, r = threadIdx.x % b, part = threadIdx.x / b;
+  if (scaled) {
+    T c = part < P ? row_dot(bsr + diag_ptr[i] * bb, w + size_t(i) * b, b, r, part, P) : T(0);
+    red[threadIdx.x] = c;
+    __syncthreads();
+    if (part == 0) {
+      T s = T(0);
+      for (int q = 0; q < P; ++q) { s += red[q * b + r]; }
+      ts[r] = s;
+    }
+    __syncthreads();
+    T acc = T(0);
+    if (part < P) {
+      for (int ij = diag_ptr[i] + 1; ij < row_ptr[i + 1]; ++ij) { acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P); }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (part == 0) {
+      T s = T(0);
+      for (int q = 0; q < P; ++q) { s += red[q * b + r]; }
+      w[size_t(i) * b + r] = ts[r] - s;
+    }
+    return;
+  }
   T acc = T(0);
   if (part < P) {
     for (int ij = diag_ptr[i] + 1; ij < row_ptr[i + 1]; ++ij) { acc += row_dot(bsr + ij * bb, w + size_t(col[ij]) * b, b, r, part, P); }
@@ -1284,12 +1363,16 @@ template <int B> struct SweepShape {
 // dst = w, and the kernel arms `arm` (the upper sweep's output).
 // UPPER: y_i = inv(U_ii) (w_i - sum_{j>i} U_ij y_j) (src/block_ilu.cpp:100-108);
 // src = w, re-armed row by row after it is read, dst = y.
+// SCALED (upper, blocks up to 64): the blocks right of the diagonal hold
+// M_ij = inv(U_ii) U_ij, so y_i = c_i - sum_{j>i} M_ij y_j with c_i =
+// inv(U_ii) w_i formed before the wait: one product after it, as in the
+// lower sweep.
 template <class T, int B, int KD, bool LOWER>
 __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__restrict__ bsr, int nb,
                                                                     const int *__restrict__ row_ptr,
                                                                     const int *__restrict__ diag_ptr,
                                                                     const int *__restrict__ col, T *src, T *dst,
-                                                                    T *arm, int *flags, int spin_limit) {
+                                                                    T *arm, int *flags, int spin_limit, int scaled) {
   constexpr int NT = SweepShape<B>::NT, P = SweepShape<B>::P, VB = SweepShape<B>::VB;
   constexpr int KB = LOWER ? KD : KD + 1; // blocks held per row (upper: + inv(U_ii))
   constexpr size_t bb = size_t(B) * B;
@@ -1319,7 +1402,21 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
       }
     }
     T own = T(0);
-    if (part == 0) {
+    if (!LOWER && scaled) {
+      T cacc = T(0);
+#pragma unroll
+      for (int v = 0; v < VB; ++v) {
+        const int c = part + v * P;
+        if (worker && c < B) { cacc += a[KD][v] * src[size_t(i) * B + c]; }
+      }
+      red[tid] = cacc;
+      __syncthreads(); // every read of w_i is done
+      if (part == 0) {
+        for (int q = 0; q < P; ++q) { own += red[q * B + r]; }
+        src[size_t(i) * B + r] = sentinel<T>();
+      }
+      __syncthreads(); // red is free again
+    } else if (part == 0) {
       own = src[size_t(i) * B + r];
       if (!LOWER) { src[size_t(i) * B + r] = sentinel<T>(); }
     }
@@ -1354,10 +1451,10 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
     if (part == 0) {
       T sum = T(0);
       for (int q = 0; q < P; ++q) { sum += red[q * B + r]; }
-      if (LOWER) { st_coherent(dst + size_t(i) * B + r, own - sum); }
+      if (LOWER || scaled) { st_coherent(dst + size_t(i) * B + r, own - sum); }
       else { ts[r] = own - sum; }
     }
-    if (!LOWER) {
+    if (!LOWER && !scaled) {
       __syncthreads();
       acc = T(0);
 #pragma unroll
@@ -1600,6 +1697,7 @@ int ilu_factor_lds_init() {
   allow_lds(reinterpret_cast<const void *>(k_ilu_slab_m<B>));
   EG_ALLOW_S(64) EG_ALLOW_S(100) EG_ALLOW_S(144) EG_ALLOW_S(196) EG_ALLOW_S(256) EG_ALLOW_S(324)
 #undef EG_ALLOW_S
+  allow_lds(reinterpret_cast<const void *>(k_ilu_scale_upper<64>));
   return 0;
 }
 bool ilu_resident_ok(int b) { return b % 4 == 0 && b >= 4 && b <= 36; }
@@ -1622,6 +1720,19 @@ void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const i
   case 324: { constexpr int B = 324; CALL; } break;                                                \
   default: break;                                                                                  \
   }
+bool ilu_scale_upper_ok(int b) { return b % 4 == 0 && b >= 4 && (b <= 36 || b == 64); }
+void launch_ilu_scale_upper(double *bsr, int b, const int *slot, const int *dslot, int count, hipStream_t s) {
+  if (count <= 0) { return; }
+  const size_t lds = 2 * size_t(b) * b * sizeof(double);
+  switch (b) {
+#define EG_SC(BB)                                                                                  \
+  case BB: k_ilu_scale_upper<BB><<<count, 256, lds, s>>>(bsr, slot, dslot); return;
+    EG_SC(4) EG_SC(8) EG_SC(12) EG_SC(16) EG_SC(20) EG_SC(24) EG_SC(28) EG_SC(32) EG_SC(36) EG_SC(64)
+#undef EG_SC
+  default: break;
+  }
+  throw std::runtime_error("scaled upper blocks: unsupported block size");
+}
 bool ilu_slab_ok(int b) { return b == 64 || b == 100 || b == 144 || b == 196 || b == 256 || b == 324; }
 void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                      const int *col, const int *upd_ptr, const int *upd, hipStream_t s) {
@@ -1662,14 +1773,19 @@ void launch_ilu_lower_level(const T *bsr, int b, const int *rows, int nrows, con
 }
 template <class T>
 void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, const int *row_ptr,
-                            const int *diag_ptr, const int *col, T *w, int threads, hipStream_t s) {
+                            const int *diag_ptr, const int *col, T *w, int threads, bool scaled, hipStream_t s) {
   if (threads == 0) { threads = b <= 64 ? 256 : 1024; }
-  if (threads == 256 && b <= 256) { k_ilu_upper_split<T, 256><<<nrows, 256, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w); }
-  else { k_ilu_upper_split<T, 1024><<<nrows, 1024, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w); }
+  const int sc = scaled ? 1 : 0;
+  if (threads == 256 && b <= 256) {
+    k_ilu_upper_split<T, 256><<<nrows, 256, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w, sc);
+  } else {
+    k_ilu_upper_split<T, 1024><<<nrows, 1024, 0, s>>>(bsr, b, rows, row_ptr, diag_ptr, col, w, sc);
+  }
 }
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
-                         const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, hipStream_t s) {
+                         const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, bool scaled,
+                         hipStream_t s) {
   if (max_deps > 8) { return false; }
   const int kd = max_deps <= 4 ? 4 : 8;
   bool ok = false, known = false;
@@ -1678,8 +1794,8 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
     const int cap = coresident(fn, threads, 0);
     if (cap <= 0) { return; }
     const int grid = std::min(nb, cap);
-    int spin = kSpinLimit;
-    void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin};
+    int spin = kSpinLimit, sc = scaled ? 1 : 0;
+    void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin, &sc};
     ok = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s) == hipSuccess;
     if (!ok) { (void)hipGetLastError(); }
   };
@@ -1735,13 +1851,13 @@ template void launch_ilu_lower_level<double>(const double *, int, const int *, i
 template void launch_ilu_lower_level<float>(const float *, int, const int *, int, const int *, const int *,
                                             const int *, const float *, float *, int, hipStream_t);
 template void launch_ilu_upper_level<double>(const double *, int, const int *, int, const int *, const int *,
-                                             const int *, double *, int, hipStream_t);
+                                             const int *, double *, int, bool, hipStream_t);
 template void launch_ilu_upper_level<float>(const float *, int, const int *, int, const int *, const int *,
-                                            const int *, float *, int, hipStream_t);
+                                            const int *, float *, int, bool, hipStream_t);
 template bool launch_ilu_sweep_df<double>(bool, const double *, int, int, int, const int *, const int *, const int *,
-                                          double *, double *, double *, int *, hipStream_t);
+                                          double *, double *, double *, int *, bool, hipStream_t);
 template bool launch_ilu_sweep_df<float>(bool, const float *, int, int, int, const int *, const int *, const int *,
-                                         float *, float *, float *, int *, hipStream_t);
+                                         float *, float *, float *, int *, bool, hipStream_t);
 template void launch_arm<double>(double *, size_t, hipStream_t);
 template void launch_arm<float>(float *, size_t, hipStream_t);
 } // namespace eg

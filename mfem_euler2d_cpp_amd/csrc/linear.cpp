@@ -56,6 +56,29 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     if (mixed) { sweep32.resize(N); }
     if (eg::ilu_resident_ok(block)) { PlanResident(); }
     if (eg::big_block(block) && !eg::ilu_slab_ok(block)) { PlanBigFactor(); }
+    // EG_SCALED_UPPER=0 keeps U_ij unscaled (two products a row in the
+    // backward sweep, as the reference writes it).
+    const char *su = std::getenv("EG_SCALED_UPPER");
+    scaled_upper = eg::ilu_scale_upper_ok(block) && !(su && std::string(su) == "0");
+    if (scaled_upper) {
+      const auto &pt = *pattern;
+      std::vector<int> us, ud;
+      for (int i = 0; i < pt.nb; ++i) {
+        for (int ij = pt.diag[i] + 1; ij < pt.row[i + 1]; ++ij) {
+          us.push_back(ij);
+          ud.push_back(pt.diag[i]);
+        }
+      }
+      d_up_slot.resize(us.size());
+      d_up_diag.resize(ud.size());
+      if (!us.empty()) {
+        hip_check(hipMemcpyAsync(d_up_slot, us.data(), sizeof(int) * us.size(), hipMemcpyHostToDevice, p.stream),
+                  "upload");
+        hip_check(hipMemcpyAsync(d_up_diag, ud.data(), sizeof(int) * ud.size(), hipMemcpyHostToDevice, p.stream),
+                  "upload");
+      }
+      p.Sync(); // the host vectors go out of scope
+    }
     if (form == 0) {
       std::vector<int> ds(p.n), sl(4 * p.n);
       for (int e = 0; e < p.n; ++e) {
@@ -260,6 +283,9 @@ void LinearSolver::Setup(bool check_flags) {
   } else {
     eg::launch_block_inverse(blocks, b, N / b, nullptr, p.flags, p.stream);
   }
+  if (scaled_upper) {
+    eg::launch_ilu_scale_upper(blocks, b, d_up_slot, d_up_diag, static_cast<int>(d_up_slot.n), p.stream);
+  }
   if (mixed) { eg::launch_convert<double, float>(blocks, blocks32, blocks.n, p.stream); }
   if (ilu && dataflow) {
     eg::launch_arm<double>(sweep, N, p.stream);
@@ -331,9 +357,9 @@ template <class T> static void precondition(LinearSolver &s, const T *bsr, const
     require(b != y, "ILU apply needs distinct input and output vectors");
     T *mid = sizeof(T) == 8 ? reinterpret_cast<T *>(s.sweep.p) : reinterpret_cast<T *>(s.sweep32.p);
     if (eg::launch_ilu_sweep_df<T>(true, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col,
-                                   const_cast<T *>(b), mid, y, s.p.flags, s.p.stream)) {
+                                   const_cast<T *>(b), mid, y, s.p.flags, false, s.p.stream)) {
       if (!eg::launch_ilu_sweep_df<T>(false, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col, mid,
-                                      y, y, s.p.flags, s.p.stream)) {
+                                      y, y, s.p.flags, s.scaled_upper, s.p.stream)) {
         throw std::runtime_error("Co-resident launch of the upper ILU sweep failed");
       }
       return;
@@ -348,7 +374,7 @@ template <class T> static void precondition(LinearSolver &s, const T *bsr, const
   for (size_t l = 0; l + 1 < pt.upper_level_ptr.size(); ++l) {
     const int first = pt.upper_level_ptr[l], count = pt.upper_level_ptr[l + 1] - first;
     eg::launch_ilu_upper_level<T>(bsr, s.block, pt.d_upper_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col, y,
-                                  s.p.opt.sweep_threads, s.p.stream);
+                                  s.p.opt.sweep_threads, s.scaled_upper, s.p.stream);
   }
 }
 void LinearSolver::Precondition(const double *b, double *y) { precondition<double>(*this, blocks, b, y); }
