@@ -1736,10 +1736,8 @@ void launch_ilu_scale_upper(double *bsr, int b, const int *slot, const int *dslo
 bool ilu_slab_ok(int b) { return b == 64 || b == 100 || b == 144 || b == 196 || b == 256 || b == 324; }
 void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                      const int *col, const int *upd_ptr, const int *upd, hipStream_t s) {
-  static const bool vector = [] {
-    const char *e = std::getenv("EG_SLAB");
-    return e && std::string(e) == "vector";
-  }();
+  const char *mode = std::getenv("EG_SLAB"); // read per call: a solver may be set up under another value
+  const bool vector = mode && std::string(mode) == "vector";
   if (vector) {
     EG_DISPATCH_SLAB(b, (k_ilu_slab<B><<<nrows * Slab<B>::NS, 256, Slab<B>::lds(), s>>>(bsr, rows, row_ptr, diag_ptr,
                                                                                        col, upd_ptr, upd)));

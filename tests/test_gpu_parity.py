@@ -322,3 +322,36 @@ def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
     for mode, tol in (("dataflow", 1e-13), ("dataflow-all", 1e-12)):
         for a, b in zip(out["levels"], out[mode]):
             assert rel(a, b) < tol, mode
+
+
+@pytest.mark.parametrize("args", ["--nx 16 --ny 16 --k 3", "--nx 5 --ny 3 --k 2", "--nx 4 --ny 4 --k 4",
+                                  "--nx 3 --ny 3 --k 6"])
+@pytest.mark.parametrize("env", [("EG_SCALED_UPPER", "0"), ("EG_SLAB", "vector"), ("EG_FACTOR_GRAPH", "0")])
+def test_ilu_variants_match_default(gpu, oracle_lib, args, env, monkeypatch):
+    """The ILU(0) preconditioner with the factor's upper blocks unscaled (the
+    reference's inv(U_ii) (w - sum U y) backward step), with the vector-FMA
+    slab products instead of the MFMA ones, and with the per-level factor
+    issued without a graph: each against the oracle (1e-10) and against the
+    default path (1e-12: other summation orders)."""
+    import torch
+    out = {}
+    for variant in ("default", "env"):
+        if variant == "env":
+            monkeypatch.setenv(*env)
+        else:
+            monkeypatch.delenv(env[0], raising=False)
+        o, c, u, du, *_ = hdg_case(gpu, oracle_lib, args + " --precond block-ilu0")
+        res = []
+        for form, size in ((1, o.nl), (0, o.nu)):
+            if form == 0:
+                c.call("eg_dg_refresh", gpu._ptr(du))
+                o.dg_refresh(u)
+            c.call("eg_pc_setup", form)
+            b = np.sin(0.11 * (np.arange(size) + 1))
+            y = torch.zeros(size, dtype=torch.float64, device="cuda")
+            c.call("eg_pc_apply", form, gpu._ptr(torch.tensor(b, device="cuda")), gpu._ptr(y))
+            res.append(y.cpu().numpy())
+            assert rel(res[-1], o.precondition(form, 1, b)) < 1e-10, (variant, form)
+        out[variant] = res
+    for a, b in zip(out["default"], out["env"]):
+        assert rel(a, b) < 1e-12
