@@ -1355,7 +1355,7 @@ template <class T> __device__ __forceinline__ bool armed(T v) {
 // thread (r, part) = (tid % B, tid / B) holds columns part + v P of row r of
 // every block of its row (VB values per block).
 template <int B> struct SweepShape {
-  static constexpr int NT = B > 64 ? 1024 : 256;
+  static constexpr int NT = B >= 64 ? 1024 : 256;
   static constexpr int P = NT / B;
   static constexpr int VB = (B + P - 1) / P;
 };
