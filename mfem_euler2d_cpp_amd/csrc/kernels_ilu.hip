@@ -1367,6 +1367,23 @@ template <int B> struct SweepShape {
 // M_ij = inv(U_ii) U_ij, so y_i = c_i - sum_{j>i} M_ij y_j with c_i =
 // inv(U_ii) w_i formed before the wait: one product after it, as in the
 // lower sweep.
+// Sum over each DPP row of 16 lanes (the same bits in every lane of the row).
+template <class T> __device__ __forceinline__ T row16_sum(T v);
+template <> __device__ __forceinline__ double row16_sum<double>(double v) {
+  v = dpp_add<0xB1>(v);  // quad_perm [1,0,3,2]
+  v = dpp_add<0x4E>(v);  // quad_perm [2,3,0,1]
+  v = dpp_add<0x141>(v); // row_half_mirror
+  return dpp_add<0x140>(v); // row_mirror
+}
+template <int CTRL> __device__ __forceinline__ float dpp_addf(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <> __device__ __forceinline__ float row16_sum<float>(float v) {
+  v = dpp_addf<0xB1>(v);
+  v = dpp_addf<0x4E>(v);
+  v = dpp_addf<0x141>(v);
+  return dpp_addf<0x140>(v);
+}
 template <class T, int B, int KD, bool LOWER>
 __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__restrict__ bsr, int nb,
                                                                     const int *__restrict__ row_ptr,
@@ -1379,7 +1396,12 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
   __shared__ T xs[KD * B];
   __shared__ T red[NT];
   __shared__ T ts[B];
-  const int tid = threadIdx.x, r = tid % B, part = tid / B;
+  // 16 x 16 blocks (P == 16): the parts of a row are one DPP row of lanes
+  // (part fastest) and the row's partial sums are added by DPP, with no LDS
+  // round trip or workgroup barrier; otherwise rows fastest (coalesced block
+  // loads; measured faster for 64 x 64) and a sum over LDS.
+  constexpr bool RR = P == 16 && B == 16;
+  const int tid = threadIdx.x, r = RR ? tid / P : tid % B, part = RR ? tid % P : tid / B;
   const bool worker = part < P;
   if (LOWER) {
     for (size_t t = size_t(blockIdx.x) * NT + tid; t < size_t(nb) * B; t += size_t(gridDim.x) * NT) {
@@ -1409,13 +1431,17 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
         const int c = part + v * P;
         if (worker && c < B) { cacc += a[KD][v] * src[size_t(i) * B + c]; }
       }
-      red[tid] = cacc;
-      __syncthreads(); // every read of w_i is done
-      if (part == 0) {
-        for (int q = 0; q < P; ++q) { own += red[q * B + r]; }
-        src[size_t(i) * B + r] = sentinel<T>();
+      if constexpr (RR) {
+        own = row16_sum<T>(cacc); // w_i is re-armed after the barrier below (every read of it done)
+      } else {
+        red[tid] = cacc;
+        __syncthreads(); // every read of w_i is done
+        if (part == 0) {
+          for (int q = 0; q < P; ++q) { own += red[q * B + r]; }
+          src[size_t(i) * B + r] = sentinel<T>();
+        }
+        __syncthreads(); // red is free again
       }
-      __syncthreads(); // red is free again
     } else if (part == 0) {
       own = src[size_t(i) * B + r];
       if (!LOWER) { src[size_t(i) * B + r] = sentinel<T>(); }
@@ -1435,6 +1461,7 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
       xs[t] = x;
     }
     __syncthreads();
+    if (RR && !LOWER && scaled && part == 0) { src[size_t(i) * B + r] = sentinel<T>(); }
     T acc = T(0);
 #pragma unroll
     for (int d = 0; d < KD; ++d) {
@@ -1446,11 +1473,17 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
         }
       }
     }
-    red[tid] = acc;
-    __syncthreads();
+    T sum = T(0);
+    if constexpr (RR) {
+      sum = row16_sum<T>(acc);
+    } else {
+      red[tid] = acc;
+      __syncthreads();
+      if (part == 0) {
+        for (int q = 0; q < P; ++q) { sum += red[q * B + r]; }
+      }
+    }
     if (part == 0) {
-      T sum = T(0);
-      for (int q = 0; q < P; ++q) { sum += red[q * B + r]; }
       if (LOWER || scaled) { st_coherent(dst + size_t(i) * B + r, own - sum); }
       else { ts[r] = own - sum; }
     }
@@ -1462,13 +1495,17 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
         const int c = part + v * P;
         if (c < B) { acc += a[KD][v] * ts[c]; }
       }
-      red[tid] = acc;
-      __syncthreads();
-      if (part == 0) {
-        T sum = T(0);
-        for (int q = 0; q < P; ++q) { sum += red[q * B + r]; }
-        st_coherent(dst + size_t(i) * B + r, sum);
+      T sum2 = T(0);
+      if constexpr (RR) {
+        sum2 = row16_sum<T>(acc);
+      } else {
+        red[tid] = acc;
+        __syncthreads();
+        if (part == 0) {
+          for (int q = 0; q < P; ++q) { sum2 += red[q * B + r]; }
+        }
       }
+      if (part == 0) { st_coherent(dst + size_t(i) * B + r, sum2); }
     }
     __syncthreads();
   }
