@@ -571,6 +571,9 @@ __device__ __forceinline__ bool gj_panel(double *__restrict__ A, GjPanelLds<B> &
   double scale = 1.;
   for (int q = 0; q < B / PW; ++q) {
     if (w == q) {
+      // the owner's pivot chain is the critical path: its instructions go
+      // first on the SIMD it shares with a follower
+      __builtin_amdgcn_s_setprio(3);
 #pragma unroll
       for (int kc = 0; kc < PW; ++kc) {
         const int k = PW * q + kc;
@@ -599,6 +602,7 @@ __device__ __forceinline__ bool gj_panel(double *__restrict__ A, GjPanelLds<B> &
         step = pivot ? k : step;
         scale = pivot ? inv : scale;
       }
+      __builtin_amdgcn_s_setprio(0);
     } else {
       for (int kc = 0; kc < PW; ++kc) {
         const int k = PW * q + kc;
