@@ -371,6 +371,7 @@ __device__ __forceinline__ bool gj2_inverse(double *__restrict__ A, Gj2Lds<B, NC
       const unsigned used_next = used | (lane == lp ? 1u << ip : 0u);
       const bool ahead = cg == s1 && k + 1 < B;
       if (ahead && j1 < NE) {
+        __builtin_amdgcn_s_setprio(3); // the next pivot is the critical path
         double pr = 0.;
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
@@ -383,6 +384,7 @@ __device__ __forceinline__ bool gj2_inverse(double *__restrict__ A, Gj2Lds<B, NC
           x[i] = v[i][j1];
         }
         gj2_publish<B, NCG>(x, used_next, lane, L, buf ^ 1);
+        __builtin_amdgcn_s_setprio(0);
       }
       // the pivot row's values of this wave's columns (scalar registers)
       double prow[NE];
