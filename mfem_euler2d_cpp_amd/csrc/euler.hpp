@@ -246,6 +246,7 @@ public:
   // inv(U_ii) U_ij after each factorisation (one product a row in the
   // backward sweep); the upper block slots and their rows' diagonal slots.
   bool scaled_upper = false;
+  bool star_rows = false; // every lower block updates only its row's diagonal
   DeviceArray<int> d_up_slot, d_up_diag;
   std::vector<double> h, cosine, sine, rotated, coefficients;
   // Completion of the Arnoldi step whose Hessenberg column sits in pinned

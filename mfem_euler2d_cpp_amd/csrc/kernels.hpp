@@ -89,13 +89,17 @@ void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const i
 // Row-slab block products of one level for b in {64, 100, 144, 196, 256,
 // 324} (the diagonal inverses follow with launch_block_inverse).
 bool ilu_slab_ok(int b);
+// star: every lower block of every row has exactly one update pair, the
+// row's diagonal, and no row has more than ilu_star_max_lower() lower
+// blocks (the two-stage slab kernel for 64 x 64 blocks).
+int ilu_star_max_lower();
 // In place M_ij = inv(U_ii) U_ij for the upper blocks (slot[i], diagonal
 // dslot[i]) after the factor, for block sizes where ilu_scale_upper_ok.
 bool ilu_scale_upper_ok(int b);
 void launch_ilu_scale_upper(double *bsr, int b, const int *slot, const int *dslot, int count, hipStream_t s);
 
 void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
-                     const int *col, const int *upd_ptr, const int *upd, hipStream_t s);
+                     const int *col, const int *upd_ptr, const int *upd, bool star, hipStream_t s);
 // Single-launch dataflow forms (grid within the device's occupancy, per-row
 // completion flags or sentinel-armed values);
 // return false when a co-resident launch is not possible (caller falls back

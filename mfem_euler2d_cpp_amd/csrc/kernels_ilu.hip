@@ -1201,6 +1201,124 @@ __global__ __launch_bounds__(256) void k_ilu_scale_upper(double *__restrict__ bs
   }
 }
 
+// The same products for rows whose every lower block ik has one update
+// pair, the diagonal (A_ii -= L_ik U_ki; the cell stencil), 64 x 64 blocks
+// (each wavefront one 16-column tile, one chunk): the L products of all
+// lower blocks are independent, so their operands are loaded in one batch
+// and their MFMA chains run together; then the update products, whose
+// operands are loaded while the L blocks are written. A_ii is updated in
+// ascending k as in k_ilu_slab_m (the same products, the same bits).
+constexpr int kStarLower = 4; // most lower blocks a row
+template <int B, int NL>
+__device__ __forceinline__ void slab_star_row(double *__restrict__ bsr, int i, int r0, int j0,
+                                              const int *__restrict__ diag_ptr, const int *__restrict__ col,
+                                              const int *__restrict__ upd_ptr, const int *__restrict__ upd,
+                                              double *Xs) {
+  using M = SlabM<B>;
+  constexpr int XS = 4 * M::KS * 16;
+  constexpr size_t bb = size_t(B) * B;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, g = lane >> 4, j = lane & 15;
+  const int c = 16 * w + j; // this lane's output column (w < CT = 4)
+  // stage 1 operands: the inverses of the lower blocks' rows, and the slabs
+  double y[NL > 0 ? NL : 1][M::S];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) { slab_chunk_load<B>(bsr + size_t(diag_ptr[col[j0 + l]]) * bb, w, 0, y[l]); }
+  constexpr int XP = (B * 16 + 255) / 256;
+  double xv[NL > 0 ? NL : 1][XP];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+#pragma unroll
+    for (int q = 0; q < XP; ++q) {
+      const int e = t + 256 * q, r = e & 15, k = e >> 4;
+      xv[l][q] = (e < B * 16 && r0 + r < B) ? bsr[size_t(j0 + l) * bb + r0 + r + size_t(k) * B] : 0.;
+    }
+  }
+  for (int e = B * 16 + t; e < 4 * M::KS * 16; e += 256) {
+#pragma unroll
+    for (int l = 0; l < NL; ++l) { Xs[l * XS + e] = 0.; } // k >= B
+  }
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+#pragma unroll
+    for (int q = 0; q < XP; ++q) {
+      const int e = t + 256 * q;
+      if (e < B * 16) { Xs[l * XS + e] = xv[l][q]; }
+    }
+  }
+  lds_barrier();
+  slab_d4 acc[NL > 0 ? NL : 1];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) { acc[l] = slab_d4{0., 0., 0., 0.}; }
+#pragma unroll
+  for (int s = 0; s < M::S; ++s) {
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      acc[l] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[l * XS + (g * M::KS + s) * 16 + j], y[l][s], acc[l], 0, 0, 0);
+    }
+  }
+  // stage 2 operands (U_ki of each lower block's row) while L is written
+#pragma unroll
+  for (int l = 0; l < NL; ++l) { slab_chunk_load<B>(bsr + size_t(upd[2 * upd_ptr[j0 + l] + 1]) * bb, w, 0, y[l]); }
+  double *dg = bsr + size_t(diag_ptr[i]) * bb;
+  double dv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = r0 + g + 4 * q;
+    dv[q] = (c < B && r < B) ? dg[r + size_t(c) * B] : 0.;
+  }
+  lds_barrier(); // every read of the slabs is done
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    if (c < B) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = g + 4 * q;
+        Xs[l * XS + c * 16 + r] = acc[l][q];
+        if (r0 + r < B) { bsr[size_t(j0 + l) * bb + r0 + r + size_t(c) * B] = acc[l][q]; }
+      }
+    }
+  }
+  lds_barrier();
+#pragma unroll
+  for (int l = 0; l < NL; ++l) { acc[l] = slab_d4{0., 0., 0., 0.}; }
+#pragma unroll
+  for (int s = 0; s < M::S; ++s) {
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      acc[l] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[l * XS + (g * M::KS + s) * 16 + j], y[l][s], acc[l], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { dv[q] -= acc[l][q]; }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int r = r0 + g + 4 * q;
+    if (c < B && r < B) { dg[r + size_t(c) * B] = dv[q]; }
+  }
+}
+template <int B>
+__global__ __launch_bounds__(256) void k_ilu_slab_star(double *__restrict__ bsr, const int *__restrict__ rows,
+                                                       const int *__restrict__ row_ptr,
+                                                       const int *__restrict__ diag_ptr,
+                                                       const int *__restrict__ col, const int *__restrict__ upd_ptr,
+                                                       const int *__restrict__ upd) {
+  using M = SlabM<B>;
+  static_assert(M::TW == 1 && M::NH == 1, "one tile a wavefront, one chunk a tile");
+  extern __shared__ double Xs[]; // [l][k][16]
+  const int i = rows[blockIdx.x / M::NS], r0 = (blockIdx.x % M::NS) * 16;
+  const int j0 = row_ptr[i], nl = diag_ptr[i] - j0;
+  switch (nl) { // the number of lower blocks as a constant: no guards in the MFMA chains
+  case 1: slab_star_row<B, 1>(bsr, i, r0, j0, diag_ptr, col, upd_ptr, upd, Xs); break;
+  case 2: slab_star_row<B, 2>(bsr, i, r0, j0, diag_ptr, col, upd_ptr, upd, Xs); break;
+  case 3: slab_star_row<B, 3>(bsr, i, r0, j0, diag_ptr, col, upd_ptr, upd, Xs); break;
+  case 4: slab_star_row<B, 4>(bsr, i, r0, j0, diag_ptr, col, upd_ptr, upd, Xs); break;
+  default: break; // no lower block: nothing to update
+  }
+}
+
 // ------------------------------------------------------------ sweeps
 // Dot products of block row r with x, split over P = blockDim/b threads
 // (columns c = part, part + P, ...), four independent loads in flight.
@@ -1741,6 +1859,7 @@ int ilu_factor_lds_init() {
   EG_ALLOW_S(64) EG_ALLOW_S(100) EG_ALLOW_S(144) EG_ALLOW_S(196) EG_ALLOW_S(256) EG_ALLOW_S(324)
 #undef EG_ALLOW_S
   allow_lds(reinterpret_cast<const void *>(k_ilu_scale_upper<64>));
+  allow_lds(reinterpret_cast<const void *>(k_ilu_slab_star<64>));
   return 0;
 }
 bool ilu_resident_ok(int b) { return b % 4 == 0 && b >= 4 && b <= 36; }
@@ -1777,14 +1896,20 @@ void launch_ilu_scale_upper(double *bsr, int b, const int *slot, const int *dslo
   throw std::runtime_error("scaled upper blocks: unsupported block size");
 }
 bool ilu_slab_ok(int b) { return b == 64 || b == 100 || b == 144 || b == 196 || b == 256 || b == 324; }
+int ilu_star_max_lower() { return kStarLower; }
 void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
-                     const int *col, const int *upd_ptr, const int *upd, hipStream_t s) {
+                     const int *col, const int *upd_ptr, const int *upd, bool star, hipStream_t s) {
   const char *mode = std::getenv("EG_SLAB"); // read per call: a solver may be set up under another value
   const bool vector = mode && std::string(mode) == "vector";
   if (vector) {
     EG_DISPATCH_SLAB(b, (k_ilu_slab<B><<<nrows * Slab<B>::NS, 256, Slab<B>::lds(), s>>>(bsr, rows, row_ptr, diag_ptr,
                                                                                        col, upd_ptr, upd)));
   } else {
+    if (star && b == 64) {
+      k_ilu_slab_star<64><<<nrows * SlabM<64>::NS, 256, kStarLower * SlabM<64>::lds(), s>>>(
+          bsr, rows, row_ptr, diag_ptr, col, upd_ptr, upd);
+      return;
+    }
     EG_DISPATCH_SLAB(b, (k_ilu_slab_m<B><<<nrows * SlabM<B>::NS, 256, SlabM<B>::lds(), s>>>(
                             bsr, rows, row_ptr, diag_ptr, col, upd_ptr, upd)));
   }

@@ -56,6 +56,19 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     if (mixed) { sweep32.resize(N); }
     if (eg::ilu_resident_ok(block)) { PlanResident(); }
     if (eg::big_block(block) && !eg::ilu_slab_ok(block)) { PlanBigFactor(); }
+    {
+      // the cell stencil: each lower block updates only the diagonal
+      const auto &pt = *pattern;
+      star_rows = true;
+      for (int i = 0; i < pt.nb && star_rows; ++i) {
+        star_rows = pt.diag[i] - pt.row[i] <= eg::ilu_star_max_lower();
+        for (int ik = pt.row[i]; ik < pt.diag[i] && star_rows; ++ik) {
+          star_rows = pt.upd_ptr[ik + 1] - pt.upd_ptr[ik] == 1 && pt.upd[2 * pt.upd_ptr[ik]] == pt.diag[i];
+        }
+      }
+      const char *st = std::getenv("EG_SLAB_STAR");
+      if (st && std::string(st) == "0") { star_rows = false; }
+    }
     // EG_SCALED_UPPER=0 keeps U_ij unscaled (two products a row in the
     // backward sweep, as the reference writes it).
     const char *su = std::getenv("EG_SCALED_UPPER");
@@ -311,7 +324,7 @@ void LinearSolver::FactorLevels() {
         }
         if (eg::ilu_slab_ok(b)) {
           eg::launch_ilu_slab(blocks, b, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col,
-                              pt.d_upd_ptr, pt.d_upd, p.stream);
+                              pt.d_upd_ptr, pt.d_upd, star_rows, p.stream);
           eg::launch_block_inverse(blocks, b, count, pt.d_lower_diag.p + first, p.flags, p.stream);
           continue;
         }
