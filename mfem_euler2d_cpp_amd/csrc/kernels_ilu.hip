@@ -303,6 +303,28 @@ __device__ __forceinline__ float wave_fmax(float x) {
   const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 48));
   return fmaxf(fmaxf(a, b), fmaxf(c, d));
 }
+template <int CTRL, int ROWS> __device__ __forceinline__ int dpp_imax(int x) {
+  // rows outside ROWS read INT_MIN, the identity of max
+  return max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, CTRL, ROWS, 0xF, false));
+}
+// Pivot lane of a wavefront: the largest float magnitude mag (sign clear; NaN
+// counts as infinity) over the lanes with used false, the lowest lane among
+// magnitudes that differ only in the six low bits of the float significand
+// (a relative difference below 2^-17). The key carries 63 - lane in the
+// magnitude's six low bits, so one integer max names the lane: DPP within
+// rows of 16 lanes, then row_bcast:15 and row_bcast:31 carry the row maxima
+// up to lane 63. At least one lane must have used false.
+__device__ __forceinline__ int wave_argmax(float mag, bool used, int lane) {
+  const int bits = min(__float_as_int(mag), 0x7F800000);
+  int x = used ? -1 : ((bits & ~63) | (63 - lane));
+  x = dpp_imax<0xB1, 0xF>(x);  // quad_perm [1,0,3,2]
+  x = dpp_imax<0x4E, 0xF>(x);  // quad_perm [2,3,0,1]
+  x = dpp_imax<0x141, 0xF>(x); // row_half_mirror
+  x = dpp_imax<0x140, 0xF>(x); // row_mirror
+  x = dpp_imax<0x142, 0xA>(x); // row_bcast:15 into rows 1 and 3
+  x = dpp_imax<0x143, 0xC>(x); // row_bcast:31 into rows 2 and 3
+  return 63 - (__builtin_amdgcn_readlane(x, 63) & 63);
+}
 // Pivot search of the owning wave over the register column x (rows lane +
 // 64 i not yet used); publishes the pivot row, its reciprocal and the column.
 template <int B, int NCG>
@@ -434,7 +456,7 @@ __device__ __forceinline__ bool gj2_inverse(double *__restrict__ A, Gj2Lds<B, NC
 }
 // Gauss-Jordan inverse of a B x B block (B <= 64) by one wavefront, lane i
 // holding row i in registers: no workgroup barrier and no LDS traffic, each
-// step a wavefront max of float pivot magnitudes (lowest lane on ties), the
+// step a wavefront arg-max of float pivot magnitudes (wave_argmax), the
 // pivot row's values by readlane and a rank-1 update. Rows stay unscaled and
 // pivoting is implicit, as in gj2_inverse: A^{-1}(step of row i, piv[j]) =
 // scale_i S(i, j). A may point to LDS or global memory.
@@ -451,10 +473,7 @@ __device__ __forceinline__ bool gj_wave(double *A) {
   bool singular = false;
 #pragma unroll
   for (int k = 0; k < B; ++k) {
-    const float a = float(v[k]);
-    const float key = used ? -1.f : (a != a ? __int_as_float(0x7F800000) : fabsf(a));
-    const float best = wave_fmax(key);
-    const int p = __ffsll(static_cast<long long>(__ballot(key == best))) - 1;
+    const int p = wave_argmax(fabsf(float(v[k])), used, lane);
     const double inv = recip(read_lane(v[k], p));
     if (!(isfinite(inv) && inv != 0.)) { singular = true; }
     const bool pivot = lane == p;
@@ -480,7 +499,7 @@ __device__ __forceinline__ bool gj_wave(double *A) {
 // The same Gauss-Jordan inverse for B <= 16 with a quad of lanes per row:
 // lane 4 r + q holds columns 4 q .. 4 q + 3 of row r, so a step updates four
 // values a lane. The pivot search is gj_wave's (float magnitudes of column k
-// over the unused rows, lowest row on ties), S(r, k) reaches the lanes of row
+// over the unused rows by wave_argmax), S(r, k) reaches the lanes of row
 // r by a quad broadcast and the pivot row by a lane permute; every value is
 // computed by the same operations as in gj_wave, so the results are the same.
 template <int KQ> __device__ __forceinline__ double quad_bcast(double v) {
@@ -504,10 +523,7 @@ __device__ __forceinline__ bool gj_quad(double *A) {
 #pragma unroll
   for (int k = 0; k < B; ++k) {
     const int kq = k >> 2, kc = k & 3;
-    const float a = float(v[kc]);
-    const float key = (used || q != kq) ? -1.f : (a != a ? __int_as_float(0x7F800000) : fabsf(a));
-    const float best = wave_fmax(key);
-    const int pl = __ffsll(static_cast<long long>(__ballot(key == best))) - 1;
+    const int pl = wave_argmax(fabsf(float(v[kc])), used || q != kq, lane);
     const int p = pl >> 2;
     const double inv = recip(read_lane(v[kc], pl));
     if (!(isfinite(inv) && inv != 0.)) { singular = true; }
@@ -547,13 +563,13 @@ __device__ __forceinline__ bool gj_quad(double *A) {
 // Gauss-Jordan inverse of a 64 x 64 block by B / PW wavefronts with panel
 // ownership: wave w keeps columns PW w .. PW w + PW - 1 of every row (lane =
 // row) and owns steps k of its panel. The owner searches the pivot of column
-// k (float magnitudes, lowest row on ties, as in gj2_inverse), publishes
+// k (wave_argmax over float magnitudes, as in gj_wave), publishes
 // the multipliers of step k (with 1 / S(p,k) in the pivot row's slot) and p
 // in LDS and raises the step counter; every other wave applies step k to its
 // columns as soon as the counter passes k. No workgroup barrier in the step
 // loop: the owner's pivot recurrence never waits for the other waves, and a
 // wave that becomes owner has applied every earlier step to its panel. Each
-// value is computed by the same operations as in gj2_inverse.
+// value is computed by the same operations as in gj_wave.
 template <int B> struct GjPanelLds {
   double m[B][64];
   int p[B];
@@ -579,10 +595,7 @@ __device__ __forceinline__ bool gj_panel(double *__restrict__ A, GjPanelLds<B> &
 #pragma unroll
       for (int kc = 0; kc < PW; ++kc) {
         const int k = PW * q + kc;
-        const float a = float(v[kc]);
-        const float key = used ? -1.f : (a != a ? __int_as_float(0x7F800000) : fabsf(a));
-        const float best = wave_fmax(key);
-        const int p = __ffsll(static_cast<long long>(__ballot(key == best))) - 1;
+        const int p = wave_argmax(fabsf(float(v[kc])), used, lane);
         const double inv = recip(read_lane(v[kc], p));
         if (!(isfinite(inv) && inv != 0.)) { singular = true; }
         const bool pivot = lane == p;
