@@ -244,8 +244,8 @@ __device__ __forceinline__ bool gj_inverse(double *__restrict__ A, double *lds) 
 // wave s. Each step is one workgroup barrier: after it every wave reads the
 // pivot row p of step k, 1 / S(p,k) and column k from LDS. The wave that
 // owns column k + 1 first updates that column alone and searches its pivot
-// (a register max over its rows, then a wavefront max of float magnitudes,
-// lowest lane on ties), and publishes p, 1 / S(p,k+1) and the column for
+// (a register max over its rows, then wave_argmax over the lanes' float
+// magnitudes), and publishes p, 1 / S(p,k+1) and the column for
 // the next step while the other waves are still updating; then every wave
 // finishes the rank-1 update S(i,j) -= S(i,k) S(p,j) / S(p,k) of its columns
 // with the pivot row's values taken from lane p % 64 (readlane). Rows are
@@ -286,23 +286,6 @@ __device__ __forceinline__ double read_lane(double v, int lane) {
   const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(u >> 32)), lane));
   return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
 }
-template <int CTRL> __device__ __forceinline__ float dpp_fmax(float x) {
-  const float o = __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(x), __float_as_int(x), CTRL, 0xF, 0xF, false));
-  return fmaxf(x, o);
-}
-// Max over the wavefront (the same value in every lane): DPP within rows of
-// 16 lanes, then the four row maxima.
-__device__ __forceinline__ float wave_fmax(float x) {
-  x = dpp_fmax<0xB1>(x);  // quad_perm [1,0,3,2]
-  x = dpp_fmax<0x4E>(x);  // quad_perm [2,3,0,1]
-  x = dpp_fmax<0x141>(x); // row_half_mirror
-  x = dpp_fmax<0x140>(x); // row_mirror
-  const float a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 0));
-  const float b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 16));
-  const float c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 32));
-  const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 48));
-  return fmaxf(fmaxf(a, b), fmaxf(c, d));
-}
 template <int CTRL, int ROWS> __device__ __forceinline__ int dpp_imax(int x) {
   // rows outside ROWS read INT_MIN, the identity of max
   return max(x, __builtin_amdgcn_update_dpp(INT_MIN, x, CTRL, ROWS, 0xF, false));
@@ -342,9 +325,7 @@ __device__ __forceinline__ void gj2_publish(const double (&x)[Gj2<B, NCG>::RW], 
     if (kk > key) { key = kk; bi = i; bv = x[i]; }
     L.col[buf][r] = x[i];
   }
-  const float best = wave_fmax(key);
-  const unsigned long long mask = __ballot(key == best);
-  const int pl = __ffsll(static_cast<long long>(mask)) - 1;
+  const int pl = wave_argmax(key, key < 0.f, lane);
   const int pi = __builtin_amdgcn_readlane(bi, pl);
   const double ap = read_lane(bv, pl);
   if (lane == 0) {
@@ -657,7 +638,6 @@ __global__ __launch_bounds__(64 * B / PW) void k_block_inverse_p(double *__restr
   if (gj_panel<B, PW>(blocks + size_t(blk) * B * B, L, flags) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
 }
 template <int B> struct Gj2Pick;
-template <> struct Gj2Pick<64> { static constexpr int NCG = 4; };
 template <> struct Gj2Pick<100> { static constexpr int NCG = 4; };
 template <> struct Gj2Pick<144> { static constexpr int NCG = 8; };
 template <> struct Gj2Pick<196> { static constexpr int NCG = 8; };
