@@ -775,44 +775,41 @@ __device__ __forceinline__ void block_dot2(const double *X, const double *Y0, co
     a1 += x * Y1[kk + c * B];
   }
 }
-template <int B>
-__global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident(double *__restrict__ bsr,
-                                                                      const int *__restrict__ prog_off,
-                                                                      const int *__restrict__ prog,
-                                                                      int *__restrict__ flags) {
-  extern __shared__ double sm[];
-  __shared__ int P[kIluProgMax];
-  constexpr int bb = B * B, NT = Shape<B>::NT, NQ = (bb + NT - 1) / NT;
-  const int t = threadIdx.x;
-  const int *pg = prog + prog_off[blockIdx.x];
-  for (int e = t; e < kIluProgMax; e += NT) { P[e] = pg[e]; } // the program array is padded by kIluProgMax
-  __syncthreads();
-  const int j0 = P[1], nrow = P[2], dpos = P[3], ndep = P[4];
-  double *Arow = sm, *Dep = sm + size_t(nrow) * bb;
-  {
-    // all loads of a batch in flight before the first LDS store
-    constexpr int U = 16;
-    const int total = (nrow + ndep) * bb;
-    for (int e0 = 0; e0 < total; e0 += U * NT) {
-      double tmp[U];
+// Blocks b0 .. b1 - 1 of a row program's LDS image (the row's own blocks,
+// then its dependencies) loaded with all loads of a batch in flight before
+// the first LDS store.
+template <int B, bool COH>
+__device__ __forceinline__ void resident_load(const double *bsr, const int *P, double *sm, int b0, int b1) {
+  constexpr int bb = B * B, NT = Shape<B>::NT, U = 16;
+  const int t = threadIdx.x, j0 = P[1], nrow = P[2];
+  const int base = b0 * bb, total = (b1 - b0) * bb;
+  for (int e0 = 0; e0 < total; e0 += U * NT) {
+    double tmp[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int e = e0 + u * NT + t;
-        tmp[u] = 0.;
-        if (e < total) {
-          const int blk = e / bb, o = e - blk * bb;
-          const int slot = blk < nrow ? j0 + blk : P[5 + blk - nrow];
-          tmp[u] = bsr[size_t(slot) * bb + o];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int e = e0 + u * NT + t;
-        if (e < total) { sm[e] = tmp[u]; }
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * NT + t;
+      tmp[u] = 0.;
+      if (e < total) {
+        const int blk = (base + e) / bb, o = base + e - blk * bb;
+        const int slot = blk < nrow ? j0 + blk : P[5 + blk - nrow];
+        tmp[u] = ld<COH>(bsr + size_t(slot) * bb + o);
       }
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * NT + t;
+      if (e < total) { sm[base + e] = tmp[u]; }
+    }
   }
-  __syncthreads();
+}
+// The row program applied in LDS (ascending k: L_ik, then its update pairs)
+// and the diagonal block inverted; returns true for a singular pivot.
+template <int B>
+__device__ __forceinline__ bool resident_factor(const int *P, double *sm) {
+  constexpr int bb = B * B, NT = Shape<B>::NT, NQ = (bb + NT - 1) / NT;
+  const int t = threadIdx.x;
+  const int nrow = P[2], dpos = P[3], ndep = P[4];
+  double *Arow = sm, *Dep = sm + size_t(nrow) * bb;
   int pc = 5 + ndep;
   for (int l = 0; l < dpos; ++l) {
     const int inv = P[pc], nupd = P[pc + 1];
@@ -857,7 +854,25 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident(double *__
     else { singular = gj_wave<B>(Arow + size_t(dpos) * bb); }
   }
   __syncthreads();
-  for (int e = t; e < nrow * bb; e += NT) { bsr[size_t(j0) * bb + e] = Arow[e]; }
+  return singular;
+}
+template <int B>
+__global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident(double *__restrict__ bsr,
+                                                                      const int *__restrict__ prog_off,
+                                                                      const int *__restrict__ prog,
+                                                                      int *__restrict__ flags) {
+  extern __shared__ double sm[];
+  __shared__ int P[kIluProgMax];
+  constexpr int bb = B * B, NT = Shape<B>::NT;
+  const int t = threadIdx.x;
+  const int *pg = prog + prog_off[blockIdx.x];
+  for (int e = t; e < kIluProgMax; e += NT) { P[e] = pg[e]; } // the program array is padded by kIluProgMax
+  __syncthreads();
+  const int j0 = P[1], nrow = P[2];
+  resident_load<B, false>(bsr, P, sm, 0, nrow + P[4]);
+  __syncthreads();
+  const bool singular = resident_factor<B>(P, sm);
+  for (int e = t; e < nrow * bb; e += NT) { bsr[size_t(j0) * bb + e] = sm[e]; }
   if (singular && t == 0) { atomicOr(flags, kErrSingular); }
 }
 template <int B> constexpr size_t resident_gj_lds() { return 0; }
@@ -1774,6 +1789,65 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_df(double *bsr, int
   }
 }
 
+// Dataflow form of the row-resident factor: one launch for all rows. A
+// workgroup takes the next row program in level order from a ticket counter,
+// loads the row's own blocks (written before the launch), waits until each
+// row named by its lower blocks has published done >= epoch, loads their
+// blocks (coherent loads), factors the row exactly as k_ilu_factor_resident
+// does, writes it back with coherent stores and publishes done[i] = epoch.
+// Rows are taken in dependency order by running workgroups, so every wait
+// ends whether or not the whole grid is resident at once. ticket[0] is the
+// counter and ticket[1] counts the workgroups that are done; the last one
+// resets both to zero for the next launch.
+template <int B>
+__global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident_df(double *__restrict__ bsr,
+                                                                         const int *__restrict__ prog_off,
+                                                                         const int *__restrict__ prog, int nrows,
+                                                                         const int *__restrict__ col, int *done,
+                                                                         int *ticket, int epoch, int *flags,
+                                                                         int spin_limit) {
+  extern __shared__ double sm[];
+  __shared__ int P[kIluProgMax];
+  __shared__ int row;
+  constexpr int bb = B * B, NT = Shape<B>::NT;
+  const int t = threadIdx.x;
+  for (;;) {
+    __syncthreads(); // the previous row's LDS image and index are no longer read
+    if (t == 0) { row = atomicAdd(ticket, 1); }
+    __syncthreads();
+    const int r = row;
+    if (r >= nrows) { break; }
+    const int *pg = prog + prog_off[r];
+    for (int e = t; e < kIluProgMax; e += NT) { P[e] = pg[e]; }
+    __syncthreads();
+    const int j0 = P[1], nrow = P[2], dpos = P[3];
+    resident_load<B, false>(bsr, P, sm, 0, nrow);
+    if (t < dpos) {
+      const int k = col[j0 + t];
+      int spins = 0;
+      while (ld_coherent(done + k) < epoch) {
+        if (++spins > spin_limit) {
+          atomicOr(flags, kErrSpin);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    __syncthreads();
+    resident_load<B, true>(bsr, P, sm, nrow, nrow + P[4]);
+    __syncthreads();
+    const bool singular = resident_factor<B>(P, sm);
+    for (int e = t; e < nrow * bb; e += NT) { st_coherent(bsr + size_t(j0) * bb + e, sm[e]); }
+    if (singular && t == 0) { atomicOr(flags, kErrSingular); }
+    publish_row(done, col[j0 + dpos], epoch);
+  }
+  if (t == 0 && atomicAdd(ticket + 1, 1) == int(gridDim.x) - 1) {
+    // every workgroup has taken its last ticket
+    atomicExch(ticket, 0);
+    atomicExch(ticket + 1, 0);
+  }
+}
+
 // Largest co-resident grid of a kernel (all CUs x blocks per CU).
 int coresident(const void *fn, int threads, size_t lds) {
   int dev = 0, cus = 0, per = 0;
@@ -1842,7 +1916,9 @@ int ilu_factor_lds_init() {
   EG_ALLOW(4) EG_ALLOW(8) EG_ALLOW(12) EG_ALLOW(16) EG_ALLOW(20) EG_ALLOW(24) EG_ALLOW(28) EG_ALLOW(32)
   EG_ALLOW(36) EG_ALLOW(64) EG_ALLOW(100)
 #undef EG_ALLOW
-#define EG_ALLOW_R(B) allow_lds(reinterpret_cast<const void *>(k_ilu_factor_resident<B>));
+#define EG_ALLOW_R(B)                                                                              \
+  allow_lds(reinterpret_cast<const void *>(k_ilu_factor_resident<B>));                             \
+  allow_lds(reinterpret_cast<const void *>(k_ilu_factor_resident_df<B>));
   EG_ALLOW_R(4) EG_ALLOW_R(8) EG_ALLOW_R(12) EG_ALLOW_R(16) EG_ALLOW_R(20) EG_ALLOW_R(24) EG_ALLOW_R(28)
   EG_ALLOW_R(32) EG_ALLOW_R(36)
 #undef EG_ALLOW_R
@@ -1998,6 +2074,25 @@ bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const 
       const int grid = std::min(nb, cap);
       int spin = kSpinLimit;
       void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &upd_ptr, &upd, &done, &epoch, &flags, &spin};
+      ok = hipLaunchKernel(fn, dim3(grid), dim3(Shape<B>::NT), args, lds, s) == hipSuccess;
+      if (!ok) { (void)hipGetLastError(); }
+    }
+  }));
+  return ok;
+}
+
+bool launch_ilu_factor_resident_df(double *bsr, int b, const int *prog_off, const int *prog, int nrows,
+                                   const int *col, int *done, int epoch, size_t lds, int *flags, hipStream_t s) {
+  if (!ilu_resident_ok(b)) { return false; }
+  bool ok = false;
+  EG_DISPATCH_B_SMALL(b, ({
+    const void *fn = reinterpret_cast<const void *>(k_ilu_factor_resident_df<B>);
+    const int cap = coresident(fn, Shape<B>::NT, lds);
+    if (cap > 0) {
+      const int grid = std::min(nrows, cap);
+      int spin = kSpinLimit;
+      int *ticket = done + nrows;
+      void *args[] = {&bsr, &prog_off, &prog, &nrows, &col, &done, &ticket, &epoch, &flags, &spin};
       ok = hipLaunchKernel(fn, dim3(grid), dim3(Shape<B>::NT), args, lds, s) == hipSuccess;
       if (!ok) { (void)hipGetLastError(); }
     }

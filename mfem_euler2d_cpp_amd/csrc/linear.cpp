@@ -50,8 +50,10 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     dataflow_factor = mode == "dataflow-all";
     const char *graph = std::getenv("EG_FACTOR_GRAPH");
     use_graph = !(graph && std::string(graph) == "0");
-    done_factor.resize(pattern->nb);
-    hip_check(hipMemsetAsync(done_factor, 0, sizeof(int) * pattern->nb, p.stream), "memset");
+    const char *rdf = std::getenv("EG_RESIDENT_DF");
+    resident_df = !(rdf && std::string(rdf) == "0");
+    done_factor.resize(pattern->nb + 2); // row flags, then the resident factor's ticket counters
+    hip_check(hipMemsetAsync(done_factor, 0, sizeof(int) * done_factor.n, p.stream), "memset");
     sweep.resize(N);
     if (mixed) { sweep32.resize(N); }
     if (eg::ilu_resident_ok(block)) { PlanResident(); }
@@ -269,8 +271,14 @@ void LinearSolver::Setup(bool check_flags) {
   }
   if (ilu) {
     const auto &pt = *pattern;
-    const bool df = dataflow_factor && eg::launch_ilu_factor_df(blocks, b, pt.nb, pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr,
-                                                         pt.d_upd, done_factor, ++epoch_factor, p.flags, p.stream);
+    bool df = dataflow_factor && eg::launch_ilu_factor_df(blocks, b, pt.nb, pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr,
+                                                   pt.d_upd, done_factor, ++epoch_factor, p.flags, p.stream);
+    // row-resident programs: one launch for all levels (the rows wait on
+    // each other's completion flags) when the sweeps are single-launch too
+    if (!df && resident && dataflow && resident_df) {
+      df = eg::launch_ilu_factor_resident_df(blocks, b, d_prog_off, d_prog, pt.nb, pt.d_col, done_factor, ++epoch_factor,
+                                             resident_lds, p.flags, p.stream);
+    }
     if (!df) {
       // The per-level launches are recorded once into a graph and replayed
       // (their arguments are fixed for the solver's lifetime), which removes
