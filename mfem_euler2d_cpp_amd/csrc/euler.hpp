@@ -234,8 +234,10 @@ public:
   std::vector<int> a_ptr, t_ptr;
   DeviceArray<int> d_a_ik, d_a_kk, d_t_slot, d_t_pair_ptr, d_pairs;
   bool dataflow = true, dataflow_factor = false;
-  // Row-resident factor as one dataflow launch (EG_RESIDENT_DF=0: per level).
+  // Row-resident factor as one dataflow launch (EG_RESIDENT_DF=0: per level);
+  // factor_pub: the published diagonal inverses and upper blocks it reads.
   bool resident_df = true;
+  DeviceArray<double> factor_pub;
   // Per-level factor launches recorded once into a graph (EG_FACTOR_GRAPH=0:
   // launched directly).
   bool use_graph = true;

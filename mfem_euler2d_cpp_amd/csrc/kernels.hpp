@@ -87,12 +87,13 @@ size_t ilu_resident_gj_lds(int b);
 void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const int *prog, int nrows, size_t lds,
                                 int *flags, hipStream_t s);
 // The same programs as one launch (programs in level order, taken from a
-// ticket counter; a row waits until done[k] >= epoch for the rows k of its
-// lower blocks and then publishes done[i] = epoch). done holds nrows flags
-// and two ticket counters, zero before the first launch. False when the
-// kernel cannot be launched on this device.
-bool launch_ilu_factor_resident_df(double *bsr, int b, const int *prog_off, const int *prog, int nrows,
-                                   const int *col, int *done, int epoch, size_t lds, int *flags, hipStream_t s);
+// ticket counter): a row reads the blocks of earlier rows from pub, which
+// must hold the sweep sentinel (launch_arm) at launch, as they are published
+// there, and publishes its diagonal inverse and upper blocks to pub as well
+// as writing bsr. ticket: two counters, zero before the first launch and
+// zero again after each. False when the kernel cannot be launched.
+bool launch_ilu_factor_resident_df(double *bsr, double *pub, int b, const int *prog_off, const int *prog, int nrows,
+                                   int *ticket, size_t lds, int *flags, hipStream_t s);
 // Row-slab block products of one level for b in {64, 100, 144, 196, 256,
 // 324} (the diagonal inverses follow with launch_block_inverse).
 bool ilu_slab_ok(int b);

@@ -1789,23 +1789,63 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_df(double *bsr, int
   }
 }
 
+// Dependencies of a row program read from the published copy of the factor
+// (pub, armed with the sentinel before the launch): every value is polled
+// with coherent loads until it is no longer the sentinel, so the data is its
+// own completion flag. Blocks b0 .. b1 - 1 of the program's LDS image.
+template <int B>
+__device__ __forceinline__ void resident_poll(const double *pub, const int *P, double *sm, int b0, int b1, int *flags,
+                                              int spin_limit) {
+  constexpr int bb = B * B, NT = Shape<B>::NT, U = 16;
+  const int t = threadIdx.x, nrow = P[2];
+  const int base = b0 * bb, total = (b1 - b0) * bb;
+  bool waiting = true; // false once a wait has exceeded its bound
+  for (int e0 = 0; e0 < total; e0 += U * NT) {
+    double tmp[U];
+    const double *src[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * NT + t;
+      src[u] = nullptr;
+      tmp[u] = 0.;
+      if (e < total) {
+        const int blk = (base + e) / bb, o = base + e - blk * bb;
+        src[u] = pub + size_t(P[5 + blk - nrow]) * bb + o;
+        tmp[u] = ld_coherent(src[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int spins = 0;
+      while (waiting && src[u] && armed(tmp[u])) {
+        if (++spins > spin_limit) {
+          atomicOr(flags, kErrSpin);
+          waiting = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        tmp[u] = ld_coherent(src[u]);
+      }
+      const int e = e0 + u * NT + t;
+      if (e < total) { sm[base + e] = tmp[u]; }
+    }
+  }
+}
 // Dataflow form of the row-resident factor: one launch for all rows. A
 // workgroup takes the next row program in level order from a ticket counter,
-// loads the row's own blocks (written before the launch), waits until each
-// row named by its lower blocks has published done >= epoch, loads their
-// blocks (coherent loads), factors the row exactly as k_ilu_factor_resident
-// does, writes it back with coherent stores and publishes done[i] = epoch.
-// Rows are taken in dependency order by running workgroups, so every wait
-// ends whether or not the whole grid is resident at once. ticket[0] is the
-// counter and ticket[1] counts the workgroups that are done; the last one
-// resets both to zero for the next launch.
+// loads the row's own blocks (written before the launch), reads the blocks of
+// the earlier rows it depends on from pub as they are published
+// (resident_poll), factors the row exactly as k_ilu_factor_resident does,
+// writes it to bsr and publishes its diagonal inverse and upper blocks to pub
+// with coherent stores. Rows are taken in dependency order by running
+// workgroups, so every wait ends whether or not the whole grid is resident
+// at once. ticket[0] is the counter and ticket[1] counts the workgroups that
+// are done; the last one resets both to zero for the next launch.
 template <int B>
-__global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident_df(double *__restrict__ bsr,
+__global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident_df(double *__restrict__ bsr, double *pub,
                                                                          const int *__restrict__ prog_off,
                                                                          const int *__restrict__ prog, int nrows,
-                                                                         const int *__restrict__ col, int *done,
-                                                                         int *ticket, int epoch, int *flags,
-                                                                         int spin_limit) {
+                                                                         int *ticket, int *flags, int spin_limit) {
   extern __shared__ double sm[];
   __shared__ int P[kIluProgMax];
   __shared__ int row;
@@ -1822,24 +1862,14 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident_df(double 
     __syncthreads();
     const int j0 = P[1], nrow = P[2], dpos = P[3];
     resident_load<B, false>(bsr, P, sm, 0, nrow);
-    if (t < dpos) {
-      const int k = col[j0 + t];
-      int spins = 0;
-      while (ld_coherent(done + k) < epoch) {
-        if (++spins > spin_limit) {
-          atomicOr(flags, kErrSpin);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    __syncthreads();
-    resident_load<B, true>(bsr, P, sm, nrow, nrow + P[4]);
+    resident_poll<B>(pub, P, sm, nrow, nrow + P[4], flags, spin_limit);
     __syncthreads();
     const bool singular = resident_factor<B>(P, sm);
-    for (int e = t; e < nrow * bb; e += NT) { st_coherent(bsr + size_t(j0) * bb + e, sm[e]); }
+    for (int e = t; e < nrow * bb; e += NT) {
+      bsr[size_t(j0) * bb + e] = sm[e];
+      if (e >= dpos * bb) { st_coherent(pub + size_t(j0) * bb + e, sm[e]); }
+    }
     if (singular && t == 0) { atomicOr(flags, kErrSingular); }
-    publish_row(done, col[j0 + dpos], epoch);
   }
   if (t == 0 && atomicAdd(ticket + 1, 1) == int(gridDim.x) - 1) {
     // every workgroup has taken its last ticket
@@ -2081,8 +2111,8 @@ bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const 
   return ok;
 }
 
-bool launch_ilu_factor_resident_df(double *bsr, int b, const int *prog_off, const int *prog, int nrows,
-                                   const int *col, int *done, int epoch, size_t lds, int *flags, hipStream_t s) {
+bool launch_ilu_factor_resident_df(double *bsr, double *pub, int b, const int *prog_off, const int *prog, int nrows,
+                                   int *ticket, size_t lds, int *flags, hipStream_t s) {
   if (!ilu_resident_ok(b)) { return false; }
   bool ok = false;
   EG_DISPATCH_B_SMALL(b, ({
@@ -2091,8 +2121,7 @@ bool launch_ilu_factor_resident_df(double *bsr, int b, const int *prog_off, cons
     if (cap > 0) {
       const int grid = std::min(nrows, cap);
       int spin = kSpinLimit;
-      int *ticket = done + nrows;
-      void *args[] = {&bsr, &prog_off, &prog, &nrows, &col, &done, &ticket, &epoch, &flags, &spin};
+      void *args[] = {&bsr, &pub, &prog_off, &prog, &nrows, &ticket, &flags, &spin};
       ok = hipLaunchKernel(fn, dim3(grid), dim3(Shape<B>::NT), args, lds, s) == hipSuccess;
       if (!ok) { (void)hipGetLastError(); }
     }

@@ -57,6 +57,7 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     sweep.resize(N);
     if (mixed) { sweep32.resize(N); }
     if (eg::ilu_resident_ok(block)) { PlanResident(); }
+    if (resident && dataflow && resident_df) { factor_pub.resize(blocks.n); }
     if (eg::big_block(block) && !eg::ilu_slab_ok(block)) { PlanBigFactor(); }
     {
       // the cell stencil: each lower block updates only the diagonal
@@ -276,7 +277,8 @@ void LinearSolver::Setup(bool check_flags) {
     // row-resident programs: one launch for all levels (the rows wait on
     // each other's completion flags) when the sweeps are single-launch too
     if (!df && resident && dataflow && resident_df) {
-      df = eg::launch_ilu_factor_resident_df(blocks, b, d_prog_off, d_prog, pt.nb, pt.d_col, done_factor, ++epoch_factor,
+      eg::launch_arm<double>(factor_pub, factor_pub.n, p.stream);
+      df = eg::launch_ilu_factor_resident_df(blocks, factor_pub, b, d_prog_off, d_prog, pt.nb, done_factor.p + pt.nb,
                                              resident_lds, p.flags, p.stream);
     }
     if (!df) {
