@@ -238,6 +238,9 @@ public:
   // factor_pub: the published diagonal inverses and upper blocks it reads.
   bool resident_df = true;
   DeviceArray<double> factor_pub;
+  // Dataflow sweep rows by XCD (PlanXcdRows); empty: natural order.
+  DeviceArray<int> xcd_rows, xcd_off;
+  int xcd_max = 0;
   // Per-level factor launches recorded once into a graph (EG_FACTOR_GRAPH=0:
   // launched directly).
   bool use_graph = true;
@@ -264,6 +267,7 @@ public:
   LinearSolver &operator=(const LinearSolver &) = delete;
   void PlanBigFactor();
   void PlanResident();
+  void PlanXcdRows();
   void FactorLevels();
   void Refresh(const double *dev_u) { state = dev_u; }
   // Fill + factor the preconditioner at the current state. check_flags

@@ -1521,7 +1521,9 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
                                                                     const int *__restrict__ row_ptr,
                                                                     const int *__restrict__ diag_ptr,
                                                                     const int *__restrict__ col, T *src, T *dst,
-                                                                    T *arm, int *flags, int spin_limit, int scaled) {
+                                                                    T *arm, int *flags, int spin_limit, int scaled,
+                                                                    const int *__restrict__ xrows,
+                                                                    const int *__restrict__ xoff) {
   constexpr int NT = SweepShape<B>::NT, P = SweepShape<B>::P, VB = SweepShape<B>::VB;
   constexpr int KB = LOWER ? KD : KD + 1; // blocks held per row (upper: + inv(U_ii))
   constexpr size_t bb = size_t(B) * B;
@@ -1540,8 +1542,21 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
       arm[t] = sentinel<T>();
     }
   }
-  for (int idx = blockIdx.x; idx < nb; idx += gridDim.x) {
-    const int i = LOWER ? idx : nb - 1 - idx;
+  // Rows of this workgroup: natural order (blockIdx.x, + gridDim.x, ..), or,
+  // with xrows, list x = blockIdx.x % 8 (xrows[xoff[x] .. xoff[x + 1]),
+  // ascending) dealt to the workgroups b = x (mod 8) in turn. Workgroups are
+  // dispatched to the eight XCDs round-robin, so one list runs on one XCD and
+  // the exchanges between its rows stay within that XCD. Needs gridDim.x >= 8.
+  int first = blockIdx.x, step = gridDim.x, count = nb, base = 0;
+  if (xrows) {
+    const int x = blockIdx.x & 7;
+    first = blockIdx.x >> 3;
+    step = (int(gridDim.x) - x + 7) >> 3;
+    base = xoff[x];
+    count = xoff[x + 1] - base;
+  }
+  for (int idx = first; idx < count; idx += step) {
+    const int i = xrows ? xrows[LOWER ? base + idx : base + count - 1 - idx] : (LOWER ? idx : nb - 1 - idx);
     const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
     const int nd = j1 - j0;
     T a[KB][VB];
@@ -2050,7 +2065,7 @@ void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, con
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
                          const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, bool scaled,
-                         hipStream_t s) {
+                         const int *xrows, const int *xoff, int xmax, hipStream_t s) {
   if (max_deps > 8) { return false; }
   const int kd = max_deps <= 4 ? 4 : 8;
   bool ok = false, known = false;
@@ -2058,9 +2073,15 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
     known = true;
     const int cap = coresident(fn, threads, 0);
     if (cap <= 0) { return; }
-    const int grid = std::min(nb, cap);
+    int grid = std::min(nb, cap);
+    const int *xr = nullptr, *xo = nullptr;
+    if (xrows && std::min(cap, 8 * xmax) >= 8) {
+      grid = std::min(cap, 8 * xmax);
+      xr = xrows;
+      xo = xoff;
+    }
     int spin = kSpinLimit, sc = scaled ? 1 : 0;
-    void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin, &sc};
+    void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin, &sc, &xr, &xo};
     ok = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s) == hipSuccess;
     if (!ok) { (void)hipGetLastError(); }
   };
@@ -2138,9 +2159,11 @@ template void launch_ilu_upper_level<double>(const double *, int, const int *, i
 template void launch_ilu_upper_level<float>(const float *, int, const int *, int, const int *, const int *,
                                             const int *, float *, int, bool, hipStream_t);
 template bool launch_ilu_sweep_df<double>(bool, const double *, int, int, int, const int *, const int *, const int *,
-                                          double *, double *, double *, int *, bool, hipStream_t);
+                                          double *, double *, double *, int *, bool, const int *, const int *, int,
+                                          hipStream_t);
 template bool launch_ilu_sweep_df<float>(bool, const float *, int, int, int, const int *, const int *, const int *,
-                                         float *, float *, float *, int *, bool, hipStream_t);
+                                         float *, float *, float *, int *, bool, const int *, const int *, int,
+                                         hipStream_t);
 template void launch_arm<double>(double *, size_t, hipStream_t);
 template void launch_arm<float>(float *, size_t, hipStream_t);
 } // namespace eg

@@ -327,7 +327,7 @@ def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
 @pytest.mark.parametrize("args", ["--nx 16 --ny 16 --k 3", "--nx 5 --ny 3 --k 2", "--nx 4 --ny 4 --k 4",
                                   "--nx 3 --ny 3 --k 6", "--nx 64 --ny 64 --k 1"])
 @pytest.mark.parametrize("env", [("EG_SCALED_UPPER", "0"), ("EG_SLAB", "vector"), ("EG_FACTOR_GRAPH", "0"),
-                                 ("EG_SLAB_STAR", "0"), ("EG_RESIDENT_DF", "0")])
+                                 ("EG_SLAB_STAR", "0"), ("EG_RESIDENT_DF", "0"), ("EG_SWEEP_XCD", "0")])
 def test_ilu_variants_match_default(gpu, oracle_lib, args, env, monkeypatch):
     """The ILU(0) preconditioner with the factor's upper blocks unscaled (the
     reference's inv(U_ii) (w - sum U y) backward step), with the vector-FMA
@@ -335,8 +335,9 @@ def test_ilu_variants_match_default(gpu, oracle_lib, args, env, monkeypatch):
     without a graph, with the general slab kernel instead of the two-stage
     one of the cell stencil, and with the face factor per level instead of
     one dataflow launch (64x64 k = 1: more face rows than workgroups, so a
-    workgroup factors several rows): each against the oracle (1e-10) and
-    against the default path (1e-12: other summation orders)."""
+    workgroup factors several rows), and with the dataflow sweeps' rows in
+    natural order instead of by XCD strips: each against the oracle (1e-10)
+    and against the default path (1e-12: other summation orders)."""
     import torch
     out = {}
     for variant in ("default", "env"):
