@@ -239,7 +239,7 @@ public:
   bool resident_df = true;
   DeviceArray<double> factor_pub;
   // Dataflow sweep rows by XCD (PlanXcdRows); empty: natural order.
-  DeviceArray<int> xcd_rows, xcd_off;
+  DeviceArray<int> xcd_rows, xcd_off, xcd_rows_up, xcd_off_up;
   int xcd_max = 0;
   // Per-level factor launches recorded once into a graph (EG_FACTOR_GRAPH=0:
   // launched directly).

@@ -115,8 +115,9 @@ void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *
 // Sweeps: src/dst/arm as documented at k_ilu_sweep_sn (lower: src = rhs,
 // dst = intermediate, arm = final output; upper: src = intermediate, dst =
 // final output). The intermediate must be armed before the first lower sweep.
-// xrows / xoff (optional, blocks up to 100): eight ascending row lists, list x
-// run by the workgroups b = x (mod 8), i.e. on one XCD; xmax: the longest.
+// xrows / xoff (optional, blocks up to 100): eight row lists, each in the
+// sweep's dependency-level order, list x run by the workgroups b = x (mod 8),
+// i.e. on one XCD; xmax: the longest.
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
                          const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, bool scaled,

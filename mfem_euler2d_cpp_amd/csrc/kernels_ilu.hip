@@ -1543,10 +1543,11 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
     }
   }
   // Rows of this workgroup: natural order (blockIdx.x, + gridDim.x, ..), or,
-  // with xrows, list x = blockIdx.x % 8 (xrows[xoff[x] .. xoff[x + 1]),
-  // ascending) dealt to the workgroups b = x (mod 8) in turn. Workgroups are
-  // dispatched to the eight XCDs round-robin, so one list runs on one XCD and
-  // the exchanges between its rows stay within that XCD. Needs gridDim.x >= 8.
+  // with xrows, list x = blockIdx.x % 8 (xrows[xoff[x] .. xoff[x + 1]), in
+  // the sweep's dependency-level order) dealt to the workgroups b = x (mod 8)
+  // in turn. Workgroups are dispatched to the eight XCDs round-robin, so one
+  // list runs on one XCD and the exchanges between its rows stay within that
+  // XCD. Needs gridDim.x >= 8.
   int first = blockIdx.x, step = gridDim.x, count = nb, base = 0;
   if (xrows) {
     const int x = blockIdx.x & 7;
@@ -1556,7 +1557,7 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
     count = xoff[x + 1] - base;
   }
   for (int idx = first; idx < count; idx += step) {
-    const int i = xrows ? xrows[LOWER ? base + idx : base + count - 1 - idx] : (LOWER ? idx : nb - 1 - idx);
+    const int i = xrows ? xrows[base + idx] : (LOWER ? idx : nb - 1 - idx);
     const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
     const int nd = j1 - j0;
     T a[KB][VB];
@@ -2075,8 +2076,10 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
     if (cap <= 0) { return; }
     int grid = std::min(nb, cap);
     const int *xr = nullptr, *xo = nullptr;
-    if (xrows && std::min(cap, 8 * xmax) >= 8) {
-      grid = std::min(cap, 8 * xmax);
+    // by XCD only when every row has a workgroup of its own (with several
+    // rows a workgroup, the natural order measured faster at 128x128 Q6)
+    if (xrows && 8 * xmax <= cap && 8 * xmax >= 8) {
+      grid = 8 * xmax;
       xr = xrows;
       xo = xoff;
     }

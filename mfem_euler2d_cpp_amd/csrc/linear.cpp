@@ -255,27 +255,36 @@ void LinearSolver::PlanResident() {
 // whole grid rows (a cell row of the DG system is its cell, a face row of
 // the HDG system the cell it is the right or top face of), so a row's
 // dependencies in its own grid row and, except at a strip edge, in the grid
-// row below are exchanged within one XCD. EG_SWEEP_XCD=0: natural order.
+// row below are exchanged within one XCD. Each strip's rows are kept in the
+// sweep's dependency-level order. EG_SWEEP_XCD=0: natural order.
 void LinearSolver::PlanXcdRows() {
   const char *env = std::getenv("EG_SWEEP_XCD");
   if (env && std::string(env) == "0") { return; }
   const auto &pt = *pattern;
   const int nx = p.opt.nx, ny = p.opt.ny, per_cell = form == 0 ? 1 : 2;
   if (pt.nb != nx * ny * per_cell || pt.nb < 64) { return; }
-  std::vector<std::vector<int>> lists(8);
-  for (int i = 0; i < pt.nb; ++i) { lists[size_t((i / per_cell / nx) * 8 / ny)].push_back(i); }
-  std::vector<int> rows, off{0};
+  // each strip's rows in the sweep's level order (lower levels for the
+  // lower sweep, upper levels for the upper one), so that the workgroups of
+  // a strip hold rows of consecutive levels
   int most = 0;
-  for (const auto &l : lists) {
-    rows.insert(rows.end(), l.begin(), l.end());
-    off.push_back(int(rows.size()));
-    most = std::max(most, int(l.size()));
-  }
-  xcd_rows.resize(rows.size());
-  xcd_off.resize(off.size());
-  hip_check(hipMemcpyAsync(xcd_rows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, p.stream), "upload");
-  hip_check(hipMemcpyAsync(xcd_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, p.stream), "upload");
-  hip_check(hipStreamSynchronize(p.stream), "sync");
+  auto plan = [&](const std::vector<int> &order, DeviceArray<int> &d_rows, DeviceArray<int> &d_off) {
+    std::vector<std::vector<int>> lists(8);
+    for (const int i : order) { lists[size_t((i / per_cell / nx) * 8 / ny)].push_back(i); }
+    std::vector<int> rows, off{0};
+    for (const auto &l : lists) {
+      rows.insert(rows.end(), l.begin(), l.end());
+      off.push_back(int(rows.size()));
+      most = std::max(most, int(l.size()));
+    }
+    d_rows.resize(rows.size());
+    d_off.resize(off.size());
+    hip_check(hipMemcpyAsync(d_rows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, p.stream),
+              "upload");
+    hip_check(hipMemcpyAsync(d_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, p.stream), "upload");
+    hip_check(hipStreamSynchronize(p.stream), "sync");
+  };
+  plan(pt.lower_rows, xcd_rows, xcd_off);
+  plan(pt.upper_rows, xcd_rows_up, xcd_off_up);
   xcd_max = most;
 }
 
@@ -408,11 +417,13 @@ template <class T> static void precondition(LinearSolver &s, const T *bsr, const
   if (s.dataflow) {
     require(b != y, "ILU apply needs distinct input and output vectors");
     T *mid = sizeof(T) == 8 ? reinterpret_cast<T *>(s.sweep.p) : reinterpret_cast<T *>(s.sweep32.p);
-    const int *xr = s.xcd_rows.n ? s.xcd_rows.p : nullptr, *xo = s.xcd_rows.n ? s.xcd_off.p : nullptr;
+    const bool xcd = s.xcd_rows.n > 0;
     if (eg::launch_ilu_sweep_df<T>(true, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col,
-                                   const_cast<T *>(b), mid, y, s.p.flags, false, xr, xo, s.xcd_max, s.p.stream)) {
+                                   const_cast<T *>(b), mid, y, s.p.flags, false, xcd ? s.xcd_rows.p : nullptr,
+                                   xcd ? s.xcd_off.p : nullptr, s.xcd_max, s.p.stream)) {
       if (!eg::launch_ilu_sweep_df<T>(false, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col, mid,
-                                      y, y, s.p.flags, s.scaled_upper, xr, xo, s.xcd_max, s.p.stream)) {
+                                      y, y, s.p.flags, s.scaled_upper, xcd ? s.xcd_rows_up.p : nullptr,
+                                      xcd ? s.xcd_off_up.p : nullptr, s.xcd_max, s.p.stream)) {
         throw std::runtime_error("Co-resident launch of the upper ILU sweep failed");
       }
       return;
