@@ -1516,7 +1516,7 @@ template <> __device__ __forceinline__ float row16_sum<float>(float v) {
   v = dpp_addf<0x141>(v);
   return dpp_addf<0x140>(v);
 }
-template <class T, int B, int KD, bool LOWER>
+template <class T, int B, int KD, bool LOWER, bool LISTED>
 __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__restrict__ bsr, int nb,
                                                                     const int *__restrict__ row_ptr,
                                                                     const int *__restrict__ diag_ptr,
@@ -1542,14 +1542,17 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
       arm[t] = sentinel<T>();
     }
   }
-  // Rows of this workgroup: natural order (blockIdx.x, + gridDim.x, ..), or,
-  // with xrows, list x = blockIdx.x % 8 (xrows[xoff[x] .. xoff[x + 1]), in
-  // the sweep's dependency-level order) dealt to the workgroups b = x (mod 8)
-  // in turn. Workgroups are dispatched to the eight XCDs round-robin, so one
-  // list runs on one XCD and the exchanges between its rows stay within that
-  // XCD. Needs gridDim.x >= 8.
+  // Rows of this workgroup: natural order (blockIdx.x, + gridDim.x, ..),
+  // with the whole grid resident when a workgroup has several rows (with one
+  // row a workgroup a row waits only on rows of lower workgroup index, which
+  // each XCD dispatches first). LISTED: list x = blockIdx.x % 8 (xrows[xoff[x]
+  // .. xoff[x + 1]), in the sweep's dependency-level order) dealt to the
+  // workgroups b = x (mod 8) in turn; workgroups are dispatched to the eight
+  // XCDs round-robin, so one list runs on one XCD and the exchanges between
+  // its rows stay within that XCD (needs the whole grid resident and
+  // gridDim.x >= 8).
   int first = blockIdx.x, step = gridDim.x, count = nb, base = 0;
-  if (xrows) {
+  if constexpr (LISTED) {
     const int x = blockIdx.x & 7;
     first = blockIdx.x >> 3;
     step = (int(gridDim.x) - x + 7) >> 3;
@@ -1557,7 +1560,7 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
     count = xoff[x + 1] - base;
   }
   for (int idx = first; idx < count; idx += step) {
-    const int i = xrows ? xrows[base + idx] : (LOWER ? idx : nb - 1 - idx);
+    const int i = LISTED ? xrows[base + idx] : (LOWER ? idx : nb - 1 - idx);
     const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
     const int nd = j1 - j0;
     T a[KB][VB];
@@ -2070,18 +2073,21 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
   if (max_deps > 8) { return false; }
   const int kd = max_deps <= 4 ? 4 : 8;
   bool ok = false, known = false;
-  auto go = [&](const void *fn, int threads) {
+  auto go = [&](const void *fn, const void *fn_listed, int threads) {
     known = true;
     const int cap = coresident(fn, threads, 0);
     if (cap <= 0) { return; }
     int grid = std::min(nb, cap);
     const int *xr = nullptr, *xo = nullptr;
-    // by XCD only when every row has a workgroup of its own (with several
-    // rows a workgroup, the natural order measured faster at 128x128 Q6)
-    if (xrows && 8 * xmax <= cap && 8 * xmax >= 8) {
+    // By XCD only when every row has a workgroup of its own (with several
+    // rows a workgroup, the natural order measured faster at 128x128 Q6) and
+    // the listed kernel's occupancy bound holds the whole grid, which the
+    // listed order needs resident at once.
+    if (xrows && 8 * xmax >= 8 && coresident(fn_listed, threads, 0) >= 8 * xmax) {
       grid = 8 * xmax;
       xr = xrows;
       xo = xoff;
+      fn = fn_listed;
     }
     int spin = kSpinLimit, sc = scaled ? 1 : 0;
     void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin, &sc, &xr, &xo};
@@ -2089,8 +2095,10 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
     if (!ok) { (void)hipGetLastError(); }
   };
 #define EG_SWEEP(DISPATCH, KD)                                                                     \
-  DISPATCH(b, (go(lower ? reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, true>)           \
-                        : reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, false>),         \
+  DISPATCH(b, (go(lower ? reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, true, false>)    \
+                        : reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, false, false>),  \
+                  lower ? reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, true, true>)     \
+                        : reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, false, true>),   \
                   SweepShape<B>::NT)))
   // Cell blocks (DG, at most 4 lower or upper neighbours) or face blocks
   // (HDG, at most 7).
