@@ -266,6 +266,25 @@ def test_mixed_precision_run(gpu, oracle_lib):
         assert rel(u32, u64) < 1e-9
 
 
+@pytest.mark.parametrize("form", ["dg", "hdg"])
+def test_mixed_precision_multirow_sweeps(gpu, form):
+    """64x64 Q4, more block rows than dataflow-sweep workgroups (several rows
+    a workgroup, FP64 and FP32 sweeps): the mixed-precision increments against
+    the FP64 ones of the same run (relative 2-norm 1e-9; about 1e-12 is
+    measured). Too large for the oracle; the FP64 path is pinned against it at
+    the smaller sizes."""
+    args = "--nx 64 --ny 64 --k 4 --alpha-deg 30 --cfl 1 --steps 1 --newton-steps 1 --restart 30 --precond block-ilu0"
+    print(f"case: 64x64 Q4 mixed {form}", flush=True)
+    c64 = gpu.Context(args)
+    c32 = gpu.Context(args + " --precision mixed")
+    u64, s64, i64, _ = c64.run(form, increments=4)
+    u32, s32, i32, _ = c32.run(form, increments=4)
+    assert s64.updates == s32.updates > 0
+    for i in range(s64.updates):
+        assert np.all(np.isfinite(i32[i])) and rel(i32[i], i64[i]) < 1e-9, (form, i)
+    assert rel(u32, u64) < 1e-9
+
+
 def test_cli_reference_smoke(gpu, tmp_path):
     """code_test/test_solver.py:47-56 thresholds on the GPU euler2d CLI."""
     import csv
