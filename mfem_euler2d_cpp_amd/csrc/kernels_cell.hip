@@ -328,6 +328,11 @@ __device__ __forceinline__ bool face_pair(const Geo &g, int s, int sj, int i, in
 // (volume + own-side face terms, mass-scaled, +I) and, when off != null, the
 // four face-coupling blocks into BSR slots slot[e*4+side] (sides sharing a
 // neighbour accumulate into the same slot in side order).
+// Threads of the per-cell HDG condensation kernels: small cells take 512
+// (one cell a workgroup leaves few waves a CU on small grids; more threads a
+// cell hide the memory latency; 1024 would spill registers), larger ones 256
+// (their register use needs it).
+template <int Q> constexpr int cell_threads() { return Q <= 4 ? 512 : 256; }
 template <int Q>
 __global__ __launch_bounds__(256) void k_dg_blocks(Geo g, const Frozen *__restrict__ fz,
                                                    const double *__restrict__ u, double *__restrict__ diag,
@@ -391,7 +396,7 @@ __global__ __launch_bounds__(256) void k_dg_blocks(Geo g, const Frozen *__restri
 // HDG cell system (src/problem.cpp:642-702): A_e (m x m) and the
 // restriction blocks C_e = -w_q (+-J(U_e) + stab I) of each face point.
 template <int Q>
-__global__ __launch_bounds__(256) void k_hdg_build(Geo g, const Frozen *__restrict__ fz,
+__global__ __launch_bounds__(cell_threads<Q>()) void k_hdg_build(Geo g, const Frozen *__restrict__ fz,
                                                    const double *__restrict__ u, double *__restrict__ A,
                                                    double *__restrict__ C) {
   constexpr int P = Q * Q, M = 4 * P;
@@ -435,7 +440,7 @@ __global__ __launch_bounds__(256) void k_hdg_build(Geo g, const Frozen *__restri
 // phi_a M^{-1} sf at rows 4*node_a + c of the normal line through (side, qq),
 // so each X column combines q columns of A_e^{-1}. Output [X | y], m x (4t+1).
 template <int Q>
-__global__ __launch_bounds__(256) void k_hdg_solve(Geo g, const Frozen *__restrict__ fz,
+__global__ __launch_bounds__(cell_threads<Q>()) void k_hdg_solve(Geo g, const Frozen *__restrict__ fz,
                                                    const double *__restrict__ Ainv, const double *__restrict__ res,
                                                    double *__restrict__ Xy) {
   constexpr int P = Q * Q, M = 4 * P, T4 = 16 * Q;
@@ -474,7 +479,7 @@ __global__ __launch_bounds__(256) void k_hdg_solve(Geo g, const Frozen *__restri
 // S_e = C_e X_e written as the per-face row panels, and C_e y_e.
 // Panel storage: [face][role(0 left cell,1 right cell)][col 4t][row t].
 template <int Q>
-__global__ __launch_bounds__(256) void k_hdg_panels(Geo g, const double *__restrict__ Xy,
+__global__ __launch_bounds__(cell_threads<Q>()) void k_hdg_panels(Geo g, const double *__restrict__ Xy,
                                                     const double *__restrict__ C, double *__restrict__ panels,
                                                     double *__restrict__ cy) {
   constexpr int P = Q * Q, M = 4 * P, T = 4 * Q, T4 = 16 * Q;
@@ -700,15 +705,15 @@ void launch_dg_blocks(const Geo &g, const Frozen *fz, const double *u, double *d
   EG_DISPATCH_Q(g.q, (k_dg_blocks<Q><<<g.n, 256, 0, s>>>(g, fz, u, diag, dslot, bsr, slot)));
 }
 void launch_hdg_build(const Geo &g, const Frozen *fz, const double *u, double *A, double *C, hipStream_t s) {
-  EG_DISPATCH_Q(g.q, (k_hdg_build<Q><<<g.n, 256, 0, s>>>(g, fz, u, A, C)));
+  EG_DISPATCH_Q(g.q, (k_hdg_build<Q><<<g.n, cell_threads<Q>(), 0, s>>>(g, fz, u, A, C)));
 }
 void launch_hdg_solve(const Geo &g, const Frozen *fz, const double *Ainv, const double *res, double *Xy,
                       hipStream_t s) {
-  EG_DISPATCH_Q(g.q, (k_hdg_solve<Q><<<g.n, 256, 0, s>>>(g, fz, Ainv, res, Xy)));
+  EG_DISPATCH_Q(g.q, (k_hdg_solve<Q><<<g.n, cell_threads<Q>(), 0, s>>>(g, fz, Ainv, res, Xy)));
 }
 void launch_hdg_panels(const Geo &g, const double *Xy, const double *C, double *panels, double *cy,
                        hipStream_t s) {
-  EG_DISPATCH_Q(g.q, (k_hdg_panels<Q><<<g.n, 256, 0, s>>>(g, Xy, C, panels, cy)));
+  EG_DISPATCH_Q(g.q, (k_hdg_panels<Q><<<g.n, cell_threads<Q>(), 0, s>>>(g, Xy, C, panels, cy)));
 }
 void launch_hdg_face(const Geo &g, const Frozen *fz, const double *cy, double *rhs, double *D, hipStream_t s) {
   const int nb = (g.nf * g.t + 255) / 256;
