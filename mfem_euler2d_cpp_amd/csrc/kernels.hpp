@@ -144,7 +144,10 @@ template <class T>
 void launch_lincomb(const T *V, long ldv, int count, const Coef &coef, T *out, long n, hipStream_t s);
 template <class T>
 void launch_dot_partials(const T *a, const T *b, const T *metric, long n, double *partials, hipStream_t s);
-void launch_sum_partials(const double *partials, int count, double *out, int mode, hipStream_t s, int rows = 1);
+// host_out / flags / host_flags (optional): the sums also to pinned host
+// memory, and flags[0] to host_flags[0], by the kernel itself.
+void launch_sum_partials(const double *partials, int count, double *out, int mode, hipStream_t s, int rows = 1,
+                         double *host_out = nullptr, const int *flags = nullptr, int *host_flags = nullptr);
 template <class A, class B>
 void launch_convert(const A *x, B *y, long n, hipStream_t s);
 template <class T>

@@ -344,11 +344,16 @@ __global__ __launch_bounds__(256) void k_dot_partials(const T *__restrict__ a, c
 
 // mode 0: sum, mode 1: max.
 // One block per row of count partials: out[row] = sum (mode 0) or max (mode 1).
+// host_out (optional, pinned host memory): the sums are also written there,
+// and host_flags[0] = flags[0], so that one stream synchronisation brings
+// both to the host without copy launches.
 __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__ partials, int count,
-                                                      double *__restrict__ out, int mode) {
+                                                      double *__restrict__ out, int mode, double *host_out,
+                                                      const int *flags, int *host_flags) {
   __shared__ double red[16];
   partials += size_t(blockIdx.x) * count;
   out += blockIdx.x;
+  if (host_flags && blockIdx.x == 0 && threadIdx.x == 0) { host_flags[0] = flags[0]; }
   double acc = 0.;
   for (int k = threadIdx.x; k < count; k += blockDim.x) {
     acc = mode ? fmax(acc, partials[k]) : acc + partials[k];
@@ -361,10 +366,14 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
       double m = 0.;
       for (int i = 0; i < int(blockDim.x >> 6); ++i) { m = fmax(m, red[i]); }
       out[0] = m;
+      if (host_out) { host_out[blockIdx.x] = m; }
     }
   } else {
     const double s = block_sum(acc, red);
-    if (threadIdx.x == 0) { out[0] = s; }
+    if (threadIdx.x == 0) {
+      out[0] = s;
+      if (host_out) { host_out[blockIdx.x] = s; }
+    }
   }
 }
 
@@ -519,8 +528,9 @@ template <class T>
 void launch_dot_partials(const T *a, const T *b, const T *metric, long n, double *partials, hipStream_t s) {
   k_dot_partials<T><<<reduce_blocks(n), 256, 0, s>>>(a, b, metric, n, partials);
 }
-void launch_sum_partials(const double *partials, int count, double *out, int mode, hipStream_t s, int rows) {
-  k_sum_partials<<<rows, 256, 0, s>>>(partials, count, out, mode);
+void launch_sum_partials(const double *partials, int count, double *out, int mode, hipStream_t s, int rows,
+                         double *host_out, const int *flags, int *host_flags) {
+  k_sum_partials<<<rows, 256, 0, s>>>(partials, count, out, mode, host_out, flags, host_flags);
 }
 template <class A, class B> void launch_convert(const A *x, B *y, long n, hipStream_t s) {
   k_convert<A, B><<<grid_for(n), 256, 0, s>>>(x, y, n);

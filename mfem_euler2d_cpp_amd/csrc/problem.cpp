@@ -236,9 +236,8 @@ Problem::~Problem() {
 }
 void Problem::Sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
 double Problem::ReduceScalar(const double *dev_partials, int count, int mode) {
-  eg::launch_sum_partials(dev_partials, count, scalars, mode, stream);
-  hip_check(hipMemcpyAsync(pinned, scalars, sizeof(double), hipMemcpyDeviceToHost, stream), "readback");
-  hip_check(hipMemcpyAsync(pinned_flags, flags, sizeof(int), hipMemcpyDeviceToHost, stream), "readback");
+  // the kernel writes the sum and the error flags to pinned memory itself
+  eg::launch_sum_partials(dev_partials, count, scalars, mode, stream, 1, pinned, flags, pinned_flags);
   Sync();
   return pinned[0];
 }
@@ -335,9 +334,7 @@ double Problem::TrialResidual(const double *dev_u, const double *dev_du, double 
                               double *dev_r, bool &admissible) {
   const int nb = eg::cell_blocks(geo);
   eg::launch_residual(geo, frozen, dev_u, dev_du, lambda, old, dev_r, dev_trial, partial, flags, stream);
-  eg::launch_sum_partials(partial, nb, scalars, 0, stream, 2);
-  hip_check(hipMemcpyAsync(pinned, scalars, 2 * sizeof(double), hipMemcpyDeviceToHost, stream), "readback");
-  hip_check(hipMemcpyAsync(pinned_flags, flags, sizeof(int), hipMemcpyDeviceToHost, stream), "readback");
+  eg::launch_sum_partials(partial, nb, scalars, 0, stream, 2, pinned, flags, pinned_flags);
   Sync();
   last_state_norm = std::sqrt(pinned[1]);
   admissible = !(TakeFlags() & eg::kErrInadmissible);
