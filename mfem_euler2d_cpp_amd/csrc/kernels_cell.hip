@@ -328,6 +328,9 @@ __device__ __forceinline__ bool face_pair(const Geo &g, int s, int sj, int i, in
 // (volume + own-side face terms, mass-scaled, +I) and, when off != null, the
 // four face-coupling blocks into BSR slots slot[e*4+side] (sides sharing a
 // neighbour accumulate into the same slot in side order).
+// Workgroups a cell of the DG block fill: small cells 4 (one cell a
+// workgroup leaves few waves a CU on small grids), larger ones 1.
+template <int Q> constexpr int dg_blocks_split() { return Q <= 4 ? 4 : 1; }
 // Threads of the per-cell HDG condensation kernels: small cells take 512
 // (one cell a workgroup leaves few waves a CU on small grids; more threads a
 // cell hide the memory latency; 1024 would spill registers), larger ones 256
@@ -338,9 +341,11 @@ __global__ __launch_bounds__(256) void k_dg_blocks(Geo g, const Frozen *__restri
                                                    const double *__restrict__ u, double *__restrict__ diag,
                                                    const int *__restrict__ dslot, double *__restrict__ bsr,
                                                    const int *__restrict__ slot) {
-  constexpr int P = Q * Q, M = 4 * P;
+  constexpr int P = Q * Q, M = 4 * P, S = dg_blocks_split<Q>();
   __shared__ CellJac<Q> cj;
-  const int e = blockIdx.x;
+  // S workgroups a cell, each filling every S-th run of blockDim.x entries
+  // (the node and face Jacobians are formed by each of them)
+  const int e = blockIdx.x / S, part = blockIdx.x % S;
   const double *cell = u + size_t(e) * M;
   stage_node_jacobians<Q>(g, cell, cj);
   for (int fp = threadIdx.x; fp < 4 * Q; fp += blockDim.x) {
@@ -365,7 +370,7 @@ __global__ __launch_bounds__(256) void k_dg_blocks(Geo g, const Frozen *__restri
   }
   __syncthreads();
   double *dblk = diag + size_t(dslot ? dslot[e] : e) * M * M;
-  for (int idx = threadIdx.x; idx < M * M; idx += blockDim.x) {
+  for (int idx = part * blockDim.x + threadIdx.x; idx < M * M; idx += S * blockDim.x) {
     const int row = idx % M, col = idx / M;
     const int i = row / 4, c = row % 4, j = col / 4, d = col % 4;
     double v = volume_entry<Q>(g, cj, i, j, c, d);
@@ -702,7 +707,7 @@ template void launch_dg_apply<float>(const Geo &, const Frozen *, const double *
                                      hipStream_t);
 void launch_dg_blocks(const Geo &g, const Frozen *fz, const double *u, double *diag, const int *dslot,
                       double *bsr, const int *slot, hipStream_t s) {
-  EG_DISPATCH_Q(g.q, (k_dg_blocks<Q><<<g.n, 256, 0, s>>>(g, fz, u, diag, dslot, bsr, slot)));
+  EG_DISPATCH_Q(g.q, (k_dg_blocks<Q><<<g.n * dg_blocks_split<Q>(), 256, 0, s>>>(g, fz, u, diag, dslot, bsr, slot)));
 }
 void launch_hdg_build(const Geo &g, const Frozen *fz, const double *u, double *A, double *C, hipStream_t s) {
   EG_DISPATCH_Q(g.q, (k_hdg_build<Q><<<g.n, cell_threads<Q>(), 0, s>>>(g, fz, u, A, C)));
