@@ -754,26 +754,36 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_rows(double *__rest
 //   [len, first slot, blocks in row, diagonal position, dependencies,
 //    dependency slots..., per lower block: (inverse dependency, updates,
 //    (target position, dependency) x updates)].
+// Entry o of a B x B block product in LDS (column-major), the even and odd
+// terms in two accumulators (half the dependent chain) added at the end.
 template <int B>
 __device__ __forceinline__ double block_dot(const double *X, const double *Y, int o) {
+  static_assert(B % 2 == 0, "even block size");
   const int r = o % B, c = o / B;
-  double acc = 0.;
+  double acc0 = 0., acc1 = 0.;
 #pragma unroll 4
-  for (int kk = 0; kk < B; ++kk) { acc += X[r + kk * B] * Y[kk + c * B]; }
-  return acc;
+  for (int kk = 0; kk < B; kk += 2) {
+    acc0 += X[r + kk * B] * Y[kk + c * B];
+    acc1 += X[r + (kk + 1) * B] * Y[kk + 1 + c * B];
+  }
+  return acc0 + acc1;
 }
 template <int B>
 __device__ __forceinline__ void block_dot2(const double *X, const double *Y0, const double *Y1, int o, double &a0,
                                            double &a1) {
+  static_assert(B % 2 == 0, "even block size");
   const int r = o % B, c = o / B;
-  a0 = 0.;
-  a1 = 0.;
+  double a00 = 0., a01 = 0., a10 = 0., a11 = 0.;
 #pragma unroll 4
-  for (int kk = 0; kk < B; ++kk) {
-    const double x = X[r + kk * B];
-    a0 += x * Y0[kk + c * B];
-    a1 += x * Y1[kk + c * B];
+  for (int kk = 0; kk < B; kk += 2) {
+    const double x0 = X[r + kk * B], x1 = X[r + (kk + 1) * B];
+    a00 += x0 * Y0[kk + c * B];
+    a01 += x1 * Y0[kk + 1 + c * B];
+    a10 += x0 * Y1[kk + c * B];
+    a11 += x1 * Y1[kk + 1 + c * B];
   }
+  a0 = a00 + a01;
+  a1 = a10 + a11;
 }
 // Blocks b0 .. b1 - 1 of a row program's LDS image (the row's own blocks,
 // then its dependencies) loaded with all loads of a batch in flight before
