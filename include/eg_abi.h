@@ -87,6 +87,9 @@ int eg_dg_apply_f32(eg_ctx *ctx, const float *dev_x, float *dev_y);
 int eg_pc_setup(eg_ctx *ctx, int form);
 /* LinearSolver::Impl::Precondition (src/linear.cpp:66-75). */
 int eg_pc_apply(eg_ctx *ctx, int form, const double *dev_x, double *dev_y);
+/* The same preconditioner applied in FP32 (the mixed-precision inner solves;
+ * the context needs --precision mixed and a set-up preconditioner). */
+int eg_pc_apply_f32(eg_ctx *ctx, int form, const float *dev_x, float *dev_y);
 
 /* Problem::StreamCondense (src/condensation.cpp:131-148). */
 int eg_hdg_condense(eg_ctx *ctx, const double *dev_u, const double *dev_r);
@@ -163,8 +166,8 @@ int eg_serve(const char *const *args, const int *forms, const double *arrivals, 
  * setup plus 16 applications (form 0 dg / 1 hdg, precision "fp64" or
  * "mixed") on a 16x16 case of that k under three shapes: "levels-256" and
  * "levels-1024" (one launch per dependency level, 256 or 1024 threads per
- * block row of the sweeps) and "dataflow" (single cooperative launch per
- * sweep, for a device not shared with other solves). out + 16*i receives the
+ * block row of the sweeps) and "dataflow" (one launch per sweep, rows taken
+ * in dependency order from a ticket counter). out + 16*i receives the
  * fastest name (NUL-terminated); times (optional, 3*nk) the three times in
  * ms, in that order. */
 int eg_autotune_ilu(const int *ks, int nk, int form, const char *precision, char *out, double *times);

@@ -70,6 +70,7 @@ _SIGNATURES = {
     "eg_dg_apply_f32": (ctypes.c_int, [_vp, _vp, _vp]),
     "eg_pc_setup": (ctypes.c_int, [_vp, ctypes.c_int]),
     "eg_pc_apply": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    "eg_pc_apply_f32": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
     "eg_hdg_condense": (ctypes.c_int, [_vp, _vp, _vp]),
     "eg_hdg_rhs": (ctypes.c_int, [_vp, _vp, _vp]),
     "eg_hdg_xy": (ctypes.c_int, [_vp, _vp]),

@@ -144,6 +144,12 @@ int eg_pc_setup(eg_ctx *ctx, int form) { return guard(ctx, [&] { ctx->lin(form).
 int eg_pc_apply(eg_ctx *ctx, int form, const double *dev_x, double *dev_y) {
   return guard(ctx, [&] { ctx->lin(form).Precondition(dev_x, dev_y); });
 }
+int eg_pc_apply_f32(eg_ctx *ctx, int form, const float *dev_x, float *dev_y) {
+  return guard(ctx, [&] {
+    require(ctx->opt.precision == "mixed", "FP32 preconditioner needs --precision mixed");
+    ctx->lin(form).Precondition(dev_x, dev_y);
+  });
+}
 int eg_hdg_condense(eg_ctx *ctx, const double *dev_u, const double *dev_r) {
   return guard(ctx, [&] { ctx->hyb().StreamCondense(dev_u, dev_r); });
 }

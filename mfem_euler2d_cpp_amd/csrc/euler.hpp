@@ -238,9 +238,8 @@ public:
   // factor_pub: the published diagonal inverses and upper blocks it reads.
   bool resident_df = true;
   DeviceArray<double> factor_pub;
-  // Dataflow sweep rows by XCD (PlanXcdRows); empty: natural order.
-  DeviceArray<int> xcd_rows, xcd_off, xcd_rows_up, xcd_off_up;
-  int xcd_max = 0;
+  // Ticket counters of the dataflow sweeps (two ints, zero between launches).
+  DeviceArray<int> sweep_ticket;
   // Per-level factor launches recorded once into a graph (EG_FACTOR_GRAPH=0:
   // launched directly).
   bool use_graph = true;
@@ -267,7 +266,6 @@ public:
   LinearSolver &operator=(const LinearSolver &) = delete;
   void PlanBigFactor();
   void PlanResident();
-  void PlanXcdRows();
   void FactorLevels();
   void Refresh(const double *dev_u) { state = dev_u; }
   // Fill + factor the preconditioner at the current state. check_flags
@@ -282,6 +280,7 @@ public:
   // x (member) is the solution; res holds rhs - A x on return.
   void Solve(const double *dev_rhs, LinearResult &out, double tol, bool refresh);
   double Norm(const double *v);
+  [[noreturn]] void Fail(const char *what);
 private:
   template <class T> void EnqueueArnoldi(T *Vb, T *zb, T *wb, const T *metric, int j);
   template <class T> double GMRES(const T *rhs, T *xv, double tol, int limit, int &iterations, bool zero_guess,

@@ -108,24 +108,26 @@ void launch_ilu_scale_upper(double *bsr, int b, const int *slot, const int *dslo
 
 void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                      const int *col, const int *upd_ptr, const int *upd, bool star, hipStream_t s);
-// Single-launch dataflow forms (grid within the device's occupancy, per-row
-// completion flags or sentinel-armed values);
-// return false when a co-resident launch is not possible (caller falls back
-// to the per-level launches).
+// Single-launch dataflow forms: rows taken from a ticket counter in
+// dependency order (per-row completion flags or sentinel-armed values), so
+// they finish whether or not the whole grid is resident; return false when
+// the block size has no such kernel (the caller uses the per-level launches).
 // Sweeps: src/dst/arm as documented at k_ilu_sweep_sn (lower: src = rhs,
 // dst = intermediate, arm = final output; upper: src = intermediate, dst =
 // final output). The intermediate must be armed before the first lower sweep.
-// xrows / xoff (optional, blocks up to 100): eight row lists, each in the
-// sweep's dependency-level order, list x run by the workgroups b = x (mod 8),
-// i.e. on one XCD; xmax: the longest.
+// order: the nb rows in the sweep's dependency-level order (the lower or
+// upper level lists of the pattern); ticket: two ints, zero before the first
+// launch and left zero by each launch; launches sharing them must be ordered
+// on one stream.
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
                          const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, bool scaled,
-                         const int *xrows, const int *xoff, int xmax, hipStream_t s);
+                         const int *order, int *ticket, hipStream_t s);
 template <class T> void launch_arm(T *v, size_t n, hipStream_t s);
 
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
-                          const int *upd_ptr, const int *upd, int *done, int epoch, int *flags, hipStream_t s);
+                          const int *upd_ptr, const int *upd, int *done, int epoch, const int *order, int *ticket,
+                          int *flags, hipStream_t s);
 
 // Vector kernels (Krylov). nb = kReduceBlocks partials per reduction.
 struct Coef {

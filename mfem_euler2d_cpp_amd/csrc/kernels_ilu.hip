@@ -1437,17 +1437,31 @@ __global__ __launch_bounds__(NT) void k_ilu_upper_split(const T *__restrict__ bs
 }
 
 // ------------------------------------------------------------ dataflow forms
-// One launch per sweep (or per factorisation) instead of one per level: the
-// grid is no larger than the device holds at once (occupancy of the kernel
-// on every CU), so on the library's in-order stream all its workgroups are
-// resident together, and each workgroup handles its rows in dependency order
+// One launch per sweep (or per factorisation) instead of one per level. The
+// workgroups take their rows from a ticket counter in dependency order
 // (increasing for the lower sweep and the factor, decreasing for the upper
-// sweep). As the rows waited on precede the waiting row in that order, every
-// wait ends; each wait is nevertheless bounded by spin_limit and then raises
-// kErrSpin instead of hanging. A plain launch is used: a cooperative launch
-// costs about 20 us more on this stack, as much as a whole sweep of a small
-// case. Data crossing workgroups is read and written with coherent (sc1)
-// accesses only.
+// sweep), so every row waited on has been taken before the waiting row by a
+// workgroup that is running: every wait ends whether or not the whole grid is
+// resident at once (other solves may share the device), and the grid is only
+// sized to what the device can hold, for speed. A workgroup takes its next
+// ticket when it starts a row (the atomic's latency is hidden behind the
+// row's loads); holding one ticket ahead keeps the order argument: the
+// smallest unfinished ticket is always held, as current row, by a running
+// workgroup whose dependencies are all finished. Each wait is nevertheless
+// bounded by spin_limit and then raises kErrSpin instead of hanging. A plain
+// launch is used: a cooperative launch costs about 20 us more on this stack,
+// as much as a whole sweep of a small case. Data crossing workgroups is read
+// and written with coherent (sc1) accesses only.
+//
+// ticket[0] counts the tickets taken and ticket[1] the workgroups that have
+// taken their last one; the last of those resets both to zero for the next
+// launch on the stream.
+__device__ __forceinline__ void release_tickets(int *ticket) {
+  if (threadIdx.x == 0 && atomicAdd(ticket + 1, 1) == int(gridDim.x) - 1) {
+    atomicExch(ticket, 0);
+    atomicExch(ticket + 1, 0);
+  }
+}
 //
 // Factor: before a row the workgroup waits until every row it depends on has
 // published done[j] >= epoch; after the row it completes its stores and
@@ -1526,20 +1540,20 @@ template <> __device__ __forceinline__ float row16_sum<float>(float v) {
   v = dpp_addf<0x141>(v);
   return dpp_addf<0x140>(v);
 }
-template <class T, int B, int KD, bool LOWER, bool LISTED>
+template <class T, int B, int KD, bool LOWER>
 __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__restrict__ bsr, int nb,
                                                                     const int *__restrict__ row_ptr,
                                                                     const int *__restrict__ diag_ptr,
                                                                     const int *__restrict__ col, T *src, T *dst,
                                                                     T *arm, int *flags, int spin_limit, int scaled,
-                                                                    const int *__restrict__ xrows,
-                                                                    const int *__restrict__ xoff) {
+                                                                    const int *__restrict__ order, int *ticket) {
   constexpr int NT = SweepShape<B>::NT, P = SweepShape<B>::P, VB = SweepShape<B>::VB;
   constexpr int KB = LOWER ? KD : KD + 1; // blocks held per row (upper: + inv(U_ii))
   constexpr size_t bb = size_t(B) * B;
   __shared__ T xs[KD * B];
   __shared__ T red[NT];
   __shared__ T ts[B];
+  __shared__ int next_row;
   // 16 x 16 blocks (P == 16): the parts of a row are one DPP row of lanes
   // (part fastest) and the row's partial sums are added by DPP, with no LDS
   // round trip or workgroup barrier; otherwise rows fastest (coalesced block
@@ -1547,30 +1561,22 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
   constexpr bool RR = P == 16 && B == 16;
   const int tid = threadIdx.x, r = RR ? tid / P : tid % B, part = RR ? tid % P : tid / B;
   const bool worker = part < P;
+  if (tid == 0) {
+    const int first = atomicAdd(ticket, 1);
+    next_row = first < nb ? order[first] : -1;
+  }
   if (LOWER) {
     for (size_t t = size_t(blockIdx.x) * NT + tid; t < size_t(nb) * B; t += size_t(gridDim.x) * NT) {
       arm[t] = sentinel<T>();
     }
   }
-  // Rows of this workgroup: natural order (blockIdx.x, + gridDim.x, ..),
-  // with the whole grid resident when a workgroup has several rows (with one
-  // row a workgroup a row waits only on rows of lower workgroup index, which
-  // each XCD dispatches first). LISTED: list x = blockIdx.x % 8 (xrows[xoff[x]
-  // .. xoff[x + 1]), in the sweep's dependency-level order) dealt to the
-  // workgroups b = x (mod 8) in turn; workgroups are dispatched to the eight
-  // XCDs round-robin, so one list runs on one XCD and the exchanges between
-  // its rows stay within that XCD (needs the whole grid resident and
-  // gridDim.x >= 8).
-  int first = blockIdx.x, step = gridDim.x, count = nb, base = 0;
-  if constexpr (LISTED) {
-    const int x = blockIdx.x & 7;
-    first = blockIdx.x >> 3;
-    step = (int(gridDim.x) - x + 7) >> 3;
-    base = xoff[x];
-    count = xoff[x + 1] - base;
-  }
-  for (int idx = first; idx < count; idx += step) {
-    const int i = LISTED ? xrows[base + idx] : (LOWER ? idx : nb - 1 - idx);
+  __syncthreads();
+  // Rows from the ticket counter (release_tickets): ticket t is row order[t],
+  // the rows of the sweep in its dependency-level order (a natural-order
+  // window of tickets would hold only a few rows of one level).
+  for (int i = next_row; i >= 0;) {
+    int ahead = 0, ahead_row = -1;
+    if (tid == 0) { ahead = atomicAdd(ticket, 1); }
     const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
     const int nd = j1 - j0;
     T a[KB][VB];
@@ -1622,6 +1628,8 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
       xs[t] = x;
     }
     __syncthreads();
+    // the ticket has arrived by now: look up its row while this one is summed
+    if (tid == 0 && ahead < nb) { ahead_row = order[ahead]; }
     if (RR && !LOWER && scaled && part == 0) { src[size_t(i) * B + r] = sentinel<T>(); }
     T acc = T(0);
 #pragma unroll
@@ -1668,8 +1676,13 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
       }
       if (part == 0) { st_coherent(dst + size_t(i) * B + r, sum2); }
     }
+    // every thread read next_row at the top of this row, before the barrier
+    // after the dependency wait
+    if (tid == 0) { next_row = ahead_row; }
     __syncthreads();
+    i = next_row;
   }
+  release_tickets(ticket);
 }
 template <class T> __global__ void k_arm(T *v, size_t n) {
   for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += size_t(gridDim.x) * blockDim.x) {
@@ -1705,22 +1718,31 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_stream(const T *__restrict__
                                                            const int *__restrict__ row_ptr,
                                                            const int *__restrict__ diag_ptr,
                                                            const int *__restrict__ col, T *src, T *dst, T *arm,
-                                                           int *flags, int spin_limit) {
+                                                           int *flags, int spin_limit,
+                                                           const int *__restrict__ order, int *ticket) {
   constexpr int NT = 1024, PRE = 4, KDM = 4; // cell blocks: at most 4 neighbours a triangle
   extern __shared__ unsigned char stream_lds[];
+  __shared__ int next_row;
   T *xs = reinterpret_cast<T *>(stream_lds); // kd x b dependency values
   T *red = xs + size_t(kd) * b;               // NT partial sums
   T *ts = red + NT;                           // b
   const int tid = threadIdx.x, P = NT / b, r = tid % b, part = tid / b;
   const bool worker = part < P;
   const size_t bb = size_t(b) * b;
+  if (tid == 0) {
+    const int first = atomicAdd(ticket, 1);
+    next_row = first < nb ? order[first] : -1;
+  }
   if (LOWER) {
     for (size_t t = size_t(blockIdx.x) * NT + tid; t < size_t(nb) * b; t += size_t(gridDim.x) * NT) {
       arm[t] = sentinel<T>();
     }
   }
-  for (int idx = blockIdx.x; idx < nb; idx += gridDim.x) {
-    const int i = LOWER ? idx : nb - 1 - idx;
+  __syncthreads();
+  // rows from the ticket counter in level order, as in k_ilu_sweep_sn
+  for (int i = next_row; i >= 0;) {
+    int ahead = 0, ahead_row = -1;
+    if (tid == 0) { ahead = atomicAdd(ticket, 1); }
     const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
     const int nd = j1 - j0;
     T pre[KDM][PRE];
@@ -1752,6 +1774,7 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_stream(const T *__restrict__
       xs[t] = x;
     }
     __syncthreads();
+    if (tid == 0 && ahead < nb) { ahead_row = order[ahead]; }
     T acc = T(0);
     if (worker) {
 #pragma unroll
@@ -1795,27 +1818,42 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_stream(const T *__restrict__
         st_coherent(dst + size_t(i) * b + r, sum);
       }
     }
+    if (tid == 0) { next_row = ahead_row; } // read by every thread before the barrier after the wait
     __syncthreads();
+    i = next_row;
   }
+  release_tickets(ticket);
 }
 
-// Dataflow ILU(0) factor: factor_row per row, rows in increasing order per
-// workgroup, waiting on the rows of the lower pattern.
+// Dataflow ILU(0) factor: factor_row per row, rows taken from the ticket
+// counter in level order (order = the rows by dependency level), waiting on
+// the rows of the lower pattern.
 template <int B>
 __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_df(double *bsr, int nb, const int *__restrict__ row_ptr,
                                                                 const int *__restrict__ diag_ptr,
                                                                 const int *__restrict__ col,
                                                                 const int *__restrict__ upd_ptr,
                                                                 const int *__restrict__ upd, int *done, int epoch,
-                                                                int *flags, int spin_limit) {
+                                                                int *flags, int spin_limit,
+                                                                const int *__restrict__ order, int *ticket) {
   extern __shared__ double sm[];
-  for (int i = blockIdx.x; i < nb; i += gridDim.x) {
+  __shared__ int row;
+  for (;;) {
+    __syncthreads(); // row is no longer read
+    if (threadIdx.x == 0) {
+      const int t = atomicAdd(ticket, 1);
+      row = t < nb ? order[t] : -1;
+    }
+    __syncthreads();
+    const int i = row;
+    if (i < 0) { break; }
     wait_rows(col, row_ptr[i], diag_ptr[i], done, epoch, spin_limit, flags);
     if (factor_row<B, true>(bsr, i, row_ptr, diag_ptr, col, upd_ptr, upd, sm) && threadIdx.x == 0) {
       atomicOr(flags, kErrSingular);
     }
     publish_row(done, i, epoch);
   }
+  release_tickets(ticket);
 }
 
 // Dependencies of a row program read from the published copy of the factor
@@ -1868,8 +1906,7 @@ __device__ __forceinline__ void resident_poll(const double *pub, const int *P, d
 // writes it to bsr and publishes its diagonal inverse and upper blocks to pub
 // with coherent stores. Rows are taken in dependency order by running
 // workgroups, so every wait ends whether or not the whole grid is resident
-// at once. ticket[0] is the counter and ticket[1] counts the workgroups that
-// are done; the last one resets both to zero for the next launch.
+// at once (ticket counters as in release_tickets).
 template <int B>
 __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident_df(double *__restrict__ bsr, double *pub,
                                                                          const int *__restrict__ prog_off,
@@ -1900,11 +1937,7 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident_df(double 
     }
     if (singular && t == 0) { atomicOr(flags, kErrSingular); }
   }
-  if (t == 0 && atomicAdd(ticket + 1, 1) == int(gridDim.x) - 1) {
-    // every workgroup has taken its last ticket
-    atomicExch(ticket, 0);
-    atomicExch(ticket + 1, 0);
-  }
+  release_tickets(ticket);
 }
 
 // Largest co-resident grid of a kernel (all CUs x blocks per CU).
@@ -2079,36 +2112,23 @@ void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, con
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
                          const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, bool scaled,
-                         const int *xrows, const int *xoff, int xmax, hipStream_t s) {
+                         const int *order, int *ticket, hipStream_t s) {
   if (max_deps > 8) { return false; }
   const int kd = max_deps <= 4 ? 4 : 8;
   bool ok = false, known = false;
-  auto go = [&](const void *fn, const void *fn_listed, int threads) {
+  auto go = [&](const void *fn, int threads) {
     known = true;
+    // more workgroups than the device holds at once would only wait for a slot
     const int cap = coresident(fn, threads, 0);
-    if (cap <= 0) { return; }
-    int grid = std::min(nb, cap);
-    const int *xr = nullptr, *xo = nullptr;
-    // By XCD only when every row has a workgroup of its own (with several
-    // rows a workgroup, the natural order measured faster at 128x128 Q6) and
-    // the listed kernel's occupancy bound holds the whole grid, which the
-    // listed order needs resident at once.
-    if (xrows && 8 * xmax >= 8 && coresident(fn_listed, threads, 0) >= 8 * xmax) {
-      grid = 8 * xmax;
-      xr = xrows;
-      xo = xoff;
-      fn = fn_listed;
-    }
+    const int grid = std::max(1, std::min(nb, cap > 0 ? cap : nb));
     int spin = kSpinLimit, sc = scaled ? 1 : 0;
-    void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin, &sc, &xr, &xo};
+    void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin, &sc, &order, &ticket};
     ok = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s) == hipSuccess;
     if (!ok) { (void)hipGetLastError(); }
   };
 #define EG_SWEEP(DISPATCH, KD)                                                                     \
-  DISPATCH(b, (go(lower ? reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, true, false>)    \
-                        : reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, false, false>),  \
-                  lower ? reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, true, true>)     \
-                        : reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, false, true>),   \
+  DISPATCH(b, (go(lower ? reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, true>)           \
+                        : reinterpret_cast<const void *>(k_ilu_sweep_sn<T, B, KD, false>),         \
                   SweepShape<B>::NT)))
   // Cell blocks (DG, at most 4 lower or upper neighbours) or face blocks
   // (HDG, at most 7).
@@ -2121,13 +2141,12 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
                            : reinterpret_cast<const void *>(k_ilu_sweep_stream<T, false>);
     const size_t lds = (size_t(kd) * b + 1024 + b) * sizeof(T);
     const int cap = coresident(fn, 1024, lds);
-    if (cap > 0) {
-      const int grid = std::min(nb, cap);
-      int spin = kSpinLimit, bv = b, kdv = kd;
-      void *args[] = {&bsr, &bv, &nb, &kdv, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin};
-      ok = hipLaunchKernel(fn, dim3(grid), dim3(1024), args, lds, s) == hipSuccess;
-      if (!ok) { (void)hipGetLastError(); }
-    }
+    const int grid = std::max(1, std::min(nb, cap > 0 ? cap : nb));
+    int spin = kSpinLimit, bv = b, kdv = kd;
+    void *args[] = {&bsr, &bv, &nb, &kdv, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin, &order,
+                    &ticket};
+    ok = hipLaunchKernel(fn, dim3(grid), dim3(1024), args, lds, s) == hipSuccess;
+    if (!ok) { (void)hipGetLastError(); }
   }
   return known && ok;
 }
@@ -2135,7 +2154,8 @@ template <class T> void launch_arm(T *v, size_t n, hipStream_t s) {
   k_arm<T><<<int(std::min<size_t>((n + 255) / 256, 4096)), 256, 0, s>>>(v, n);
 }
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
-                          const int *upd_ptr, const int *upd, int *done, int epoch, int *flags, hipStream_t s) {
+                          const int *upd_ptr, const int *upd, int *done, int epoch, const int *order, int *ticket,
+                          int *flags, hipStream_t s) {
   if (!fast_dense_ok(b)) { return false; }
   bool ok = false;
   EG_DISPATCH_B(b, ({
@@ -2145,7 +2165,8 @@ bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const 
     if (cap > 0) {
       const int grid = std::min(nb, cap);
       int spin = kSpinLimit;
-      void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &upd_ptr, &upd, &done, &epoch, &flags, &spin};
+      void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &upd_ptr, &upd, &done, &epoch, &flags, &spin, &order,
+                      &ticket};
       ok = hipLaunchKernel(fn, dim3(grid), dim3(Shape<B>::NT), args, lds, s) == hipSuccess;
       if (!ok) { (void)hipGetLastError(); }
     }
@@ -2180,11 +2201,10 @@ template void launch_ilu_upper_level<double>(const double *, int, const int *, i
 template void launch_ilu_upper_level<float>(const float *, int, const int *, int, const int *, const int *,
                                             const int *, float *, int, bool, hipStream_t);
 template bool launch_ilu_sweep_df<double>(bool, const double *, int, int, int, const int *, const int *, const int *,
-                                          double *, double *, double *, int *, bool, const int *, const int *, int,
+                                          double *, double *, double *, int *, bool, const int *, int *,
                                           hipStream_t);
 template bool launch_ilu_sweep_df<float>(bool, const float *, int, int, int, const int *, const int *, const int *,
-                                         float *, float *, float *, int *, bool, const int *, const int *, int,
-                                         hipStream_t);
+                                         float *, float *, float *, int *, bool, const int *, int *, hipStream_t);
 template void launch_arm<double>(double *, size_t, hipStream_t);
 template void launch_arm<float>(float *, size_t, hipStream_t);
 } // namespace eg

@@ -52,13 +52,14 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     use_graph = !(graph && std::string(graph) == "0");
     const char *rdf = std::getenv("EG_RESIDENT_DF");
     resident_df = !(rdf && std::string(rdf) == "0");
-    done_factor.resize(pattern->nb + 2); // row flags, then the resident factor's ticket counters
+    done_factor.resize(pattern->nb + 2); // row flags, then the single-launch factors' ticket counters
     hip_check(hipMemsetAsync(done_factor, 0, sizeof(int) * done_factor.n, p.stream), "memset");
     sweep.resize(N);
     if (mixed) { sweep32.resize(N); }
     if (eg::ilu_resident_ok(block)) { PlanResident(); }
     if (resident && dataflow && resident_df) { factor_pub.resize(blocks.n); }
-    if (dataflow) { PlanXcdRows(); }
+    sweep_ticket.resize(2);
+    hip_check(hipMemsetAsync(sweep_ticket, 0, sizeof(int) * 2, p.stream), "memset");
     if (eg::big_block(block) && !eg::ilu_slab_ok(block)) { PlanBigFactor(); }
     {
       // the cell stencil: each lower block updates only the diagonal
@@ -251,43 +252,6 @@ void LinearSolver::PlanResident() {
   resident = true;
 }
 
-// Rows of the dataflow sweeps by XCD: the cell grid cut into eight strips of
-// whole grid rows (a cell row of the DG system is its cell, a face row of
-// the HDG system the cell it is the right or top face of), so a row's
-// dependencies in its own grid row and, except at a strip edge, in the grid
-// row below are exchanged within one XCD. Each strip's rows are kept in the
-// sweep's dependency-level order. EG_SWEEP_XCD=0: natural order.
-void LinearSolver::PlanXcdRows() {
-  const char *env = std::getenv("EG_SWEEP_XCD");
-  if (env && std::string(env) == "0") { return; }
-  const auto &pt = *pattern;
-  const int nx = p.opt.nx, ny = p.opt.ny, per_cell = form == 0 ? 1 : 2;
-  if (pt.nb != nx * ny * per_cell || pt.nb < 64) { return; }
-  // each strip's rows in the sweep's level order (lower levels for the
-  // lower sweep, upper levels for the upper one), so that the workgroups of
-  // a strip hold rows of consecutive levels
-  int most = 0;
-  auto plan = [&](const std::vector<int> &order, DeviceArray<int> &d_rows, DeviceArray<int> &d_off) {
-    std::vector<std::vector<int>> lists(8);
-    for (const int i : order) { lists[size_t((i / per_cell / nx) * 8 / ny)].push_back(i); }
-    std::vector<int> rows, off{0};
-    for (const auto &l : lists) {
-      rows.insert(rows.end(), l.begin(), l.end());
-      off.push_back(int(rows.size()));
-      most = std::max(most, int(l.size()));
-    }
-    d_rows.resize(rows.size());
-    d_off.resize(off.size());
-    hip_check(hipMemcpyAsync(d_rows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, p.stream),
-              "upload");
-    hip_check(hipMemcpyAsync(d_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, p.stream), "upload");
-    hip_check(hipStreamSynchronize(p.stream), "sync");
-  };
-  plan(pt.lower_rows, xcd_rows, xcd_off);
-  plan(pt.upper_rows, xcd_rows_up, xcd_off_up);
-  xcd_max = most;
-}
-
 // Fill and factor the preconditioner at the current Newton state
 // (src/linear.cpp:182-203; src/partial.cpp:140-173,265-298;
 // src/operators.cpp:48-74,269-303; src/block_ilu.cpp:73-86).
@@ -311,7 +275,8 @@ void LinearSolver::Setup(bool check_flags) {
   if (ilu) {
     const auto &pt = *pattern;
     bool df = dataflow_factor && eg::launch_ilu_factor_df(blocks, b, pt.nb, pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr,
-                                                   pt.d_upd, done_factor, ++epoch_factor, p.flags, p.stream);
+                                                   pt.d_upd, done_factor, ++epoch_factor, pt.d_lower_rows,
+                                                   done_factor.p + pt.nb, p.flags, p.stream);
     // row-resident programs: one launch for all levels (the rows wait on
     // each other's completion flags) when the sweeps are single-launch too
     if (!df && resident && dataflow && resident_df) {
@@ -417,18 +382,17 @@ template <class T> static void precondition(LinearSolver &s, const T *bsr, const
   if (s.dataflow) {
     require(b != y, "ILU apply needs distinct input and output vectors");
     T *mid = sizeof(T) == 8 ? reinterpret_cast<T *>(s.sweep.p) : reinterpret_cast<T *>(s.sweep32.p);
-    const bool xcd = s.xcd_rows.n > 0;
     if (eg::launch_ilu_sweep_df<T>(true, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col,
-                                   const_cast<T *>(b), mid, y, s.p.flags, false, xcd ? s.xcd_rows.p : nullptr,
-                                   xcd ? s.xcd_off.p : nullptr, s.xcd_max, s.p.stream)) {
+                                   const_cast<T *>(b), mid, y, s.p.flags, false, pt.d_lower_rows, s.sweep_ticket,
+                                   s.p.stream)) {
       if (!eg::launch_ilu_sweep_df<T>(false, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col, mid,
-                                      y, y, s.p.flags, s.scaled_upper, xcd ? s.xcd_rows_up.p : nullptr,
-                                      xcd ? s.xcd_off_up.p : nullptr, s.xcd_max, s.p.stream)) {
-        throw std::runtime_error("Co-resident launch of the upper ILU sweep failed");
+                                      y, y, s.p.flags, s.scaled_upper, pt.d_upper_rows, s.sweep_ticket,
+                                      s.p.stream)) {
+        throw std::runtime_error("Launch of the upper ILU sweep failed");
       }
       return;
     }
-    s.dataflow = false; // not co-resident on this device: per-level launches from now on
+    s.dataflow = false; // no single-launch sweep for this block size: per-level launches from now on
   }
   for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
     const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
@@ -449,6 +413,15 @@ double LinearSolver::Norm(const double *v) {
   eg::launch_dot_partials<double>(v, v, metric, N, p.partial, p.stream);
   return std::sqrt(p.ReduceScalar(p.partial, eg::reduce_blocks(N), 0));
 }
+// A Krylov failure (breakdown, nonfinite residual) is first checked against
+// the device error flags, so that its cause is named: a dataflow dependency
+// wait that exceeded its bound (its output is the NaN sentinel) or a zero
+// pivot of the preconditioner, rather than the Krylov symptom.
+void LinearSolver::Fail(const char *what) {
+  p.CheckFlags(ilu ? "Singular BSR ILU diagonal" : "Singular block Jacobi factor");
+  throw std::runtime_error(what);
+}
+
 // Arnoldi step j of GMRES (src/linear.cpp:100-131): z = M^-1 v_j, w = A z,
 // CGS2 against v_0..v_j (two fused projection passes), the norm and
 // v_{j+1} = w / |w|. The Hessenberg column (both passes and |w|^2) is written
@@ -543,7 +516,7 @@ double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iter
         H(i + 1, j) = -sine[i] * u + cosine[i] * vr;
       }
       const double d = std::hypot(H(j, j), H(j + 1, j));
-      require(std::isfinite(d) && d > 0, "GMRES Arnoldi breakdown");
+      if (!(std::isfinite(d) && d > 0)) { Fail("GMRES Arnoldi breakdown"); }
       cosine[j] = H(j, j) / d;
       sine[j] = H(j + 1, j) / d;
       H(j, j) = d;
@@ -569,7 +542,7 @@ double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iter
     eg::launch_axpby<T>(rb, -1., ab, 1., N, s);
     residual_norm = norm(rb);
     require(!(*p.pinned_flags & eg::kErrSpin), "ILU dataflow dependency wait exceeded its bound");
-    require(std::isfinite(residual_norm), "Nonfinite GMRES residual");
+    if (!std::isfinite(residual_norm)) { Fail("Nonfinite GMRES residual"); }
   }
   if (true_norm) { *true_norm = residual_norm; }
   return residual_norm / rhs_norm;
@@ -610,7 +583,7 @@ double LinearSolver::MixedSolve(const double *rhs, double tol, int limit, int &i
     Apply(x.p, act.p);
     eg::launch_sub(rhs, act, res, N, s);
     residual_norm = Norm(res);
-    require(std::isfinite(residual_norm), "Nonfinite GMRES residual");
+    if (!std::isfinite(residual_norm)) { Fail("Nonfinite GMRES residual"); }
     if (inner == 0) { break; }
     if (residual_norm > 0.5 * before && residual_norm > tol * rhs_norm) {
       // The FP32 inner solves stall (the system is too ill-conditioned for

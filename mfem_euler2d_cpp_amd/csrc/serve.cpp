@@ -69,11 +69,6 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
     try {
       require(requests[i].form == 0 || requests[i].form == 1, "form must be 0 (dg) or 1 (hdg)");
       opts[i] = ParseOptions(requests[i].args);
-      // The single-launch ILU schedules assume that their grid
-      // is resident on the whole device at once, which concurrent solves on
-      // other streams break: solves that share the device run the factor and
-      // the sweeps one launch per dependency level.
-      opts[i].ilu_schedule = "levels";
       out[i].footprint = SolveFootprint(opts[i], requests[i].form);
       if (double(out[i].footprint) > budget) {
         out[i].status = 1;
@@ -98,9 +93,20 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
   const auto t0 = std::chrono::steady_clock::now();
   auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
 
+  // The workers' streams are made here, before any thread starts, so that a
+  // failure ends Serve with an error instead of inside a worker thread.
+  std::vector<hipStream_t> streams(cfg.workers, nullptr);
+  for (auto &st : streams) {
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+      (void)hipGetLastError();
+      for (auto &made : streams) {
+        if (made) { (void)hipStreamDestroy(made); }
+      }
+      throw std::runtime_error("Serve could not create its worker streams");
+    }
+  }
   auto worker = [&](int wid) {
-    hipStream_t stream = nullptr;
-    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    hipStream_t stream = streams[wid];
     for (;;) {
       int i = -1;
       {
@@ -144,7 +150,6 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
       }
       wake_sched.notify_one();
     }
-    (void)hipStreamDestroy(stream);
   };
   std::vector<std::thread> pool;
   for (int w = 0; w < cfg.workers; ++w) { pool.emplace_back(worker, w); }
@@ -180,6 +185,7 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
   }
   wake_work.notify_all();
   for (auto &th : pool) { th.join(); }
+  for (auto &st : streams) { (void)hipStreamDestroy(st); }
   // Give the memory the solves cached in the stream-ordered pool back.
   int dev = 0;
   hipMemPool_t mempool = nullptr;

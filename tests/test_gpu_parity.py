@@ -266,15 +266,22 @@ def test_mixed_precision_run(gpu, oracle_lib):
         assert rel(u32, u64) < 1e-9
 
 
+LARGE = {"64x64 Q4": "--nx 64 --ny 64 --k 4 --alpha-deg 30 --cfl 1 --steps 1 --newton-steps 1 --restart 30 "
+                      "--precond block-ilu0",
+         "128x128 Q6": "--nx 128 --ny 128 --k 6 --alpha-deg 30 --cfl 1 --steps 1 --newton-steps 1 --restart 30 "
+                       "--precond block-ilu0"}
+
+
+@pytest.mark.parametrize("size", list(LARGE))
 @pytest.mark.parametrize("form", ["dg", "hdg"])
-def test_mixed_precision_multirow_sweeps(gpu, form):
-    """64x64 Q4, more block rows than dataflow-sweep workgroups (several rows
-    a workgroup, FP64 and FP32 sweeps): the mixed-precision increments against
-    the FP64 ones of the same run (relative 2-norm 1e-9; about 1e-12 is
-    measured). Too large for the oracle; the FP64 path is pinned against it at
-    the smaller sizes."""
-    args = "--nx 64 --ny 64 --k 4 --alpha-deg 30 --cfl 1 --steps 1 --newton-steps 1 --restart 30 --precond block-ilu0"
-    print(f"case: 64x64 Q4 mixed {form}", flush=True)
+def test_mixed_precision_multirow_sweeps(gpu, form, size):
+    """64x64 Q4 and 128x128 Q6 (the mixed-precision workloads), more block
+    rows than dataflow-sweep workgroups (several rows a workgroup, FP64 and
+    FP32 sweeps): the mixed-precision increments against the FP64 ones of the
+    same run (relative 2-norm 1e-9; about 1e-12 is measured). Too large for
+    the oracle; the FP64 path is pinned against it at the smaller sizes."""
+    args = LARGE[size]
+    print(f"case: {size} mixed {form}", flush=True)
     c64 = gpu.Context(args)
     c32 = gpu.Context(args + " --precision mixed")
     u64, s64, i64, _ = c64.run(form, increments=4)
@@ -346,7 +353,7 @@ def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
 @pytest.mark.parametrize("args", ["--nx 16 --ny 16 --k 3", "--nx 5 --ny 3 --k 2", "--nx 4 --ny 4 --k 4",
                                   "--nx 3 --ny 3 --k 6", "--nx 64 --ny 64 --k 1"])
 @pytest.mark.parametrize("env", [("EG_SCALED_UPPER", "0"), ("EG_SLAB", "vector"), ("EG_FACTOR_GRAPH", "0"),
-                                 ("EG_SLAB_STAR", "0"), ("EG_RESIDENT_DF", "0"), ("EG_SWEEP_XCD", "0")])
+                                 ("EG_SLAB_STAR", "0"), ("EG_RESIDENT_DF", "0")])
 def test_ilu_variants_match_default(gpu, oracle_lib, args, env, monkeypatch):
     """The ILU(0) preconditioner with the factor's upper blocks unscaled (the
     reference's inv(U_ii) (w - sum U y) backward step), with the vector-FMA
@@ -354,9 +361,8 @@ def test_ilu_variants_match_default(gpu, oracle_lib, args, env, monkeypatch):
     without a graph, with the general slab kernel instead of the two-stage
     one of the cell stencil, and with the face factor per level instead of
     one dataflow launch (64x64 k = 1: more face rows than workgroups, so a
-    workgroup factors several rows), and with the dataflow sweeps' rows in
-    natural order instead of by XCD strips: each against the oracle (1e-10)
-    and against the default path (1e-12: other summation orders)."""
+    workgroup factors several rows): each against the oracle (1e-10) and
+    against the default path (1e-12: other summation orders)."""
     import torch
     out = {}
     for variant in ("default", "env"):
@@ -379,3 +385,43 @@ def test_ilu_variants_match_default(gpu, oracle_lib, args, env, monkeypatch):
         out[variant] = res
     for a, b in zip(out["default"], out["env"]):
         assert rel(a, b) < 1e-12
+
+
+@pytest.mark.parametrize("size", list(LARGE))
+def test_large_dataflow_matches_levels(gpu, size, monkeypatch):
+    """The single-launch dataflow sweeps (rows from a ticket counter, several
+    rows a workgroup at these sizes) against the one-launch-per-level
+    schedule at 64x64 Q4 and 128x128 Q6, both systems, FP64 and FP32: the
+    same preconditioned vector (same summation order; 1e-13 relative)."""
+    import torch
+    args = LARGE[size].replace(" --steps 1 --newton-steps 1", "") + " --precision mixed"
+    out = {}
+    for mode in ("levels", "dataflow"):
+        monkeypatch.setenv("EG_ILU_SCHEDULE", mode)
+        print(f"case: {size} {mode}", flush=True)
+        c = gpu.Context(args)
+        u = torch.tensor(c.initial(), device="cuda")
+        c.freeze(u, 1e-3)
+        c.set_stage(u, 0.01)
+        r = torch.zeros_like(u)
+        c.residual(u, r)
+        c.call("eg_hdg_condense", gpu._ptr(u), gpu._ptr(r))
+        c.call("eg_dg_refresh", gpu._ptr(u))
+        res = []
+        for form, size_ in ((1, c.nl), (0, c.nu)):
+            c.call("eg_pc_setup", form)
+            b = torch.sin(0.13 * (torch.arange(size_, device="cuda", dtype=torch.float64) + 1))
+            y = torch.zeros_like(b)
+            c.call("eg_pc_apply", form, gpu._ptr(b), gpu._ptr(y))
+            y32 = torch.zeros(size_, dtype=torch.float32, device="cuda")
+            c.call("eg_pc_apply_f32", form, gpu._ptr(b.float()), gpu._ptr(y32))
+            res += [y.cpu().numpy(), y32.double().cpu().numpy()]
+            assert np.all(np.isfinite(res[-2])) and np.all(np.isfinite(res[-1]))
+            assert rel(res[-1], res[-2]) < 1e-4, (mode, form)  # FP32 against FP64
+        out[mode] = res
+        del c
+    # FP64: the same summation order; FP32: the per-level kernels reduce a
+    # row's products in another order (single-precision rounding, carried
+    # through up to 255 dependency levels at 128x128)
+    for a, b, what in zip(out["levels"], out["dataflow"], ("hdg fp64", "hdg fp32", "dg fp64", "dg fp32")):
+        assert rel(b, a) < (1e-13 if "fp64" in what else 1e-4), what

@@ -300,13 +300,98 @@ def test_reference_integration_backward_euler(oracle_lib, tmp_path):
     assert float(comp["max_nodal_difference"]) < 1e-8 and float(comp["solution_L2_difference"]) < 1e-8
 
 
-def test_l2_error_band_and_conservation(oracle_lib):
-    """RESULTS.md:93-100,145: k=2, CFL 2, T=.2, N=32 gives L2(rho) ~1.005e-3
-    (published before the per-solve frozen-bound policy, so a band), drift
-    at roundoff, DG = HDG."""
-    args = "--nx 32 --ny 32 --k 2 --cfl 2 --T .2 --newton-tol 1e-13"
-    o = oracle_lib.Oracle(args)
-    u_dg, s_dg, *_ = o.run("dg")
-    assert .6e-3 < s_dg[5] < 1.6e-3
-    assert s_dg[4] < 1e-12
-    assert abs(s_dg[3] - .2) < 1e-15
+# --------------------------------------------------------------- anisotropic geometry and published values
+def iterations_csv(path):
+    import csv
+    return list(csv.DictReader(open(path)))
+
+
+@pytest.mark.parametrize("alpha", [30., 60.])
+def test_wave_rate_closed_form_anisotropic(oracle_lib, alpha):
+    """WaveRate = max HLL bound / min(hx, hy) (src/problem.cpp:377-384). For
+    the initial simple wave at gamma = 3 (v = 1 + j/2, c = (2 - j)/2,
+    j = sin(pi xi) in [-1, 1]) the largest bound is c + v n.d at j = -1:
+    1.1 (1.5 + cos(a)/2) on x-faces, 1.1 (1.5 + sin(a)/2) on y-faces. On a
+    12 x 8 mesh at 30 or 60 degrees hx != hy and the two face families
+    differ, so max(hx, hy), a swapped axis or a swapped angle all fail."""
+    a = math.radians(alpha)
+    o = oracle_lib.Oracle(f"--nx 12 --ny 8 --k 2 --alpha-deg {alpha}")
+    hx, hy, area, lx, ly, _ = o.geometry()
+    assert abs(lx - 2 / math.cos(a)) < 1e-14 and abs(ly - 2 / math.sin(a)) < 1e-14
+    assert abs(hx - lx / 12) < 1e-15 and abs(hy - ly / 8) < 1e-15 and abs(area - hx * hy) < 1e-15
+    rate = o.freeze(o.initial(), 1e-3)
+    want = 1.1 * (1.5 + max(math.cos(a), math.sin(a)) / 2) / min(hx, hy)
+    assert abs(rate - want) < 0.01 * want, (rate, want)
+    for wrong in (1.1 * (1.5 + max(math.cos(a), math.sin(a)) / 2) / max(hx, hy),
+                  1.1 * (1.5 + min(math.cos(a), math.sin(a)) / 2) / min(hx, hy)):
+        assert abs(rate - wrong) > 0.03 * wrong
+
+
+# RESULTS.md:93-100,124-128,141-145 of the reference: k = 2, SSP-DIRK3,
+# CFL 2, T = 0.2, Newton and GMRES tolerances 1e-13 (DG = HDG to the digits
+# shown). Published before the per-solve frozen-bound policy of the current
+# code (RESULTS.md:30-33), which moves the errors by about 0.1 % and the
+# Newton update counts (52 published for N = 32, 39 now), so the errors are
+# pinned to 0.5 %, the step counts and mean dt exactly.
+PUBLISHED_L2 = {32: [1.00539387e-3, 7.41333053e-4, 7.41333053e-4, 6.88933246e-4],
+                64: [1.51106794e-4, 1.12145056e-4, 1.12145056e-4, 9.26221620e-5]}
+PUBLISHED_STEPS = {32: (3, 0.0666666667), 64: (5, 0.04)}
+
+
+def published_case(n):
+    return f"--nx {n} --ny {n} --k 2 --cfl 2 --T .2 --newton-tol 1e-13 --linear-tol 1e-13"
+
+
+def test_published_l2_n32_both_forms(oracle_lib, tmp_path):
+    o = oracle_lib.Oracle(published_case(32))
+    metric, _ = o.metrics()
+    finals = {}
+    for form in ("dg", "hdg"):
+        u, s, *_ = o.run(form, prefix=str(tmp_path / f"n32_{form}"))
+        finals[form] = u
+        assert np.allclose(s[5:9], PUBLISHED_L2[32], rtol=5e-3, atol=0), (form, s[5:9])
+        steps, mean_dt = PUBLISHED_STEPS[32]
+        assert int(s[1]) == steps and abs(s[3] - .2) < 1e-15
+        row = iterations_csv(tmp_path / f"n32_{form}_summary.csv")[0]
+        assert int(row["time_steps"]) == steps and abs(float(row["mean_dt"]) - mean_dt) < 1e-9
+        assert abs(float(row["actual_max_cfl"]) - 2.0) < 1e-12
+        assert s[4] < 3e-13  # conservation drift, RESULTS.md:145 (1.2e-13)
+    d = finals["dg"] - finals["hdg"]
+    joint = math.sqrt(float(np.sum(metric * d * d)))
+    assert joint < 1e-13  # RESULTS.md:124-128 (7.8e-15)
+
+
+def test_published_l2_n64_dg(oracle_lib):
+    u, s, *_ = oracle_lib.Oracle(published_case(64)).run("dg")
+    assert np.allclose(s[5:9], PUBLISHED_L2[64], rtol=5e-3, atol=0), s[5:9]
+    assert int(s[1]) == PUBLISHED_STEPS[64][0] and abs(s[3] - .2) < 1e-15 and s[4] < 3e-13
+
+
+def test_convergence_at_30_degrees(oracle_lib):
+    """The alpha = 30 degree wave (lx != ly, hx != hy, the wave not x <-> y
+    symmetric) converges at the scheme's order: an x <-> y slip in Initial,
+    Errors or the geometry leaves O(0.1) errors that do not converge."""
+    errs = {}
+    for n in (16, 32):
+        u, s, *_ = oracle_lib.Oracle(f"--nx {n} --ny {n} --k 2 --alpha-deg 30 --cfl 1 --T .1").run("dg")
+        errs[n] = np.array(s[5:9])
+    rates = np.log2(errs[16] / errs[32])
+    assert np.all(rates > 2.3), rates
+    assert np.all(errs[16] < 1.2e-3) and np.all(errs[32] < 2e-4), errs
+
+
+# A workload whose first update of stage 2 takes the damped branch of the
+# line search (src/stepper.cpp:202-217): lambda = 1/2 is accepted.
+LINE_SEARCH_CASE = "--nx 16 --ny 16 --k 2 --cfl 3 --steps 1"
+
+
+def test_line_search_damping_branch(oracle_lib, tmp_path):
+    o = oracle_lib.Oracle(LINE_SEARCH_CASE)
+    finals = {}
+    for form in ("dg", "hdg"):
+        u, s, *_ = o.run(form, prefix=str(tmp_path / f"ls_{form}"))
+        finals[form] = u
+        damp = [float(r["damping"]) for r in iterations_csv(tmp_path / f"ls_{form}_iterations.csv")]
+        assert len(damp) == int(s[0]) and min(damp) == 0.5 and damp.count(0.5) >= 1, damp
+        assert all(d in (1.0, 0.5) for d in damp)
+    assert np.max(np.abs(finals["dg"] - finals["hdg"])) < 1e-7  # src/main.cpp:47

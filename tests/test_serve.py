@@ -58,8 +58,9 @@ def test_alone_equals_in_mix(gpu):
 def test_served_matches_single_context_run(gpu):
     args, form, _ = MIX[0]
     out, _, states = eg.serve([(args, form, 0.)], workers=2, keep_states=True)
-    # served solves run the ILU(0) one launch per dependency level
-    u, st, _, _ = gpu.Context(args + " --ilu-schedule levels").run(form)
+    # served solves run the same (default, single-launch dataflow) ILU(0)
+    # schedule as a solve alone on the device
+    u, st, _, _ = gpu.Context(args).run(form)
     assert np.array_equal(states[0], u) and out[0]["updates"] == st.updates
 
 
