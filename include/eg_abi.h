@@ -71,6 +71,17 @@ int eg_trial_residual(eg_ctx *ctx, const double *dev_u, const double *dev_du, do
                       double *dev_r, double *norm, int *admissible);
 /* Problem::CellNorm (src/problem.cpp:385-391). */
 int eg_cell_norm(eg_ctx *ctx, const double *dev_v, double *out);
+/* No counterpart in the reference: an allocation-free form of eg_cell_norm
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops). Writes sqrt(sum_i dev_metric[i] dev_v[i]^2)
+ * over n values (with the CellNorm weights area * w_i of the context's
+ * metric it is CellNorm, src/problem.cpp:385-391) to the device double
+ * dev_out, enqueued on stream (NULL: the null stream) with no context, no
+ * allocation and no host synchronisation. Its partial sums go to a fixed
+ * 4 KiB device array per GPU that every call on that GPU shares: calls that
+ * overlap on one GPU race. n must be in 1 .. 2147483647 (else an error,
+ * with the message in eg_last_error(NULL)). */
+int eg_cell_norm_noalloc(const double *dev_v, const double *dev_metric, long n, void *stream, double *dev_out);
 /* Problem::Admissible (src/problem.cpp:259-275). */
 int eg_admissible(eg_ctx *ctx, const double *dev_u, int *ok);
 

@@ -64,6 +64,7 @@ _SIGNATURES = {
     "eg_residual": (ctypes.c_int, [_vp, _vp, _vp, _c_double_p]),
     "eg_trial_residual": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, _vp, _vp, _c_double_p, _c_int_p]),
     "eg_cell_norm": (ctypes.c_int, [_vp, _vp, _c_double_p]),
+    "eg_cell_norm_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_long, _vp, _vp]),
     "eg_admissible": (ctypes.c_int, [_vp, _vp, _c_int_p]),
     "eg_dg_refresh": (ctypes.c_int, [_vp, _vp]),
     "eg_dg_apply": (ctypes.c_int, [_vp, _vp, _vp]),

@@ -117,6 +117,11 @@ int eg_trial_residual(eg_ctx *ctx, const double *dev_u, const double *dev_du, do
 int eg_cell_norm(eg_ctx *ctx, const double *dev_v, double *out) {
   return guard(ctx, [&] { *out = ctx->problem->CellNorm(dev_v); });
 }
+int eg_cell_norm_noalloc(const double *dev_v, const double *dev_metric, long n, void *stream, double *dev_out) {
+  return guard(nullptr, [&] {
+    eg::launch_cell_norm_noalloc(dev_v, dev_metric, n, dev_out, static_cast<hipStream_t>(stream));
+  });
+}
 int eg_admissible(eg_ctx *ctx, const double *dev_u, int *ok) {
   return guard(ctx, [&] { *ok = ctx->problem->Admissible(dev_u) ? 1 : 0; });
 }

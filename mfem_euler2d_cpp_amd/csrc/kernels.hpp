@@ -129,6 +129,12 @@ bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const 
                           const int *upd_ptr, const int *upd, int *done, int epoch, const int *order, int *ticket,
                           int *flags, hipStream_t s);
 
+// Allocation-free weighted norm sqrt(sum metric v^2) into the device double
+// out: its partial sums go to a fixed per-device array of kReduceBlocks
+// doubles (no allocation, no host synchronisation); n in 1 .. kNoallocMaxValues.
+constexpr long kNoallocMaxValues = 2147483647L;
+void launch_cell_norm_noalloc(const double *v, const double *metric, long n, double *out, hipStream_t s);
+
 // Vector kernels (Krylov). nb = kReduceBlocks partials per reduction.
 struct Coef {
   double c[64];

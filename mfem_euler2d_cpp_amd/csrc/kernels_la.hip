@@ -4,6 +4,8 @@
 // weighted restarted GMRES (fused CGS2 passes with deterministic two-stage
 // reductions).
 #include <algorithm>
+#include <mutex>
+#include <stdexcept>
 
 #include "kernels.hpp"
 
@@ -342,8 +344,9 @@ __global__ __launch_bounds__(256) void k_dot_partials(const T *__restrict__ a, c
   if (threadIdx.x == 0) { partials[blockIdx.x] = s; }
 }
 
-// mode 0: sum, mode 1: max.
-// One block per row of count partials: out[row] = sum (mode 0) or max (mode 1).
+// mode 0: sum, mode 1: max, mode 2: square root of the sum.
+// One block per row of count partials: out[row] = sum (mode 0), max (mode 1)
+// or sqrt(sum) (mode 2).
 // host_out (optional, pinned host memory): the sums are also written there,
 // and host_flags[0] = flags[0], so that one stream synchronisation brings
 // both to the host without copy launches.
@@ -356,9 +359,9 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
   if (host_flags && blockIdx.x == 0 && threadIdx.x == 0) { host_flags[0] = flags[0]; }
   double acc = 0.;
   for (int k = threadIdx.x; k < count; k += blockDim.x) {
-    acc = mode ? fmax(acc, partials[k]) : acc + partials[k];
+    acc = mode == 1 ? fmax(acc, partials[k]) : acc + partials[k];
   }
-  if (mode) {
+  if (mode == 1) {
     for (int off = 32; off > 0; off >>= 1) { acc = fmax(acc, __shfl_down(acc, off, 64)); }
     if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6] = acc; }
     __syncthreads();
@@ -371,8 +374,9 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double *__restrict__
   } else {
     const double s = block_sum(acc, red);
     if (threadIdx.x == 0) {
-      out[0] = s;
-      if (host_out) { host_out[blockIdx.x] = s; }
+      const double v = mode == 2 ? sqrt(s) : s;
+      out[0] = v;
+      if (host_out) { host_out[blockIdx.x] = v; }
     }
   }
 }
@@ -552,6 +556,57 @@ void launch_dirk_combine(double *out, const double *base, const double *const *s
 }
 void launch_slope(const double *u, const double *base, double inv_dt, double *slope, long n, hipStream_t s) {
   k_slope<<<grid_for(n), 256, 0, s>>>(u, base, inv_dt, slope, n);
+}
+
+// ------------------------------------------------------------ no allocation
+// Scratch of the allocation-free norm (eg_cell_norm_noalloc): the first-stage
+// partial sums of one reduction, at most kReduceBlocks of them (4 KiB), one
+// copy per device, held until the process ends. Every call on a device shares
+// its copy, so calls that overlap on one device race; each call writes every
+// partial it then reads, so nothing has to be cleared between calls.
+static __device__ __align__(256) double scratch[kReduceBlocks];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex scratch_mutex;
+double *scratch_address[kScratchDevices] = {};
+// Device address of the current device's copy of scratch, resolved once per
+// device with that device current (host code never passes the array's name,
+// which is only a host-side stand-in, to a kernel or a copy).
+double *device_scratch() {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kScratchDevices) {
+    (void)hipGetLastError();
+    throw std::runtime_error("eg_cell_norm_noalloc: no usable current device");
+  }
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (scratch_address[dev] == nullptr) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+      (void)hipGetLastError();
+      throw std::runtime_error(std::string("eg_cell_norm_noalloc: hipGetSymbolAddress failed: ") +
+                               hipGetErrorString(e));
+    }
+    scratch_address[dev] = static_cast<double *>(ptr);
+  }
+  return scratch_address[dev];
+}
+} // namespace
+
+void launch_cell_norm_noalloc(const double *v, const double *metric, long n, double *out, hipStream_t s) {
+  if (n < 1 || n > kNoallocMaxValues) {
+    throw std::runtime_error("eg_cell_norm_noalloc: n must be in 1 .. 2147483647");
+  }
+  if (v == nullptr || metric == nullptr || out == nullptr) {
+    throw std::runtime_error("eg_cell_norm_noalloc: null device pointer");
+  }
+  double *partials = device_scratch();
+  const int count = reduce_blocks(n); // at most kReduceBlocks, the size of scratch
+  k_dot_partials<double><<<count, 256, 0, s>>>(v, v, metric, n, partials);
+  k_sum_partials<<<1, 256, 0, s>>>(partials, count, out, 2, nullptr, nullptr, nullptr);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { throw std::runtime_error(std::string("eg_cell_norm_noalloc: ") + hipGetErrorString(e)); }
 }
 
 #define EG_INST_T(T)                                                                                          \
