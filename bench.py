@@ -212,8 +212,8 @@ def run_served(count, workers, load=0.8, seed=0x5E12ED):
 
 
 MIXED_CASES = {
-    "64x64 Q4": "--nx 64 --ny 64 --k 4 --alpha-deg 30 --cfl 1 --steps 1 --newton-steps 1 --restart 30 --precond block-ilu0",
-    "128x128 Q6": "--nx 128 --ny 128 --k 6 --alpha-deg 30 --cfl 1 --steps 1 --newton-steps 1 --restart 30 "
+    "64x64 Q4": "--nx 64 --ny 64 --k 4 --alpha-deg 30 --cfl 1 --steps 3 --newton-steps 3 --restart 30 --precond block-ilu0",
+    "128x128 Q6": "--nx 128 --ny 128 --k 6 --alpha-deg 30 --cfl 1 --steps 3 --newton-steps 3 --restart 30 "
                   "--precond block-ilu0",
 }
 
@@ -232,7 +232,7 @@ def run_mixed_sizes():
             runs = {}
             for prec in ("fp64", "mixed"):
                 c = eg.Context(args + f" --precision {prec}")
-                u, st, inc, kr = c.run(form, increments=4)
+                u, st, inc, kr = c.run(form, increments=36)
                 runs[prec] = (st.newton_seconds / st.updates, inc[:st.updates], list(kr[:st.updates]))
                 del c
             d = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(runs["mixed"][1], runs["fp64"][1])]

@@ -1574,9 +1574,13 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
   // Rows from the ticket counter (release_tickets): ticket t is row order[t],
   // the rows of the sweep in its dependency-level order (a natural-order
   // window of tickets would hold only a few rows of one level).
-  for (int i = next_row; i >= 0;) {
-    int ahead = 0, ahead_row = -1;
-    if (tid == 0) { ahead = atomicAdd(ticket, 1); }
+  // the row index is the same in every lane: read it into a scalar register
+  // (the row's pattern and block addresses then stay scalar)
+  for (int i = __builtin_amdgcn_readfirstlane(next_row); i >= 0;) {
+    // With a workgroup for every row the first tickets cover all rows, so no
+    // workgroup takes a second one (and none delays its row for it).
+    int ahead = nb, ahead_row = -1;
+    if (tid == 0 && int(gridDim.x) < nb) { ahead = atomicAdd(ticket, 1); }
     const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
     const int nd = j1 - j0;
     T a[KB][VB];
@@ -1680,7 +1684,7 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
     // after the dependency wait
     if (tid == 0) { next_row = ahead_row; }
     __syncthreads();
-    i = next_row;
+    i = __builtin_amdgcn_readfirstlane(next_row);
   }
   release_tickets(ticket);
 }
@@ -1721,7 +1725,9 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_stream(const T *__restrict__
                                                            int *flags, int spin_limit,
                                                            const int *__restrict__ order, int *ticket) {
   constexpr int NT = 1024, PRE = 4, KDM = 4; // cell blocks: at most 4 neighbours a triangle
-  extern __shared__ unsigned char stream_lds[];
+  // 16-byte aligned: the dynamic LDS follows the static next_row, and its
+  // doubles must stay 8-byte aligned for ds_read_b64
+  extern __shared__ __attribute__((aligned(16))) unsigned char stream_lds[];
   __shared__ int next_row;
   T *xs = reinterpret_cast<T *>(stream_lds); // kd x b dependency values
   T *red = xs + size_t(kd) * b;               // NT partial sums
@@ -1740,9 +1746,13 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_stream(const T *__restrict__
   }
   __syncthreads();
   // rows from the ticket counter in level order, as in k_ilu_sweep_sn
-  for (int i = next_row; i >= 0;) {
-    int ahead = 0, ahead_row = -1;
-    if (tid == 0) { ahead = atomicAdd(ticket, 1); }
+  // the row index is the same in every lane: read it into a scalar register
+  // (the row's pattern and block addresses then stay scalar)
+  for (int i = __builtin_amdgcn_readfirstlane(next_row); i >= 0;) {
+    // With a workgroup for every row the first tickets cover all rows, so no
+    // workgroup takes a second one (and none delays its row for it).
+    int ahead = nb, ahead_row = -1;
+    if (tid == 0 && int(gridDim.x) < nb) { ahead = atomicAdd(ticket, 1); }
     const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
     const int nd = j1 - j0;
     T pre[KDM][PRE];
@@ -1820,7 +1830,7 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_stream(const T *__restrict__
     }
     if (tid == 0) { next_row = ahead_row; } // read by every thread before the barrier after the wait
     __syncthreads();
-    i = next_row;
+    i = __builtin_amdgcn_readfirstlane(next_row);
   }
   release_tickets(ticket);
 }
@@ -1845,7 +1855,7 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_df(double *bsr, int
       row = t < nb ? order[t] : -1;
     }
     __syncthreads();
-    const int i = row;
+    const int i = __builtin_amdgcn_readfirstlane(row);
     if (i < 0) { break; }
     wait_rows(col, row_ptr[i], diag_ptr[i], done, epoch, spin_limit, flags);
     if (factor_row<B, true>(bsr, i, row_ptr, diag_ptr, col, upd_ptr, upd, sm) && threadIdx.x == 0) {
@@ -1921,7 +1931,7 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident_df(double 
     __syncthreads(); // the previous row's LDS image and index are no longer read
     if (t == 0) { row = atomicAdd(ticket, 1); }
     __syncthreads();
-    const int r = row;
+    const int r = __builtin_amdgcn_readfirstlane(row); // uniform: keep it scalar
     if (r >= nrows) { break; }
     const int *pg = prog + prog_off[r];
     for (int e = t; e < kIluProgMax; e += NT) { P[e] = pg[e]; }
