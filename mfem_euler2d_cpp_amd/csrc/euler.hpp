@@ -304,6 +304,9 @@ struct RunResult {
   std::vector<std::vector<double>> increments;
 };
 RunResult Run(Problem &p, const std::string &form, const std::string &prefix, bool keep_increments = false);
+// include/euler.hpp:257: the --verify self-test (src/verification.cpp:7-169),
+// run on the device kernels; throws on a failed check.
+void Verify(const Options &base, const EulerGas &gas);
 
 // ------------------------------------------------------------ many solves
 // Batched and served execution of independent solves (no counterpart in the

@@ -37,6 +37,8 @@ void launch_recover(const Geo &g, const double *Xy, const double *dl, double *du
 void launch_recovered_residual(const Geo &g, const Frozen *fz, const double *D, const double *eps, double *out,
                                double *partial, hipStream_t s);
 void launch_hdg_diag_blocks(const Geo &g, const double *panels, const double *D, double *blocks, hipStream_t s);
+// Pressure, fluxes and flux Jacobians of count states (41 doubles each).
+void launch_physics_probe(const double *states, int count, double gamma, double *out, hipStream_t s);
 void launch_hdg_fill(const Geo &g, const double *panels, const double *D, const int *contrib, const int *row_ptr,
                      const int *diag_ptr, double *bsr, hipStream_t s);
 

@@ -67,6 +67,23 @@ __global__ __launch_bounds__(256) void k_freeze(Geo g, const double *__restrict_
   }
 }
 
+// Device physics at given states: pressure, the x and y fluxes and their
+// Jacobians (column-major), 41 values a state (the --verify known answers).
+__global__ __launch_bounds__(64) void k_physics_probe(const double *__restrict__ states, int count, double gamma,
+                                                      double *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) { return; }
+  double u[4];
+  for (int c = 0; c < 4; ++c) { u[c] = states[4 * i + c]; }
+  const double gm1 = gamma - 1.;
+  double *o = out + size_t(i) * 41;
+  o[0] = pressure(u, gm1);
+  flux(u, 0, gm1, o + 1);
+  flux(u, 1, gm1, o + 5);
+  jacobian(u, 0, gm1, o + 9);
+  jacobian(u, 1, gm1, o + 25);
+}
+
 // State value at a node: u + lambda*du (du may be null).
 __device__ __forceinline__ void load_state(const double *u, const double *du, double lambda, size_t off,
                                            double s[4]) {
@@ -741,6 +758,9 @@ void launch_recovered_residual(const Geo &g, const Frozen *fz, const double *D, 
                                double *partial, hipStream_t s) {
   const int nb = recovered_blocks(g);
   EG_DISPATCH_Q(g.q, (k_recovered_residual<Q><<<nb, 256, 0, s>>>(g, fz, D, eps, out, partial)));
+}
+void launch_physics_probe(const double *states, int count, double gamma, double *out, hipStream_t s) {
+  k_physics_probe<<<(count + 63) / 64, 64, 0, s>>>(states, count, gamma, out);
 }
 void launch_hdg_diag_blocks(const Geo &g, const double *panels, const double *D, double *blocks, hipStream_t s) {
   EG_DISPATCH_Q(g.q, (k_hdg_diag_blocks<Q, double><<<g.nf, 256, 0, s>>>(g, panels, D, blocks)));

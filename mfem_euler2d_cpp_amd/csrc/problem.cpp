@@ -439,26 +439,47 @@ std::array<double, 4> Problem::Differences(const std::vector<double> &u, const s
 // mesh and L2 GL grid functions (element-wise lexicographic nodal values).
 void Problem::Save(const std::vector<double> &u, const std::string &prefix) const {
   {
+    // The reference prints its mesh (src/problem.cpp:763-764): the
+    // non-periodic nx x ny Cartesian quadrilateral mesh in lexicographic
+    // element order, shifted to the box [-lx/2, lx/2] x [-ly/2, ly/2]
+    // (src/problem.cpp:37-38,47-51), in the MFEM mesh v1.0 text format:
+    // elements "attribute geometry vertices" (attribute 1, square = 3), the
+    // boundary segments bottom (attribute 1), top (3), left (4), right (2),
+    // and the vertices x fastest, at the stream's default precision.
     std::ofstream ms(prefix + ".mesh");
-    ms << "MFEM mesh v1.0\n\ndimension\n2\n\nelements\n" << n << '\n';
-    const int nvx = opt.nx + 1;
-    for (int iy = 0; iy < opt.ny; ++iy) {
-      for (int ix = 0; ix < opt.nx; ++ix) {
+    require(bool(ms), "Cannot open mesh output");
+    const int nx = opt.nx, ny = opt.ny, nvx = nx + 1;
+    ms << "MFEM mesh v1.0\n\n#\n# MFEM Geometry Types (see fem/geom.hpp):\n#\n# POINT       = 0\n"
+          "# SEGMENT     = 1\n# TRIANGLE    = 2\n# SQUARE      = 3\n# TETRAHEDRON = 4\n# CUBE        = 5\n"
+          "# PRISM       = 6\n# PYRAMID     = 7\n#\n\ndimension\n2\n\nelements\n"
+       << n << '\n';
+    for (int iy = 0; iy < ny; ++iy) {
+      for (int ix = 0; ix < nx; ++ix) {
         const int v0 = iy * nvx + ix;
         ms << "1 3 " << v0 << ' ' << v0 + 1 << ' ' << v0 + 1 + nvx << ' ' << v0 + nvx << '\n';
       }
     }
-    ms << "\nboundary\n0\n\nvertices\n" << (opt.nx + 1) * (opt.ny + 1) << "\n2\n" << std::setprecision(17);
-    for (int iy = 0; iy <= opt.ny; ++iy) {
-      for (int ix = 0; ix <= opt.nx; ++ix) {
-        ms << double(ix) / opt.nx * lx - .5 * lx << ' ' << double(iy) / opt.ny * ly - .5 * ly << '\n';
+    ms << "\nboundary\n" << 2 * (nx + ny) << '\n';
+    const int top = nvx * ny;
+    for (int ix = 0; ix < nx; ++ix) { ms << "1 1 " << ix << ' ' << ix + 1 << '\n'; }
+    for (int ix = 0; ix < nx; ++ix) { ms << "3 1 " << top + ix + 1 << ' ' << top + ix << '\n'; }
+    for (int iy = 0; iy < ny; ++iy) { ms << "4 1 " << (iy + 1) * nvx << ' ' << iy * nvx << '\n'; }
+    for (int iy = 0; iy < ny; ++iy) { ms << "2 1 " << iy * nvx + nx << ' ' << (iy + 1) * nvx + nx << '\n'; }
+    ms << "\nvertices\n" << nvx * (ny + 1) << "\n2\n";
+    for (int iy = 0; iy <= ny; ++iy) {
+      for (int ix = 0; ix <= nx; ++ix) {
+        ms << double(ix) / nx * lx - .5 * lx << ' ' << double(iy) / ny * ly - .5 * ly << '\n';
       }
     }
   }
+  // One GridFunction file per component (src/problem.cpp:765-777): the
+  // L2 Gauss-Legendre space's header, then the values in element-DOF order
+  // (element e, native node i).
   const char *names[] = {"rho", "rhou", "rhov", "energy"};
   for (int c = 0; c < 4; ++c) {
     std::ofstream fs(prefix + "_" + names[c] + ".gf");
-    fs << "FiniteElementSpace\nFiniteElementCollection: L2_T1_2D_P" << opt.k << "\nVDim: 1\nOrdering: 0\n\n"
+    require(bool(fs), "Cannot open solution output");
+    fs << "FiniteElementSpace\nFiniteElementCollection: L2_2D_P" << opt.k << "\nVDim: 1\nOrdering: 0\n\n"
        << std::setprecision(17);
     for (int e = 0; e < n; ++e) {
       for (int i = 0; i < p; ++i) { fs << u[size_t(e) * m + 4 * i + c] << '\n'; }

@@ -120,3 +120,76 @@ def test_mixed_precision_vs_oracle(gpu, oracle_lib, form):
         ok, e = increment_ok(o, i32[i], io[i], u_scale, bound=1e-8)
         assert ok, (form, i, e)
     assert rel(u32, uo) < 1e-9
+
+
+def test_save_outputs(gpu, tmp_path):
+    """Save (src/problem.cpp:762-790): _coefficients.csv holds the final state
+    (element, node, four components at 17 digits), each component's .gf
+    holds the same values in element-DOF order after the L2 Gauss-Legendre
+    space header, and the .mesh is the non-periodic Cartesian quadrilateral
+    mesh of the shifted box: its vertices span [-lx/2, lx/2] x [-ly/2, ly/2]
+    and each boundary attribute's segments lie on their side of the box."""
+    import csv
+    args = "--nx 6 --ny 4 --k 2 --alpha-deg 30 --dt 1e-3 --steps 1 --newton-steps 1"
+    c = gpu.Context(args)
+    prefix = str(tmp_path / "save")
+    u, st, *_ = c.run("hdg", prefix=prefix)
+    n, p, m = c.n, c.p, c.m
+    rows = list(csv.DictReader(open(prefix + "_coefficients.csv")))
+    assert len(rows) == n * p
+    for r in rows:
+        e, i = int(r["element"]), int(r["node"])
+        got = [float(r[k]) for k in ("rho", "rhou", "rhov", "energy")]
+        assert got == [u[e * m + 4 * i + cc] for cc in range(4)]
+    for cc, name in enumerate(("rho", "rhou", "rhov", "energy")):
+        lines = open(f"{prefix}_{name}.gf").read().split("\n")
+        assert lines[:4] == ["FiniteElementSpace", "FiniteElementCollection: L2_2D_P2", "VDim: 1", "Ordering: 0"]
+        vals = [float(v) for v in lines[5:] if v.strip()]
+        assert vals == [u[e * m + 4 * i + cc] for e in range(n) for i in range(p)]
+    hx, hy, area, lx, ly, _ = c.geometry()
+    text = open(prefix + ".mesh").read().split("\n")
+    assert text[0] == "MFEM mesh v1.0"
+    sec = {name: text.index(name) for name in ("dimension", "elements", "boundary", "vertices")}
+    assert text[sec["dimension"] + 1] == "2"
+    ne = int(text[sec["elements"] + 1])
+    elems = [list(map(int, text[sec["elements"] + 2 + j].split())) for j in range(ne)]
+    nb = int(text[sec["boundary"] + 1])
+    bdr = [list(map(int, text[sec["boundary"] + 2 + j].split())) for j in range(nb)]
+    nv = int(text[sec["vertices"] + 1])
+    assert text[sec["vertices"] + 2] == "2"
+    xy = np.array([list(map(float, text[sec["vertices"] + 3 + j].split())) for j in range(nv)])
+    assert ne == n and nb == 2 * (6 + 4) and nv == 7 * 5
+    assert np.allclose(xy.min(0), [-lx / 2, -ly / 2], rtol=1e-5) and np.allclose(xy.max(0), [lx / 2, ly / 2], rtol=1e-5)
+    for e, el in enumerate(elems):  # lexicographic cells of size hx x hy, counter-clockwise
+        assert el[:2] == [1, 3]
+        v = xy[el[2:]]
+        ix, iy = e % 6, e // 6
+        assert np.allclose(v[0], [-lx / 2 + ix * hx, -ly / 2 + iy * hy], rtol=1e-5, atol=1e-5)
+        assert np.allclose(v[2] - v[0], [hx, hy], rtol=1e-4)
+    side = {1: (1, -ly / 2), 3: (1, ly / 2), 4: (0, -lx / 2), 2: (0, lx / 2)}
+    for attr, geom, a, b in bdr:
+        axis, value = side[attr]
+        assert geom == 1 and abs(xy[a, axis] - value) < 1e-5 and abs(xy[b, axis] - value) < 1e-5
+
+
+def test_cli_verify_self_test(gpu, tmp_path):
+    """euler2d --verify (src/verification.cpp:7-169 on the device kernels):
+    exit 0, one 'verified k=' line a degree, and the reference's CSV columns
+    with every check within the reference's thresholds."""
+    import csv
+    import subprocess
+    out = tmp_path / "verify"
+    r = subprocess.run([str(gpu.CLI), "--verify", "--output", str(out)], capture_output=True, text=True, timeout=600)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert [line.split()[1] for line in r.stdout.splitlines() if line.startswith("verified k=")] == \
+        [f"k={k}" for k in range(9)]
+    rows = list(csv.DictReader(open(f"{out}_verification.csv")))
+    assert [int(x["k"]) for x in rows] == list(range(9))
+    for x in rows:
+        assert float(x["jacobian_fd_relative"]) < 1e-7 and float(x["dg_hdg_increment_relative"]) < 1e-8
+        assert float(x["assembled_mf_relative"]) < 1e-13 and float(x["mass_off_diagonal"]) < 1e-12
+        assert float(x["partial_schur_relative"]) < 1e-12
+    for x in csv.DictReader(open(f"{out}_storage.csv")):
+        assert float(x["dg_bsr_ilu_relative"]) < 1e-10 and float(x["hdg_bsr_ilu_relative"]) < 1e-10
+        assert float(x["streamed_schur_relative"]) < 1e-12 and float(x["schur_diagonal_relative"]) < 1e-12
