@@ -28,6 +28,21 @@ struct Geo {
   double e0[kMaxQ], e1[kMaxQ], w1[kMaxQ];
 };
 
+// Tensor Gauss-Legendre rule of the initial projection and the error
+// integrals (order max(24, 2k+8): nq = 13 points a direction for k <= 8) on
+// [0,1], with the 1-D basis values b[s*q + i] = l_i(x_s) at its points.
+constexpr int kMaxQuad = 13;
+struct QuadRule {
+  int nq;
+  double x[kMaxQuad], w[kMaxQuad];
+  double b[kMaxQuad * kMaxQ];
+};
+// Cell origins and the wave of the exact solution (angle, its cosine and sine).
+struct WaveGeo {
+  const double *origx, *origy;
+  double alpha, ca, sa;
+};
+
 // Frozen HLL data of one face point, fixed for one implicit stage solve.
 struct Frozen {
   double a, b, inv_sum, wl, wr, pen, pad0, pad1;

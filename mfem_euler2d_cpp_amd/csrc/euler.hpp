@@ -147,6 +147,9 @@ public:
   hipStream_t stream = nullptr;
   bool own_stream = false;
   eg::Geo geo{};
+  eg::QuadRule rule{};  // the projection / error rule (src/problem.cpp:329-349,704-726)
+  eg::WaveGeo wave{};   // cell origins (device) and the wave angle
+  DeviceArray<double> d_origx, d_origy, moments;
   DeviceArray<eg::Frozen> frozen;
   DeviceArray<double> old, d_cell_metric, d_trace_metric, partial, scalars;
   DeviceArray<float> d_cell_metric32, d_trace_metric32;
@@ -160,7 +163,7 @@ public:
   Problem(const Options &o, const EulerGas &gas, hipStream_t s = nullptr);
   ~Problem();
   Problem(const Problem &) = delete;
-  std::vector<double> Initial() const;
+  std::vector<double> Initial();
   void Freeze(const double *dev_u, double step, bool check_admissible = false);
   double WaveRate() const { return rate; }
   void SetStage(const double *dev_old, double step);
@@ -170,8 +173,13 @@ public:
   double TrialResidual(const double *dev_u, const double *dev_du, double lambda, double *dev_trial,
                        double *dev_r, bool &admissible);
   std::vector<std::vector<int>> SolverGraph(bool hdg) const;
-  std::array<double, 4> Errors(const std::vector<double> &u, double time) const;
-  std::array<double, 4> Integral(const std::vector<double> &u) const;
+  // Initial, Errors and Integral run on the device: the *Device forms on a
+  // device body vector, the others on a host vector (copied through).
+  void InitialDevice(double *dev_u);
+  std::array<double, 4> ErrorsDevice(const double *dev_u, double time);
+  std::array<double, 4> IntegralDevice(const double *dev_u);
+  std::array<double, 4> Errors(const std::vector<double> &u, double time);
+  std::array<double, 4> Integral(const std::vector<double> &u);
   std::array<double, 4> Differences(const std::vector<double> &u, const std::vector<double> &v) const;
   void Save(const std::vector<double> &u, const std::string &prefix) const;
   // scalar plumbing
@@ -303,7 +311,10 @@ struct RunResult {
   std::vector<UpdateRecord> records;
   std::vector<std::vector<double>> increments;
 };
-RunResult Run(Problem &p, const std::string &form, const std::string &prefix, bool keep_increments = false);
+// keep_state false: the final state stays on the device (RunResult::u is
+// left empty unless the solution is saved).
+RunResult Run(Problem &p, const std::string &form, const std::string &prefix, bool keep_increments = false,
+              bool keep_state = true);
 // include/euler.hpp:257: the --verify self-test (src/verification.cpp:7-169),
 // run on the device kernels; throws on a failed check.
 void Verify(const Options &base, const EulerGas &gas);

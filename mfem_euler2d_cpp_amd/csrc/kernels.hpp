@@ -37,6 +37,12 @@ void launch_recover(const Geo &g, const double *Xy, const double *dl, double *du
 void launch_recovered_residual(const Geo &g, const Frozen *fz, const double *D, const double *eps, double *out,
                                double *partial, hipStream_t s);
 void launch_hdg_diag_blocks(const Geo &g, const double *panels, const double *D, double *blocks, hipStream_t s);
+// Initial projection of the exact wave into the body vector u (device).
+void launch_initial(const Geo &g, const QuadRule &r, const WaveGeo &w, double *u, hipStream_t s);
+// Per-cell conserved integrals (mode 0) or squared L2 errors at `time`
+// (mode 1) of u, partials[c * n + e] (4 n doubles).
+void launch_cell_moments(const Geo &g, const QuadRule &r, const WaveGeo &w, const double *u, double time, int mode,
+                         double *partials, hipStream_t s);
 // Pressure, fluxes and flux Jacobians of count states (41 doubles each).
 void launch_physics_probe(const double *states, int count, double gamma, double *out, hipStream_t s);
 void launch_hdg_fill(const Geo &g, const double *panels, const double *D, const int *contrib, const int *row_ptr,

@@ -67,6 +67,97 @@ __global__ __launch_bounds__(256) void k_freeze(Geo g, const double *__restrict_
   }
 }
 
+// Rotated simple wave (src/physics.cpp:64-93) at (x, y, time): Riemann
+// invariant J+ = 2, J- = sin(pi xi) carried along xi + t((3-g)/2 + (g+1)/4
+// sin(pi xi)) = z mod 2; the characteristic foot by Newton safeguarded by
+// bisection (at most 80 steps, |residual| < 2e-15), as the host EulerGas.
+__device__ void exact_state(double x, double y, double time, const WaveGeo &w, double gamma, double out[4]) {
+  constexpr double pi = 3.141592653589793238462643383279502884;
+  const double gm1 = gamma - 1., drift = (3. - gamma) / 2., amp = (gamma + 1.) / 4.;
+  auto wrap = [](double z) { return z - 2 * floor((z + 1) / 2); };
+  const double phase = wrap(wrap(w.ca * x + w.sa * y) - time * drift);
+  const double ta = time * amp;
+  double lo = phase - ta, hi = phase + ta, z = phase;
+  for (int it = 0; it < 80; ++it) {
+    const double g = z + ta * sin(pi * z) - phase;
+    if (fabs(g) < 2e-15) { break; }
+    if (g > 0) { hi = z; } else { lo = z; }
+    const double step = z - g / (1 + ta * pi * cos(pi * z));
+    z = (step >= lo && step <= hi) ? step : .5 * (lo + hi);
+  }
+  const double jm = sin(pi * z), v = 1 + .5 * jm;
+  const double c = gm1 * (2. - jm) / 4.;
+  const double rho = pow(c * c / gamma, 1. / gm1);
+  const double pr = pow(rho, gamma);
+  out[0] = rho;
+  out[1] = rho * v * w.ca;
+  out[2] = rho * v * w.sa;
+  out[3] = pr / gm1 + .5 * rho * v * v;
+}
+
+// Initial (src/problem.cpp:329-349): L2 projection of the exact wave at
+// time 0 with the tensor rule r, divided by the diagonal GL mass. One
+// workgroup a cell: the exact state at every rule point into LDS, then one
+// thread a (node, component) sums over the points (rows, then columns).
+template <int Q>
+__global__ __launch_bounds__(256) void k_initial(Geo g, QuadRule r, WaveGeo w, double *__restrict__ u) {
+  constexpr int P = Q * Q;
+  __shared__ double vals[kMaxQuad * kMaxQuad * 4];
+  const int e = blockIdx.x, nq = r.nq;
+  for (int s = threadIdx.x; s < nq * nq; s += blockDim.x) {
+    const int qx = s % nq, qy = s / nq;
+    exact_state(w.origx[e] + g.hx * r.x[qx], w.origy[e] + g.hy * r.x[qy], 0., w, g.gamma, vals + 4 * s);
+  }
+  __syncthreads();
+  for (int id = threadIdx.x; id < 4 * P; id += blockDim.x) {
+    const int i = id / 4, c = id % 4;
+    double acc = 0.;
+    for (int qy = 0; qy < nq; ++qy) {
+      for (int qx = 0; qx < nq; ++qx) {
+        const double wgt = r.w[qx] * r.w[qy];
+        const double sh = r.b[qx * Q + i % Q] * r.b[qy * Q + i / Q];
+        acc += wgt * sh * vals[4 * (qy * nq + qx) + c];
+      }
+    }
+    u[size_t(e) * 4 * P + id] = acc * (1. / (g.w1[i % Q] * g.w1[i / Q]));
+  }
+}
+
+// Per-cell moments of a body vector, partials[c * n + e] for component c:
+// mode 0: the conserved integral sum_i area w_ix w_iy u (Integral,
+// src/problem.cpp:727-738); mode 1: the squared L2 error against the exact
+// wave at `time` with the tensor rule r (Errors, src/problem.cpp:704-726).
+template <int Q>
+__global__ __launch_bounds__(256) void k_cell_moments(Geo g, QuadRule r, WaveGeo w, const double *__restrict__ u,
+                                                      double time, int mode, double *__restrict__ partials) {
+  constexpr int P = Q * Q;
+  __shared__ double red[16 * 4];
+  const int e = blockIdx.x;
+  const double *cell = u + size_t(e) * 4 * P;
+  double acc[4] = {0., 0., 0., 0.};
+  if (mode == 0) {
+    for (int i = threadIdx.x; i < P; i += blockDim.x) {
+      const double wt = g.area * (g.w1[i % Q] * g.w1[i / Q]);
+      for (int c = 0; c < 4; ++c) { acc[c] += wt * cell[4 * i + c]; }
+    }
+  } else {
+    const int nq = r.nq;
+    for (int s = threadIdx.x; s < nq * nq; s += blockDim.x) {
+      const int qx = s % nq, qy = s / nq;
+      double v[4];
+      exact_state(w.origx[e] + g.hx * r.x[qx], w.origy[e] + g.hy * r.x[qy], time, w, g.gamma, v);
+      double z[4] = {0., 0., 0., 0.};
+      for (int i = 0; i < P; ++i) {
+        const double sh = r.b[qx * Q + i % Q] * r.b[qy * Q + i / Q];
+        for (int c = 0; c < 4; ++c) { z[c] += sh * cell[4 * i + c]; }
+      }
+      for (int c = 0; c < 4; ++c) { acc[c] += g.area * r.w[qx] * r.w[qy] * (z[c] - v[c]) * (z[c] - v[c]); }
+    }
+  }
+  const double s4 = block_sum_k<4>(acc, red);
+  if (threadIdx.x < 4) { partials[size_t(threadIdx.x) * g.n + e] = s4; }
+}
+
 // Device physics at given states: pressure, the x and y fluxes and their
 // Jacobians (column-major), 41 values a state (the --verify known answers).
 __global__ __launch_bounds__(64) void k_physics_probe(const double *__restrict__ states, int count, double gamma,
@@ -758,6 +849,13 @@ void launch_recovered_residual(const Geo &g, const Frozen *fz, const double *D, 
                                double *partial, hipStream_t s) {
   const int nb = recovered_blocks(g);
   EG_DISPATCH_Q(g.q, (k_recovered_residual<Q><<<nb, 256, 0, s>>>(g, fz, D, eps, out, partial)));
+}
+void launch_initial(const Geo &g, const QuadRule &r, const WaveGeo &w, double *u, hipStream_t s) {
+  EG_DISPATCH_Q(g.q, (k_initial<Q><<<g.n, 256, 0, s>>>(g, r, w, u)));
+}
+void launch_cell_moments(const Geo &g, const QuadRule &r, const WaveGeo &w, const double *u, double time, int mode,
+                         double *partials, hipStream_t s) {
+  EG_DISPATCH_Q(g.q, (k_cell_moments<Q><<<g.n, 256, 0, s>>>(g, r, w, u, time, mode, partials)));
 }
 void launch_physics_probe(const double *states, int count, double gamma, double *out, hipStream_t s) {
   k_physics_probe<<<(count + 63) / 64, 64, 0, s>>>(states, count, gamma, out);

@@ -226,6 +226,30 @@ Problem::Problem(const Options &o, const EulerGas &g, hipStream_t s)
   eg::launch_convert<double, float>(d_trace_metric, d_trace_metric32, nl, stream);
   hip_check(hipMemsetAsync(flags, 0, sizeof(int), stream), "memset");
   hip_check(hipMemsetAsync(old, 0, sizeof(double) * nu, stream), "memset");
+  // the projection / error rule and the cell origins for the device
+  {
+    std::vector<double> gx, gw;
+    const int nq = std::max(24, 2 * opt.k + 8) / 2 + 1;
+    require(nq <= eg::kMaxQuad, "Projection rule above the device table size");
+    gl_rule(nq, gx, gw);
+    rule.nq = nq;
+    for (int s2 = 0; s2 < nq; ++s2) {
+      rule.x[s2] = gx[s2];
+      rule.w[s2] = gw[s2];
+      const auto v = lagrange_values(x1, gx[s2]);
+      for (int i = 0; i < q; ++i) { rule.b[s2 * q + i] = v[i]; }
+    }
+    d_origx.resize(n);
+    d_origy.resize(n);
+    moments.resize(4 * size_t(n));
+    hip_check(hipMemcpyAsync(d_origx, origx.data(), sizeof(double) * n, hipMemcpyHostToDevice, stream), "upload");
+    hip_check(hipMemcpyAsync(d_origy, origy.data(), sizeof(double) * n, hipMemcpyHostToDevice, stream), "upload");
+    wave.origx = d_origx;
+    wave.origy = d_origy;
+    wave.alpha = alpha;
+    wave.ca = std::cos(alpha);
+    wave.sa = std::sin(alpha);
+  }
   Sync();
 }
 Problem::~Problem() {
@@ -264,34 +288,17 @@ void Problem::CheckPinnedFlags(const char *context) {
 }
 
 // src/problem.cpp:329-349: L2 projection with the 13x13 tensor GL rule
-// (order max(24, 2k+8)), divided by the diagonal GL mass.
-std::vector<double> Problem::Initial() const {
-  std::vector<double> u(nu, 0.), gx, gw;
-  const int nq = std::max(24, 2 * opt.k + 8) / 2 + 1;
-  gl_rule(nq, gx, gw);
-  std::vector<double> bx(size_t(nq) * q);
-  for (int a = 0; a < nq; ++a) {
-    const auto v = lagrange_values(x1, gx[a]);
-    for (int i = 0; i < q; ++i) { bx[a * q + i] = v[i]; }
-  }
-  for (int e = 0; e < n; ++e) {
-    for (int qy = 0; qy < nq; ++qy) {
-      for (int qx = 0; qx < nq; ++qx) {
-        const double wgt = gw[qx] * gw[qy];
-        const auto v = gas.exact(origx[e] + hx * gx[qx], origy[e] + hy * gx[qy], 0, alpha);
-        for (int i = 0; i < p; ++i) {
-          const double sh = bx[qx * q + i % q] * bx[qy * q + i / q];
-          for (int c = 0; c < 4; ++c) { u[size_t(e) * m + 4 * i + c] += wgt * sh * v[c]; }
-        }
-      }
-    }
-  }
-  for (int e = 0; e < n; ++e) {
-    for (int i = 0; i < p; ++i) {
-      const double im = 1. / (w1[i % q] * w1[i / q]);
-      for (int c = 0; c < 4; ++c) { u[size_t(e) * m + 4 * i + c] *= im; }
-    }
-  }
+// (order max(24, 2k+8)), divided by the diagonal GL mass, on the device.
+void Problem::InitialDevice(double *dev_u) {
+  eg::launch_initial(geo, rule, wave, dev_u, stream);
+  hip_check(hipGetLastError(), "initial projection");
+}
+std::vector<double> Problem::Initial() {
+  DeviceArray<double> d(nu);
+  InitialDevice(d);
+  std::vector<double> u(nu);
+  hip_check(hipMemcpyAsync(u.data(), d, sizeof(double) * nu, hipMemcpyDeviceToHost, stream), "download");
+  Sync();
   return u;
 }
 // src/problem.cpp:350-384 on the device.
@@ -372,40 +379,37 @@ std::vector<std::vector<int>> Problem::SolverGraph(bool hdg) const {
   return graph;
 }
 // src/problem.cpp:704-726
-std::array<double, 4> Problem::Errors(const std::vector<double> &u, double time) const {
+// src/problem.cpp:704-738 on the device: per-cell sums (the error integrals
+// with the 13x13 rule, or the conserved integrals), then one fixed-order
+// reduction per component.
+std::array<double, 4> Problem::ErrorsDevice(const double *dev_u, double time) {
+  require(time >= 0 && time * pi * (gas.Gamma() + 1.) / 4. < 1, "Exact solution requires T < 4/((gamma+1)*pi)");
+  eg::launch_cell_moments(geo, rule, wave, dev_u, time, 1, moments, stream);
+  eg::launch_sum_partials(moments, n, scalars, 0, stream, 4, pinned, flags, pinned_flags);
+  Sync();
   std::array<double, 4> a{};
-  std::vector<double> gx, gw;
-  const int nq = std::max(24, 2 * opt.k + 8) / 2 + 1;
-  gl_rule(nq, gx, gw);
-  std::vector<double> bx(size_t(nq) * q);
-  for (int s = 0; s < nq; ++s) {
-    const auto v = lagrange_values(x1, gx[s]);
-    for (int i = 0; i < q; ++i) { bx[s * q + i] = v[i]; }
-  }
-  for (int e = 0; e < n; ++e) {
-    for (int qy = 0; qy < nq; ++qy) {
-      for (int qx = 0; qx < nq; ++qx) {
-        const auto v = gas.exact(origx[e] + hx * gx[qx], origy[e] + hy * gx[qy], time, alpha);
-        for (int c = 0; c < 4; ++c) {
-          double z = 0;
-          for (int i = 0; i < p; ++i) { z += bx[qx * q + i % q] * bx[qy * q + i / q] * u[size_t(e) * m + 4 * i + c]; }
-          a[c] += area * gw[qx] * gw[qy] * (z - v[c]) * (z - v[c]);
-        }
-      }
-    }
-  }
-  for (auto &v : a) { v = std::sqrt(v); }
+  for (int c = 0; c < 4; ++c) { a[c] = std::sqrt(pinned[c]); }
   return a;
 }
-// src/problem.cpp:727-738
-std::array<double, 4> Problem::Integral(const std::vector<double> &u) const {
+std::array<double, 4> Problem::IntegralDevice(const double *dev_u) {
+  eg::launch_cell_moments(geo, rule, wave, dev_u, 0., 0, moments, stream);
+  eg::launch_sum_partials(moments, n, scalars, 0, stream, 4, pinned, flags, pinned_flags);
+  Sync();
   std::array<double, 4> a{};
-  for (int e = 0; e < n; ++e) {
-    for (int i = 0; i < p; ++i) {
-      for (int c = 0; c < 4; ++c) { a[c] += area * (w1[i % q] * w1[i / q]) * u[size_t(e) * m + 4 * i + c]; }
-    }
-  }
+  for (int c = 0; c < 4; ++c) { a[c] = pinned[c]; }
   return a;
+}
+std::array<double, 4> Problem::Errors(const std::vector<double> &u, double time) {
+  require(int(u.size()) == nu, "Solution size mismatch");
+  DeviceArray<double> d(nu);
+  hip_check(hipMemcpyAsync(d, u.data(), sizeof(double) * nu, hipMemcpyHostToDevice, stream), "upload");
+  return ErrorsDevice(d, time);
+}
+std::array<double, 4> Problem::Integral(const std::vector<double> &u) {
+  require(int(u.size()) == nu, "Solution size mismatch");
+  DeviceArray<double> d(nu);
+  hip_check(hipMemcpyAsync(d, u.data(), sizeof(double) * nu, hipMemcpyHostToDevice, stream), "upload");
+  return IntegralDevice(d);
 }
 // src/problem.cpp:739-761
 std::array<double, 4> Problem::Differences(const std::vector<double> &u, const std::vector<double> &v) const {

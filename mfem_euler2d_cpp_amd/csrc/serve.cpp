@@ -127,7 +127,7 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
         try {
           EulerGas gas(requests[i].gamma);
           Problem p(opts[i], gas, stream);
-          RunResult res = Run(p, requests[i].form ? "hdg" : "dg", "");
+          RunResult res = Run(p, requests[i].form ? "hdg" : "dg", "", false, cfg.keep_states);
           r.updates = res.updates;
           r.time_steps = res.time_steps;
           r.newton_seconds = res.newton_seconds;

@@ -26,7 +26,8 @@ void download(const Problem &p, const double *dev, std::vector<double> &host) {
 }
 } // namespace
 
-RunResult Run(Problem &p, const std::string &form, const std::string &prefix, bool keep_increments) {
+RunResult Run(Problem &p, const std::string &form, const std::string &prefix, bool keep_increments,
+              bool keep_state) {
   const auto &o = p.opt;
   RunResult out;
   const int nu = p.nu;
@@ -36,10 +37,10 @@ RunResult Run(Problem &p, const std::string &form, const std::string &prefix, bo
       stage_base(nu), neg(nu), check_y(nu);
   Vec slope0(nu), slope1(nu), slope2(nu), slope3(nu);
   double *slope[4] = {slope0.p(), slope1.p(), slope2.p(), slope3.p()};
-  const std::vector<double> init = p.Initial();
-  hip_check(hipMemcpyAsync(u.p(), init.data(), sizeof(double) * nu, hipMemcpyHostToDevice, s), "upload");
+  // the initial projection and its integrals on the device (src/stepper.cpp:31-36)
+  p.InitialDevice(u.p());
   require(p.Admissible(u.p()), "Initial projection not positive; refine mesh");
-  const auto initial = p.Integral(init);
+  const auto initial = p.IntegralDevice(u.p());
   std::ofstream log;
   if (!o.timing_only && !prefix.empty()) {
     log.open(prefix + "_iterations.csv");
@@ -226,11 +227,14 @@ RunResult Run(Problem &p, const std::string &form, const std::string &prefix, bo
   out.seconds = clock_s() - begin;
   out.final_time = time;
   out.time_steps = step;
-  download(p, u.p(), out.u);
-  out.errors = p.Errors(out.u, time);
-  const auto final_integral = p.Integral(out.u);
+  // errors and conservation on the device; the state is copied to the host
+  // only when it is returned or saved
+  out.errors = p.ErrorsDevice(u.p(), time);
+  const auto final_integral = p.IntegralDevice(u.p());
   for (int c = 0; c < 4; ++c) { out.drift = std::max(out.drift, std::abs(final_integral[c] - initial[c])); }
-  if (o.save_solution && !prefix.empty()) { p.Save(out.u, prefix); }
+  const bool save = o.save_solution && !prefix.empty();
+  if (keep_state || save) { download(p, u.p(), out.u); }
+  if (save) { p.Save(out.u, prefix); }
   if (!prefix.empty()) {
     std::ofstream summary(prefix + "_summary.csv");
     require(bool(summary), "Cannot open summary output");
