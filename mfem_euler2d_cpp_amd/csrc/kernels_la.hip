@@ -4,8 +4,10 @@
 // weighted restarted GMRES (fused CGS2 passes with deterministic two-stage
 // reductions).
 #include <algorithm>
+#include <cstdlib>
 #include <mutex>
 #include <stdexcept>
+#include <string>
 
 #include "kernels.hpp"
 
@@ -17,12 +19,16 @@ namespace {
 // fit one CU's registers and LDS, so they are worked on in global memory (L2)
 // by one 1024-thread workgroup, with the pivot column and row staged in LDS.
 constexpr int kBigThreads = 1024;
-constexpr int kBigMax = 511; // pivot keys keep the row in 9 bits
+constexpr int kBigMax = 324; // largest cell block (k = 8)
+constexpr int kPanelW = 32;  // columns of a Gauss-Jordan panel
+constexpr int kChunkMax = 112; // columns a chunk of the rank-kPanelW update (b = 144)
 
 struct BigLds {
-  double col[kBigMax], row[kBigMax];
+  double pan[kPanelW * kBigMax]; // the panel, [column][row]
+  double ak[kPanelW * kChunkMax]; // pivot rows of a column chunk, [panel column][chunk column]
+  double col[kBigMax], row[kPanelW];
   unsigned long long key[kBigThreads / 64];
-  int piv[kBigMax];
+  int piv[kBigMax], perm[kBigMax];
 };
 
 __device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsigned long long b) { return a > b ? a : b; }
@@ -30,51 +36,131 @@ __device__ __forceinline__ unsigned long long shfl_xor64(unsigned long long v, i
   const int lo = __shfl_xor(int(unsigned(v)), mask, 64), hi = __shfl_xor(int(unsigned(v >> 32)), mask, 64);
   return (static_cast<unsigned long long>(unsigned(hi)) << 32) | unsigned(lo);
 }
+// Columns of one chunk of the panel update: (b / 4) x (chunk / 4) thread
+// tiles of 4 x 4 fill the 1024 threads (b a multiple of 4).
+__device__ __forceinline__ int update_chunk(int b) {
+  const int c = 4 * (kBigThreads / (b / 4));
+  return c < kChunkMax ? c : kChunkMax;
+}
 
 // Partially pivoted in-place Gauss-Jordan inverse of one b x b column-major
 // block in global memory (pivot: the row of maximal |a| among rows k..b-1,
 // near-ties to the lowest row; explicit row interchanges, undone on the
-// columns at the end). Mathematically the inverse of the block whose LU the
-// reference uses (src/linear.cpp:198-202, src/block_ilu.cpp:83-84,
-// src/condensation.cpp:137-140). Returns true on a zero or non-finite pivot.
+// columns at the end), blocked by panels of kPanelW columns: the panel is
+// eliminated in LDS (step k: swap the pivot row into row k, scale it, and
+// subtract its multiples from every other row of the panel, leaving the
+// panel's columns of the inverse), then every other column a, read once
+// from memory, becomes P a + G (P a)_K with P the panel's row interchanges,
+// K its pivot rows and G its final columns minus the identity on K: the
+// same transformation as the panel's kPanelW single Gauss-Jordan steps, as a
+// rank-kPanelW update (4 x 4 register tiles). Mathematically the inverse of
+// the block whose LU the reference uses (src/linear.cpp:198-202,
+// src/block_ilu.cpp:83-84, src/condensation.cpp:137-140). Returns true on a
+// zero or non-finite pivot.
 __device__ bool gj_global(double *__restrict__ A, int b, BigLds &L) {
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
+  const int chunk = update_chunk(b);
   bool singular = false;
-  for (int k = 0; k < b; ++k) {
-    unsigned long long key = 0ull;
-    for (int i = k + tid; i < b; i += nt) {
-      const unsigned long long kk =
-          (static_cast<unsigned long long>(__double_as_longlong(A[i + size_t(k) * b])) & 0x7FFFFFFFFFFFFE00ull) |
-          unsigned(kBigMax - i);
-      key = umax64(key, kk);
+  for (int k0 = 0; k0 < b; k0 += kPanelW) {
+    const int W = min(kPanelW, b - k0);
+    for (int e = tid; e < W * b; e += nt) {
+      const int w = e / b, i = e % b;
+      L.pan[w * b + i] = A[i + size_t(k0 + w) * b];
     }
-    for (int m = 32; m > 0; m >>= 1) { key = umax64(key, shfl_xor64(key, m)); }
-    if (lane == 0) { L.key[wave] = key; }
+    for (int i = tid; i < b; i += nt) { L.perm[i] = i; }
     __syncthreads();
-    unsigned long long best = 0ull;
-    for (int w = 0; w < nt / 64; ++w) { best = umax64(best, L.key[w]); }
-    const int p = kBigMax - int(best & 511ull); // best > 0: rows k..b-1 all carry their index
-    if (tid == 0) { L.piv[k] = p; }
-    if (p != k) {
-      for (int j = tid; j < b; j += nt) {
-        const double t = A[k + size_t(j) * b];
-        A[k + size_t(j) * b] = A[p + size_t(j) * b];
-        A[p + size_t(j) * b] = t;
+    for (int w = 0; w < W; ++w) {
+      const int k = k0 + w;
+      double *pc = L.pan + w * b;
+      unsigned long long key = 0ull;
+      for (int i = k + tid; i < b; i += nt) {
+        key = umax64(key, (static_cast<unsigned long long>(__double_as_longlong(pc[i])) & 0x7FFFFFFFFFFFFE00ull) |
+                              unsigned(511 - i));
+      }
+      for (int m = 32; m > 0; m >>= 1) { key = umax64(key, shfl_xor64(key, m)); }
+      if (lane == 0) { L.key[wave] = key; }
+      __syncthreads();
+      unsigned long long best = 0ull;
+      for (int v = 0; v < nt / 64; ++v) { best = umax64(best, L.key[v]); }
+      const int p = 511 - int(best & 511ull); // rows k..b-1 all carry their index
+      if (p != k && tid < W) {
+        const double t = L.pan[tid * b + k];
+        L.pan[tid * b + k] = L.pan[tid * b + p];
+        L.pan[tid * b + p] = t;
+      }
+      if (tid == 0) {
+        L.piv[k] = p;
+        const int t = L.perm[k];
+        L.perm[k] = L.perm[p];
+        L.perm[p] = t;
+      }
+      __syncthreads();
+      for (int i = tid; i < b; i += nt) { L.col[i] = pc[i]; }
+      if (tid < W) { L.row[tid] = L.pan[tid * b + k]; }
+      __syncthreads();
+      const double pivot = L.row[w];
+      if (!(fabs(pivot) > 0.) || !isfinite(pivot)) { singular = true; }
+      const double inv = 1. / pivot;
+      for (int e = tid; e < W * b; e += nt) {
+        const int c = e / b, i = e % b;
+        double v;
+        if (i == k) { v = c == w ? inv : L.row[c] * inv; }
+        else if (c == w) { v = -L.col[i] * inv; }
+        else { v = L.pan[c * b + i] - L.col[i] * (L.row[c] * inv); }
+        L.pan[c * b + i] = v;
       }
       __syncthreads();
     }
-    for (int i = tid; i < b; i += nt) { L.col[i] = A[i + size_t(k) * b]; L.row[i] = A[k + size_t(i) * b]; }
-    __syncthreads();
-    const double pivot = L.row[k];
-    if (!(fabs(pivot) > 0.) || !isfinite(pivot)) { singular = true; }
-    const double inv = 1. / pivot;
-    for (int idx = tid; idx < b * b; idx += nt) {
-      const int i = idx % b, j = idx / b;
-      double v;
-      if (i == k) { v = j == k ? inv : L.row[j] * inv; }
-      else if (j == k) { v = -L.col[i] * inv; }
-      else { v = A[idx] - L.col[i] * (L.row[j] * inv); }
-      A[idx] = v;
+    // every other column: a <- P a + G (P a)_K
+    const int tr = tid % (b / 4), tc = tid / (b / 4);
+    const bool act = tc < chunk / 4;
+    for (int c0 = 0; c0 < b; c0 += chunk) {
+      const int nc = min(chunk, b - c0);
+      for (int e = tid; e < W * nc; e += nt) {
+        const int w = e / nc, jj = e % nc;
+        L.ak[w * chunk + jj] = A[L.perm[k0 + w] + size_t(c0 + jj) * b];
+      }
+      __syncthreads();
+      double acc[4][4];
+      const int i0 = 4 * tr, j0 = 4 * tc;
+      if (act) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int src = L.perm[i0 + r];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int j = c0 + j0 + c;
+            acc[r][c] = (j0 + c < nc) ? A[src + size_t(j) * b] : 0.;
+          }
+        }
+        for (int w = 0; w < W; ++w) {
+          double g[4], a4[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) { g[r] = L.pan[w * b + i0 + r] - ((i0 + r == k0 + w) ? 1. : 0.); }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) { a4[c] = L.ak[w * chunk + j0 + c]; }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) { acc[r][c] += g[r] * a4[c]; }
+          }
+        }
+      }
+      __syncthreads(); // every read of this chunk's columns is done
+      if (act) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int j = c0 + j0 + c;
+          if (j0 + c < nc && (j < k0 || j >= k0 + W)) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) { A[i0 + r + size_t(j) * b] = acc[r][c]; }
+          }
+        }
+      }
+    }
+    for (int e = tid; e < W * b; e += nt) {
+      const int w = e / b, i = e % b;
+      A[i + size_t(k0 + w) * b] = L.pan[w * b + i];
     }
     __syncthreads();
   }

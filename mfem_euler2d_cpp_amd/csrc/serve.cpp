@@ -11,6 +11,8 @@
 // worker's stream, so a finishing solve does not synchronise the device.
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <deque>
 #include <mutex>
 #include <thread>
@@ -90,6 +92,9 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
   int idle = cfg.workers, running = 0, done = 0, max_running = 0;
   double reserved = 0., max_reserved = 0.;
   bool stop = false;
+  // EG_SERVE_PROGRESS=1: one line on stderr per finished request
+  const char *progress_env = std::getenv("EG_SERVE_PROGRESS");
+  const bool progress = progress_env && std::string(progress_env) == "1";
   const auto t0 = std::chrono::steady_clock::now();
   auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
 
@@ -147,6 +152,10 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
         ++idle;
         --running;
         ++done;
+        if (progress) {
+          std::fprintf(stderr, "[serve] %d/%d done at %.1f s (%d running)%s%s\n", done, int(order.size()), r.finish,
+                       running, r.status ? ": failed: " : "", r.status ? r.error.c_str() : "");
+        }
       }
       wake_sched.notify_one();
     }

@@ -120,3 +120,58 @@ def test_sweep_threads_same_preconditioner(gpu, oracle_lib, args):
         res[threads] = out
     for a, b in zip(res[256], res[1024]):
         assert rel(a, b) < 1e-13
+
+
+def _alone_and_mixed(reqs, workers):
+    alone = []
+    for r in reqs:
+        out, _, st = eg.serve([r], workers=1, keep_states=True)
+        assert out[0]["status"] == 0, (r, out[0]["error"])
+        alone.append((out[0], st[0]))
+    out, rep, states = eg.serve(reqs, workers=workers, keep_states=True)
+    assert rep["max_concurrent"] >= 2
+    for i, (o, u) in enumerate(zip(out, states)):
+        assert o["status"] == 0, (i, o["error"])
+        assert o["updates"] == alone[i][0]["updates"]
+        assert np.array_equal(u, alone[i][1]), (reqs[i], float(np.max(np.abs(u - alone[i][1]))))
+    return out
+
+
+@pytest.mark.gpu
+def test_batched_draws_alone_equal_in_mix(gpu):
+    """The first six draws of the batched form's seeded stream (HDG, mixed
+    precision, 2 DIRK steps x 3 Newton updates): each result bitwise the same
+    alone and run concurrently."""
+    import bench
+    gen = bench.batched_draws()
+    reqs = [(next(gen) + " --precision mixed", "hdg", 0.) for _ in range(6)]
+    out = _alone_and_mixed(reqs, workers=6)
+    assert all(o["updates"] == 24 for o in out)
+
+
+@pytest.mark.gpu
+def test_batched_draw_matches_oracle(gpu, oracle_lib):
+    """The smallest of those draws in plain FP64 against the CPU oracle."""
+    import bench
+    gen = bench.batched_draws()
+    cases = [next(gen) for _ in range(6)]
+    size = lambda c: (int(c.split("--nx ")[1].split()[0]) * int(c.split("--ny ")[1].split()[0])
+                      * (int(c.split("--k ")[1].split()[0]) + 1) ** 2)
+    case = min(cases, key=size)
+    print(f"case: {case}", flush=True)
+    u, st, *_ = gpu.Context(case).run("hdg")
+    uo, so, *_ = oracle_lib.Oracle(case).run("hdg")
+    assert st.updates == int(so[0]) == 24
+    assert float(np.max(np.abs(u - uo)) / np.max(np.abs(uo))) < 1e-10
+
+
+@pytest.mark.gpu
+def test_served_large_draws_alone_equal_in_mix(gpu):
+    """Requests from the served distribution's corners, k = 7 and 8 at up to
+    128 x 128 next to small ones (Newton to convergence, CFL 1, short final
+    times): each result bitwise the same alone and in the mix."""
+    reqs = [("--nx 128 --ny 128 --k 8 --alpha-deg 40 --cfl 1 --T 0.002 --precond block-ilu0", "hdg", 0.),
+            ("--nx 96 --ny 96 --k 7 --alpha-deg 65 --cfl 1 --T 0.002 --precond block-ilu0", "dg", 0.),
+            ("--nx 8 --ny 8 --k 0 --alpha-deg 12 --cfl 1 --T 0.05 --precond block-ilu0", "dg", 0.),
+            ("--nx 40 --ny 40 --k 5 --alpha-deg 80 --cfl 1 --T 0.02 --precond block-ilu0", "hdg", 0.)]
+    _alone_and_mixed(reqs, workers=4)
