@@ -137,24 +137,35 @@ def _alone_and_mixed(reqs, workers):
     return out
 
 
+def _batched_six():
+    """The batched form's first six instances under its seeded
+    reject-and-redraw rule (the plain FP64 solves screen the draws)."""
+    import bench
+    screen = lambda cases: eg.serve([(c, "hdg", 0.) for c in cases], workers=6)[0]
+    cases, rejected = bench.batched_cases(6, screen)
+    return cases, rejected
+
+
 @pytest.mark.gpu
 def test_batched_draws_alone_equal_in_mix(gpu):
-    """The first six draws of the batched form's seeded stream (HDG, mixed
-    precision, 2 DIRK steps x 3 Newton updates): each result bitwise the same
-    alone and run concurrently."""
-    import bench
-    gen = bench.batched_draws()
-    reqs = [(next(gen) + " --precision mixed", "hdg", 0.) for _ in range(6)]
+    """The batched form's first six instances (HDG, mixed precision, 2 DIRK
+    steps x 3 Newton updates; a draw the reference algorithm rejects is
+    redrawn): each result bitwise the same alone and run concurrently."""
+    cases, rejected = _batched_six()
+    assert all(e == "SSP-DIRK3 step produced an inadmissible state" for _, e in rejected)
+    reqs = [(c + " --precision mixed", "hdg", 0.) for c in cases]
     out = _alone_and_mixed(reqs, workers=6)
     assert all(o["updates"] == 24 for o in out)
 
 
 @pytest.mark.gpu
 def test_batched_draw_matches_oracle(gpu, oracle_lib):
-    """The smallest of those draws in plain FP64 against the CPU oracle."""
-    import bench
-    gen = bench.batched_draws()
-    cases = [next(gen) for _ in range(6)]
+    """The smallest of those instances in plain FP64 against the CPU oracle,
+    and a rejected draw rejected by the oracle with the same message."""
+    cases, rejected = _batched_six()
+    if rejected:
+        with pytest.raises(oracle_lib.OracleError, match="inadmissible"):
+            oracle_lib.Oracle(rejected[0][0]).run("hdg")
     size = lambda c: (int(c.split("--nx ")[1].split()[0]) * int(c.split("--ny ")[1].split()[0])
                       * (int(c.split("--k ")[1].split()[0]) + 1) ** 2)
     case = min(cases, key=size)
