@@ -173,6 +173,26 @@ int eg_solve_footprint(const char *args, int form, double *bytes);
 int eg_serve(const char *const *args, const int *forms, const double *arrivals, int n, int workers,
              double hbm_budget, eg_solve_outcome *out, double *const *states, char *errors,
              eg_serve_report *report);
+
+/* The batched form of eg_serve (the same requests, admission and outcomes;
+ * up to 256 workers): the Arnoldi steps of the condensed (form 1) solves
+ * with block ILU(0) that run at the same time are launched together, those
+ * of one trace order k + 1 and arithmetic as one replay of a recorded graph
+ * of persistent grouped kernels (ILU(0) sweeps, Schur apply, CGS2) over all
+ * of them, instead of each solve's own launches; every result is the same,
+ * bit for bit, as that solve's own launches give. window_us (<= 0: 150): how
+ * long the first waiting step waits for the steps of the other solves inside
+ * a Krylov iteration. batch (optional) receives the grouping counts. */
+typedef struct {
+  long long flushes;        /* grouped launches */
+  long long steps;          /* Arnoldi steps in them */
+  long long graph_launches; /* of those, graph replays */
+  int max_group;            /* most steps in one launch */
+  int tile[10];             /* per trace order q = k + 1: virtual sweep workgroups per solve, chosen online (0: none) */
+} eg_batch_report;
+int eg_serve_batched(const char *const *args, const int *forms, const double *arrivals, int n, int workers,
+                     double hbm_budget, double window_us, eg_solve_outcome *out, double *const *states,
+                     char *errors, eg_serve_report *report, eg_batch_report *batch);
 /* ILU(0) launch-shape autotuner: for each k in ks, times the preconditioner
  * setup plus 16 applications (form 0 dg / 1 hdg, precision "fp64" or
  * "mixed") on a 16x16 case of that k under three shapes: "levels-256" and

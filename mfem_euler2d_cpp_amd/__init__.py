@@ -50,6 +50,11 @@ class ServeReport(ctypes.Structure):
     _fields_ = [("wall", ctypes.c_double), ("max_reserved", ctypes.c_double), ("max_concurrent", ctypes.c_int)]
 
 
+class BatchReport(ctypes.Structure):
+    _fields_ = [("flushes", ctypes.c_longlong), ("steps", ctypes.c_longlong), ("graph_launches", ctypes.c_longlong),
+                ("max_group", ctypes.c_int), ("tile", ctypes.c_int * 10)]
+
+
 _SIGNATURES = {
     "eg_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, _vp, ctypes.POINTER(_vp)]),
     "eg_destroy": (None, [_vp]),
@@ -87,6 +92,10 @@ _SIGNATURES = {
     "eg_serve": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), _c_int_p, _c_double_p, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_double, ctypes.POINTER(SolveOutcome), ctypes.POINTER(_c_double_p),
                                 ctypes.c_char_p, ctypes.POINTER(ServeReport)]),
+    "eg_serve_batched": (ctypes.c_int, [ctypes.POINTER(ctypes.c_char_p), _c_int_p, _c_double_p, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(SolveOutcome),
+                                        ctypes.POINTER(_c_double_p), ctypes.c_char_p, ctypes.POINTER(ServeReport),
+                                        ctypes.POINTER(BatchReport)]),
     "eg_autotune_ilu": (ctypes.c_int, [_c_int_p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
                                        _c_double_p]),
 }
@@ -274,11 +283,13 @@ def solve_footprint(args: str, form) -> float:
     return out.value
 
 
-def serve(requests, workers=8, hbm_budget=0.0, keep_states=False):
-    """Run independent solves concurrently (eg_serve). requests: list of
-    (args, form, arrival_seconds). Returns (outcomes, report, states): one
-    dict per request in request order, the serve report, and the final states
-    (numpy arrays, or None) when keep_states."""
+def serve(requests, workers=8, hbm_budget=0.0, keep_states=False, batched=False, window_us=0.0):
+    """Run independent solves concurrently (eg_serve, or eg_serve_batched
+    when batched: the Arnoldi steps of concurrent condensed solves launched
+    together). requests: list of (args, form, arrival_seconds). Returns
+    (outcomes, report, states): one dict per request in request order, the
+    serve report (with the grouping counts under "batch" when batched), and
+    the final states (numpy arrays, or None) when keep_states."""
     import numpy as np
     lib = load()
     n = len(requests)
@@ -297,7 +308,12 @@ def serve(requests, workers=8, hbm_budget=0.0, keep_states=False):
             holders.append(a)
             ptrs[i] = a.ctypes.data_as(_c_double_p)
         states = ptrs
-    _check(lib.eg_serve(args, forms, arrivals, n, workers, hbm_budget, out, states, errors, ctypes.byref(rep)))
+    brep = BatchReport()
+    if batched:
+        _check(lib.eg_serve_batched(args, forms, arrivals, n, workers, hbm_budget, window_us, out, states, errors,
+                                    ctypes.byref(rep), ctypes.byref(brep)))
+    else:
+        _check(lib.eg_serve(args, forms, arrivals, n, workers, hbm_budget, out, states, errors, ctypes.byref(rep)))
     res = []
     for i in range(n):
         o = out[i]
@@ -307,6 +323,10 @@ def serve(requests, workers=8, hbm_budget=0.0, keep_states=False):
                     "time_steps": o.time_steps, "worker": o.worker,
                     "error": errors.raw[256 * i:256 * (i + 1)].split(b"\0", 1)[0].decode()})
     report = {"wall": rep.wall, "max_reserved": rep.max_reserved, "max_concurrent": rep.max_concurrent}
+    if batched:
+        report["batch"] = {"flushes": brep.flushes, "steps": brep.steps, "graph_launches": brep.graph_launches,
+                           "max_group": brep.max_group,
+                           "tile": {q - 1: brep.tile[q] for q in range(1, 10) if brep.tile[q]}}
     return res, report, (holders if keep_states else None)
 
 

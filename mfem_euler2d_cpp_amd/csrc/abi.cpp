@@ -326,49 +326,78 @@ int eg_solve_footprint(const char *args, int form, double *bytes) {
     *bytes = double(SolveFootprint(ParseOptions(std::string(args ? args : "")), form));
   });
 }
+namespace {
+void serve_into(const char *const *args, const int *forms, const double *arrivals, int n, const ServeConfig &cfg,
+                eg_solve_outcome *out, double *const *states, char *errors, eg_serve_report *report,
+                eg_batch_report *batch) {
+  require(n >= 0 && args && forms && out, "eg_serve needs args, forms and out");
+  std::vector<SolveRequest> reqs(n);
+  for (int i = 0; i < n; ++i) {
+    reqs[i].args = args[i] ? args[i] : "";
+    reqs[i].form = forms[i];
+    reqs[i].arrival = arrivals ? arrivals[i] : 0.;
+  }
+  ServeConfig c = cfg;
+  c.keep_states = states != nullptr;
+  ServeReport rep;
+  auto res = Serve(reqs, c, &rep);
+  for (int i = 0; i < n; ++i) {
+    const auto &r = res[i];
+    eg_solve_outcome o{};
+    o.arrival = r.arrival;
+    o.start = r.start;
+    o.finish = r.finish;
+    o.newton_seconds = r.newton_seconds;
+    o.footprint = double(r.footprint);
+    o.measured_peak = double(r.measured_peak);
+    for (int k = 0; k < 4; ++k) { o.errors[k] = r.errors[k]; }
+    o.status = r.status;
+    o.updates = r.updates;
+    o.time_steps = r.time_steps;
+    o.worker = r.worker;
+    out[i] = o;
+    if (states && states[i] && !r.u.empty()) { std::memcpy(states[i], r.u.data(), sizeof(double) * r.u.size()); }
+    if (errors) {
+      std::strncpy(errors + size_t(i) * 256, r.error.c_str(), 255);
+      errors[size_t(i) * 256 + 255] = 0;
+    }
+  }
+  if (report) {
+    report->wall = rep.wall;
+    report->max_reserved = double(rep.max_reserved);
+    report->max_concurrent = rep.max_concurrent;
+  }
+  if (batch) {
+    *batch = eg_batch_report{};
+    batch->flushes = rep.batch.flushes;
+    batch->steps = rep.batch.steps;
+    batch->graph_launches = rep.batch.graph_launches;
+    batch->max_group = rep.batch.max_group;
+    for (int q = 0; q <= eg::kMaxQ; ++q) { batch->tile[q] = rep.batch.tile[q]; }
+  }
+}
+} // namespace
+
 int eg_serve(const char *const *args, const int *forms, const double *arrivals, int n, int workers,
              double hbm_budget, eg_solve_outcome *out, double *const *states, char *errors,
              eg_serve_report *report) {
   return guard(nullptr, [&] {
-    require(n >= 0 && args && forms && out, "eg_serve needs args, forms and out");
-    std::vector<SolveRequest> reqs(n);
-    for (int i = 0; i < n; ++i) {
-      reqs[i].args = args[i] ? args[i] : "";
-      reqs[i].form = forms[i];
-      reqs[i].arrival = arrivals ? arrivals[i] : 0.;
-    }
     ServeConfig cfg;
     cfg.workers = workers;
     cfg.hbm_budget = hbm_budget;
-    cfg.keep_states = states != nullptr;
-    ServeReport rep;
-    auto res = Serve(reqs, cfg, &rep);
-    for (int i = 0; i < n; ++i) {
-      const auto &r = res[i];
-      eg_solve_outcome o{};
-      o.arrival = r.arrival;
-      o.start = r.start;
-      o.finish = r.finish;
-      o.newton_seconds = r.newton_seconds;
-      o.footprint = double(r.footprint);
-      o.measured_peak = double(r.measured_peak);
-      for (int c = 0; c < 4; ++c) { o.errors[c] = r.errors[c]; }
-      o.status = r.status;
-      o.updates = r.updates;
-      o.time_steps = r.time_steps;
-      o.worker = r.worker;
-      out[i] = o;
-      if (states && states[i] && !r.u.empty()) { std::memcpy(states[i], r.u.data(), sizeof(double) * r.u.size()); }
-      if (errors) {
-        std::strncpy(errors + size_t(i) * 256, r.error.c_str(), 255);
-        errors[size_t(i) * 256 + 255] = 0;
-      }
-    }
-    if (report) {
-      report->wall = rep.wall;
-      report->max_reserved = double(rep.max_reserved);
-      report->max_concurrent = rep.max_concurrent;
-    }
+    serve_into(args, forms, arrivals, n, cfg, out, states, errors, report, nullptr);
+  });
+}
+int eg_serve_batched(const char *const *args, const int *forms, const double *arrivals, int n, int workers,
+                     double hbm_budget, double window_us, eg_solve_outcome *out, double *const *states,
+                     char *errors, eg_serve_report *report, eg_batch_report *batch) {
+  return guard(nullptr, [&] {
+    ServeConfig cfg;
+    cfg.workers = workers;
+    cfg.hbm_budget = hbm_budget;
+    cfg.batched = true;
+    if (window_us > 0.) { cfg.window_us = window_us; }
+    serve_into(args, forms, arrivals, n, cfg, out, states, errors, report, batch);
   });
 }
 int eg_autotune_ilu(const int *ks, int nk, int form, const char *precision, char *out, double *times) {

@@ -48,6 +48,44 @@ struct Frozen {
   double a, b, inv_sum, wl, wr, pen, pad0, pad1;
 };
 
+// ------------------------------------------------------------ grouped launches
+// One Arnoldi step of many independent condensed (Schur) solves of one trace
+// order in one launch per stage (kernels k_group_*): each instance is a slot
+// of the table, and the virtual blocks of stage X of slot i are
+// [X_off[i], X_off[i+1]): exactly the blocks its own single-solve launch of
+// that stage would have, so every instance computes what its own launches
+// compute, bit for bit, whatever else shares the launch. A fixed persistent
+// grid walks the virtual blocks (the launches are recorded once in a graph
+// and replayed with a new table).
+constexpr int kGroupMax = 256;
+struct ArnoldiSlot {
+  void *V, *z, *w;        // Krylov basis (v_j at V + j N), z = M^-1 v_j, w = S z
+  const void *metric;     // trace metric
+  const void *bsr;        // ILU(0) factors
+  void *mid;              // the sweeps' intermediate vector (armed between sweeps)
+  const int *row_ptr, *diag_ptr, *col, *lower_rows, *upper_rows;
+  int *ticket, *flags;    // the sweeps' ticket counters, the solve's error flags
+  const void *panels;     // Schur face panels
+  const double *D;        // trace diagonal
+  double *partA, *partB, *scal, *host_col; // reduction partials, projections, pinned column
+  long N;
+  int j, nb, scaled, nx, ny, nf;
+};
+struct GroupTable {
+  int count;
+  int sweep_off[kGroupMax + 1], schur_off[kGroupMax + 1], red_off[kGroupMax + 1], norm_off[kGroupMax + 1];
+  ArnoldiSlot slot[kGroupMax];
+};
+// Slot of virtual block v: off[i] <= v < off[i + 1] (off increasing, count slots).
+__device__ __forceinline__ int group_slot(const int *off, int count, int v) {
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= v) { lo = mid; } else { hi = mid; }
+  }
+  return lo;
+}
+
 // ------------------------------------------------------------ physics
 template <class T> __host__ __device__ __forceinline__ T pressure(const T u[4], T gm1) {
   return gm1 * (u[3] - T(.5) * (u[1] * u[1] + u[2] * u[2]) / u[0]);
@@ -102,7 +140,7 @@ __device__ __forceinline__ bool positive(const double u[4], double gm1) {
 }
 
 // Cell/face topology helpers.
-__device__ __forceinline__ int face_of(const Geo &g, int e, int side) {
+template <class G> __device__ __forceinline__ int face_of(const G &g, int e, int side) {
   const int ix = e % g.nx, iy = e / g.nx;
   switch (side) {
   case 0: return 2 * (iy * g.nx + (ix + g.nx - 1) % g.nx);
@@ -111,7 +149,7 @@ __device__ __forceinline__ int face_of(const Geo &g, int e, int side) {
   default: return 2 * e + 1;
   }
 }
-__device__ __forceinline__ int neighbour_of(const Geo &g, int e, int side) {
+template <class G> __device__ __forceinline__ int neighbour_of(const G &g, int e, int side) {
   const int ix = e % g.nx, iy = e / g.nx;
   switch (side) {
   case 0: return iy * g.nx + (ix + g.nx - 1) % g.nx;

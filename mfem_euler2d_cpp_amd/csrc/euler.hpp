@@ -7,9 +7,12 @@
 #pragma once
 #include <algorithm>
 #include <array>
+#include <condition_variable>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.hpp"
@@ -135,6 +138,8 @@ struct BlockPattern {
   int Find(int r, int c) const;
 };
 
+class KrylovBatcher;
+
 // include/euler.hpp:161-230: device-resident discretisation of one case.
 class Problem {
 public:
@@ -159,6 +164,9 @@ public:
   int freeze_generation = 0;
   double rate = 0;
   double last_state_norm = 0; // CellNorm of the state of the last (Trial)Residual
+  // Grouped Arnoldi steps with other solves (set by the batched form before
+  // the solvers of this problem are made; null: every launch is its own).
+  KrylovBatcher *batcher = nullptr;
 
   Problem(const Options &o, const EulerGas &gas, hipStream_t s = nullptr);
   ~Problem();
@@ -267,6 +275,13 @@ public:
   // slot j % 2 (pinned + 256 (j % 2)).
   hipEvent_t step_done[2] = {nullptr, nullptr};
   hipEvent_t setup_mark[2] = {nullptr, nullptr}; // around the set-up on the stream (its device time)
+  // The batched form: the grouped Arnoldi steps' batcher (from the problem)
+  // and the event marking this solve's work before a step it hands over.
+  KrylovBatcher *batcher = nullptr;
+  hipEvent_t batch_ready = nullptr;
+  // Whether the Arnoldi steps in T can be handed to the batcher (condensed
+  // system, single-launch ILU(0) sweeps of face blocks).
+  bool Batchable() const;
 
   LinearSolver(Problem &p, int form, Hybrid *hybrid = nullptr);
   ~LinearSolver();
@@ -289,7 +304,10 @@ public:
   void Solve(const double *dev_rhs, LinearResult &out, double tol, bool refresh);
   double Norm(const double *v);
   [[noreturn]] void Fail(const char *what);
+  template <class T> const T *Factors() const;
+  template <class T> T *SweepBuffer();
 private:
+  friend class KrylovBatcher;
   template <class T> void EnqueueArnoldi(T *Vb, T *zb, T *wb, const T *metric, int j);
   template <class T> double GMRES(const T *rhs, T *xv, double tol, int limit, int &iterations, bool zero_guess,
                                   double *true_norm,
@@ -343,11 +361,66 @@ struct ServeConfig {
   int workers = 8;         // concurrent solves (one host thread and stream each)
   double hbm_budget = 0.;  // bytes of device memory the solves may hold; 0: 80% of the free memory
   bool keep_states = false;
+  // Batched form: the Arnoldi steps of the condensed solves that run at the
+  // same time are grouped (KrylovBatcher); window_us: how long the first
+  // step of a group waits for the other solves' steps.
+  bool batched = false;
+  double window_us = 150.;
+};
+// Grouped Arnoldi steps of concurrent solves: counts, and per trace order
+// q (index q) the chosen virtual sweep workgroups per solve (0: none seen).
+struct BatchReport {
+  long flushes = 0, steps = 0, graph_launches = 0;
+  int max_group = 0;
+  int tile[eg::kMaxQ + 1] = {};
 };
 struct ServeReport {
   double wall = 0.;
   int max_concurrent = 0;
   size_t max_reserved = 0; // largest sum of admitted footprints
+  BatchReport batch;
+};
+
+// Collects the Arnoldi steps that concurrent condensed solves hand over
+// (Submit, from each solve's own thread) and launches those of one trace
+// order and arithmetic together: one replay of a recorded graph of the
+// grouped kernels (two ILU(0) sweeps, Schur apply, four CGS2 stages) over a
+// table of all of them, on the group's stream, ordered after each solve's
+// earlier work and before its later work by events. Each solve's results
+// are those of its own launches, bit for bit. The virtual sweep workgroups
+// per solve (the per-order tile) are chosen online: the first flushes of
+// each group time the candidates (any choice gives the same results).
+class KrylovBatcher {
+public:
+  explicit KrylovBatcher(double window_us = 150.);
+  ~KrylovBatcher();
+  KrylovBatcher(const KrylovBatcher &) = delete;
+  KrylovBatcher &operator=(const KrylovBatcher &) = delete;
+  // A solve enters / leaves its Krylov iteration (a flush waits for the
+  // steps of the solves inside one, up to the window).
+  void Enter();
+  void Leave();
+  // Hands over Arnoldi step j of s; returns once the step is launched (its
+  // completion is then recorded in s.step_done[j % 2] on s's stream).
+  template <class T> void Submit(LinearSolver &s, T *V, T *z, T *w, const T *metric, int j);
+  BatchReport Report();
+private:
+  struct Entry;
+  struct Group;
+  void Loop();
+  void Flush(std::vector<Entry *> &batch);
+  void Launch(Group &g, Entry *const *entries, int count);
+  Group &GroupFor(int q, bool f32);
+  std::mutex mu_;
+  std::condition_variable wake_, launched_;
+  std::vector<Entry *> queue_;
+  std::vector<std::unique_ptr<Group>> groups_;
+  int in_krylov_ = 0;
+  bool stop_ = false;
+  double window_us_;
+  bool use_graph_ = true;
+  BatchReport report_;
+  std::thread thread_;
 };
 // Requests are admitted in arrival order when a worker is free and their
 // footprint fits in the budget next to the solves already running (a request

@@ -174,4 +174,15 @@ void launch_dirk_combine(double *out, const double *base, const double *const *s
                          int count, long n, hipStream_t s);
 void launch_slope(const double *u, const double *base, double inv_dt, double *slope, long n, hipStream_t s);
 void launch_admissible(const Geo &g, const double *u, int *flags, hipStream_t s);
+
+// Grouped Arnoldi steps (GroupTable in device memory; grid: the persistent
+// grid, any size >= 1). The stages of one step in order: lower sweep, upper
+// sweep (launch_group_sweep, face blocks b = 4 q), Schur apply
+// (launch_group_schur, trace order q), CGS2 stages 0..3
+// (launch_group_krylov). False when the block size or order has no kernel.
+template <class T> bool launch_group_sweep(bool lower, int b, const GroupTable *tab, int grid, hipStream_t s);
+template <class T> int group_sweep_capacity(int b); // workgroups of the sweeps the device holds at once
+template <class T> bool launch_group_schur(int q, const GroupTable *tab, int grid, hipStream_t s);
+template <class T> void launch_group_krylov(int stage, const GroupTable *tab, int grid, hipStream_t s);
+int elementwise_grid(long n); // blocks of the elementwise kernels (k_normalize) over n entries
 } // namespace eg

@@ -641,14 +641,15 @@ __global__ void k_hdg_face(Geo g, const Frozen *__restrict__ fz, const double *_
 
 // Schur action (src/operators.cpp:147-179): y_f = D_f x_f - S(first) - S(second),
 // one 64-lane wave per face, lane = row of the face block.
-template <int Q, class T>
-__global__ __launch_bounds__(256) void k_schur_apply(Geo g, const T *__restrict__ panels, const double *__restrict__ D,
-                                                     const T *__restrict__ x, T *__restrict__ y) {
+// Faces 4 block .. 4 block + 3 of y = S x, one wave a face (g: nx, ny, nf).
+template <int Q, class T, class G>
+__device__ __forceinline__ void schur_apply_block(const G &g, int block, const T *__restrict__ panels,
+                                                  const double *__restrict__ D, const T *__restrict__ x,
+                                                  T *__restrict__ y, T (&xs)[4][2][16 * Q]) {
   constexpr int TT = 4 * Q, T4 = 16 * Q;
-  __shared__ T xs[4][2][T4];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int f = min(blockIdx.x * 4 + wid, g.nf - 1);
-  const bool valid = blockIdx.x * 4 + wid < g.nf;
+  const int f = min(block * 4 + wid, g.nf - 1);
+  const bool valid = block * 4 + wid < g.nf;
   const int e = f / 2, axis = f & 1, ls = axis ? 3 : 1;
   const int cells[2] = {e, neighbour_of(g, e, ls)};
   for (int k = lane; k < 2 * T4; k += 64) {
@@ -668,6 +669,29 @@ __global__ __launch_bounds__(256) void k_schur_apply(Geo g, const T *__restrict_
     const T dx = T(D[id]) * x[id];
     const bool left_first = cells[0] < cells[1];
     y[id] = left_first ? (dx - acc[0]) - acc[1] : (dx - acc[1]) - acc[0];
+  }
+}
+template <int Q, class T>
+__global__ __launch_bounds__(256) void k_schur_apply(Geo g, const T *__restrict__ panels, const double *__restrict__ D,
+                                                     const T *__restrict__ x, T *__restrict__ y) {
+  __shared__ T xs[4][2][16 * Q];
+  schur_apply_block<Q, T>(g, blockIdx.x, panels, D, x, y, xs);
+}
+// The Schur apply w = S z of every slot of a grouped Arnoldi step.
+struct FaceGrid {
+  int nx, ny, nf;
+};
+template <int Q, class T>
+__global__ __launch_bounds__(256) void k_group_schur(const GroupTable *__restrict__ tab) {
+  __shared__ T xs[4][2][16 * Q];
+  const int count = tab->count, total = tab->schur_off[count];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int i = group_slot(tab->schur_off, count, v);
+    const ArnoldiSlot &sl = tab->slot[i];
+    const FaceGrid g{sl.nx, sl.ny, sl.nf};
+    __syncthreads(); // xs is free again
+    schur_apply_block<Q, T>(g, v - tab->schur_off[i], static_cast<const T *>(sl.panels), sl.D,
+                            static_cast<const T *>(sl.z), static_cast<T *>(sl.w), xs);
   }
 }
 
@@ -841,6 +865,16 @@ template void launch_schur_apply<double>(const Geo &, const double *, const doub
                                          hipStream_t);
 template void launch_schur_apply<float>(const Geo &, const float *, const double *, const float *, float *,
                                         hipStream_t);
+template <class T> bool launch_group_schur(int q, const GroupTable *tab, int grid, hipStream_t s) {
+  bool ok = false;
+  EG_DISPATCH_Q(q, ({
+    k_group_schur<Q, T><<<grid, 256, 0, s>>>(tab);
+    ok = true;
+  }));
+  return ok;
+}
+template bool launch_group_schur<double>(int, const GroupTable *, int, hipStream_t);
+template bool launch_group_schur<float>(int, const GroupTable *, int, hipStream_t);
 void launch_recover(const Geo &g, const double *Xy, const double *dl, double *du, hipStream_t s) {
   EG_DISPATCH_Q(g.q, (k_recover<Q><<<g.n, 256, 0, s>>>(g, Xy, dl, du)));
 }

@@ -1456,8 +1456,8 @@ __global__ __launch_bounds__(NT) void k_ilu_upper_split(const T *__restrict__ bs
 // ticket[0] counts the tickets taken and ticket[1] the workgroups that have
 // taken their last one; the last of those resets both to zero for the next
 // launch on the stream.
-__device__ __forceinline__ void release_tickets(int *ticket) {
-  if (threadIdx.x == 0 && atomicAdd(ticket + 1, 1) == int(gridDim.x) - 1) {
+__device__ __forceinline__ void release_tickets(int *ticket, int grid) {
+  if (threadIdx.x == 0 && atomicAdd(ticket + 1, 1) == grid - 1) {
     atomicExch(ticket, 0);
     atomicExch(ticket + 1, 0);
   }
@@ -1540,20 +1540,24 @@ template <> __device__ __forceinline__ float row16_sum<float>(float v) {
   v = dpp_addf<0x141>(v);
   return dpp_addf<0x140>(v);
 }
+// Shared memory of a dataflow sweep workgroup.
+template <class T, int B, int KD> struct SweepLds {
+  T xs[KD * B];
+  T red[SweepShape<B>::NT];
+  T ts[B];
+  int next_row;
+};
+// The rows one workgroup of a dataflow sweep takes: workgroup vblock of a
+// launch of vgrid workgroups (the ticket protocol counts vgrid exits).
 template <class T, int B, int KD, bool LOWER>
-__global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__restrict__ bsr, int nb,
-                                                                    const int *__restrict__ row_ptr,
-                                                                    const int *__restrict__ diag_ptr,
-                                                                    const int *__restrict__ col, T *src, T *dst,
-                                                                    T *arm, int *flags, int spin_limit, int scaled,
-                                                                    const int *__restrict__ order, int *ticket) {
+__device__ __forceinline__ void sweep_sn_rows(const T *__restrict__ bsr, int nb, const int *__restrict__ row_ptr,
+                                              const int *__restrict__ diag_ptr, const int *__restrict__ col,
+                                              T *src, T *dst, T *arm, int *flags, int spin_limit, int scaled,
+                                              const int *__restrict__ order, int *ticket, int vblock, int vgrid,
+                                              SweepLds<T, B, KD> &S) {
   constexpr int NT = SweepShape<B>::NT, P = SweepShape<B>::P, VB = SweepShape<B>::VB;
   constexpr int KB = LOWER ? KD : KD + 1; // blocks held per row (upper: + inv(U_ii))
   constexpr size_t bb = size_t(B) * B;
-  __shared__ T xs[KD * B];
-  __shared__ T red[NT];
-  __shared__ T ts[B];
-  __shared__ int next_row;
   // 16 x 16 blocks (P == 16): the parts of a row are one DPP row of lanes
   // (part fastest) and the row's partial sums are added by DPP, with no LDS
   // round trip or workgroup barrier; otherwise rows fastest (coalesced block
@@ -1563,10 +1567,10 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
   const bool worker = part < P;
   if (tid == 0) {
     const int first = atomicAdd(ticket, 1);
-    next_row = first < nb ? order[first] : -1;
+    S.next_row = first < nb ? order[first] : -1;
   }
   if (LOWER) {
-    for (size_t t = size_t(blockIdx.x) * NT + tid; t < size_t(nb) * B; t += size_t(gridDim.x) * NT) {
+    for (size_t t = size_t(vblock) * NT + tid; t < size_t(nb) * B; t += size_t(vgrid) * NT) {
       arm[t] = sentinel<T>();
     }
   }
@@ -1576,11 +1580,11 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
   // window of tickets would hold only a few rows of one level).
   // the row index is the same in every lane: read it into a scalar register
   // (the row's pattern and block addresses then stay scalar)
-  for (int i = __builtin_amdgcn_readfirstlane(next_row); i >= 0;) {
+  for (int i = __builtin_amdgcn_readfirstlane(S.next_row); i >= 0;) {
     // With a workgroup for every row the first tickets cover all rows, so no
     // workgroup takes a second one (and none delays its row for it).
     int ahead = nb, ahead_row = -1;
-    if (tid == 0 && int(gridDim.x) < nb) { ahead = atomicAdd(ticket, 1); }
+    if (tid == 0 && vgrid < nb) { ahead = atomicAdd(ticket, 1); }
     const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
     const int nd = j1 - j0;
     T a[KB][VB];
@@ -1605,10 +1609,10 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
       if constexpr (RR) {
         own = row16_sum<T>(cacc); // w_i is re-armed after the barrier below (every read of it done)
       } else {
-        red[tid] = cacc;
+        S.red[tid] = cacc;
         __syncthreads(); // every read of w_i is done
         if (part == 0) {
-          for (int q = 0; q < P; ++q) { own += red[q * B + r]; }
+          for (int q = 0; q < P; ++q) { own += S.red[q * B + r]; }
           src[size_t(i) * B + r] = sentinel<T>();
         }
         __syncthreads(); // red is free again
@@ -1629,7 +1633,7 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
         __builtin_amdgcn_s_sleep(1);
         x = ld_coherent(p);
       }
-      xs[t] = x;
+      S.xs[t] = x;
     }
     __syncthreads();
     // the ticket has arrived by now: look up its row while this one is summed
@@ -1642,7 +1646,7 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
 #pragma unroll
         for (int v = 0; v < VB; ++v) {
           const int c = part + v * P;
-          if (c < B) { acc += a[d][v] * xs[d * B + c]; }
+          if (c < B) { acc += a[d][v] * S.xs[d * B + c]; }
         }
       }
     }
@@ -1650,15 +1654,15 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
     if constexpr (RR) {
       sum = row16_sum<T>(acc);
     } else {
-      red[tid] = acc;
+      S.red[tid] = acc;
       __syncthreads();
       if (part == 0) {
-        for (int q = 0; q < P; ++q) { sum += red[q * B + r]; }
+        for (int q = 0; q < P; ++q) { sum += S.red[q * B + r]; }
       }
     }
     if (part == 0) {
       if (LOWER || scaled) { st_coherent(dst + size_t(i) * B + r, own - sum); }
-      else { ts[r] = own - sum; }
+      else { S.ts[r] = own - sum; }
     }
     if (!LOWER && !scaled) {
       __syncthreads();
@@ -1666,27 +1670,62 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__r
 #pragma unroll
       for (int v = 0; v < VB; ++v) {
         const int c = part + v * P;
-        if (c < B) { acc += a[KD][v] * ts[c]; }
+        if (c < B) { acc += a[KD][v] * S.ts[c]; }
       }
       T sum2 = T(0);
       if constexpr (RR) {
         sum2 = row16_sum<T>(acc);
       } else {
-        red[tid] = acc;
+        S.red[tid] = acc;
         __syncthreads();
         if (part == 0) {
-          for (int q = 0; q < P; ++q) { sum2 += red[q * B + r]; }
+          for (int q = 0; q < P; ++q) { sum2 += S.red[q * B + r]; }
         }
       }
       if (part == 0) { st_coherent(dst + size_t(i) * B + r, sum2); }
     }
-    // every thread read next_row at the top of this row, before the barrier
+    // every thread read S.next_row at the top of this row, before the barrier
     // after the dependency wait
-    if (tid == 0) { next_row = ahead_row; }
+    if (tid == 0) { S.next_row = ahead_row; }
     __syncthreads();
-    i = __builtin_amdgcn_readfirstlane(next_row);
+    i = __builtin_amdgcn_readfirstlane(S.next_row);
   }
-  release_tickets(ticket);
+  release_tickets(ticket, vgrid);
+}
+template <class T, int B, int KD, bool LOWER>
+__global__ __launch_bounds__(SweepShape<B>::NT) void k_ilu_sweep_sn(const T *__restrict__ bsr, int nb,
+                                                                    const int *__restrict__ row_ptr,
+                                                                    const int *__restrict__ diag_ptr,
+                                                                    const int *__restrict__ col, T *src, T *dst,
+                                                                    T *arm, int *flags, int spin_limit, int scaled,
+                                                                    const int *__restrict__ order, int *ticket) {
+  __shared__ SweepLds<T, B, KD> S;
+  sweep_sn_rows<T, B, KD, LOWER>(bsr, nb, row_ptr, diag_ptr, col, src, dst, arm, flags, spin_limit, scaled, order,
+                                 ticket, blockIdx.x, gridDim.x, S);
+}
+// The sweeps of every slot of a grouped Arnoldi step (z = M^-1 v_j: lower
+// src v_j, dst mid, arm z; upper src mid, dst z). Slot i has
+// sweep_off[i+1] - sweep_off[i] virtual workgroups on its own ticket
+// counters; a workgroup of the persistent grid finishes each virtual
+// workgroup (all rows it took) before the next, so the order argument of the
+// single-solve sweep holds per slot.
+template <class T, int B, int KD, bool LOWER>
+__global__ __launch_bounds__(SweepShape<B>::NT) void k_group_sweep(const GroupTable *__restrict__ tab,
+                                                                   int spin_limit) {
+  __shared__ SweepLds<T, B, KD> S;
+  const int count = tab->count, total = tab->sweep_off[count];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int i = group_slot(tab->sweep_off, count, v);
+    const ArnoldiSlot &sl = tab->slot[i];
+    const int first = tab->sweep_off[i];
+    T *vj = static_cast<T *>(sl.V) + size_t(sl.j) * sl.N;
+    T *mid = static_cast<T *>(sl.mid), *z = static_cast<T *>(sl.z);
+    __syncthreads(); // S is free again
+    sweep_sn_rows<T, B, KD, LOWER>(static_cast<const T *>(sl.bsr), sl.nb, sl.row_ptr, sl.diag_ptr, sl.col,
+                                   LOWER ? vj : mid, LOWER ? mid : z, z, sl.flags, spin_limit,
+                                   LOWER ? 0 : sl.scaled, LOWER ? sl.lower_rows : sl.upper_rows, sl.ticket,
+                                   v - first, tab->sweep_off[i + 1] - first, S);
+  }
 }
 template <class T> __global__ void k_arm(T *v, size_t n) {
   for (size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n; t += size_t(gridDim.x) * blockDim.x) {
@@ -1832,7 +1871,7 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_stream(const T *__restrict__
     __syncthreads();
     i = __builtin_amdgcn_readfirstlane(next_row);
   }
-  release_tickets(ticket);
+  release_tickets(ticket, int(gridDim.x));
 }
 
 // Dataflow ILU(0) factor: factor_row per row, rows taken from the ticket
@@ -1863,7 +1902,7 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_df(double *bsr, int
     }
     publish_row(done, i, epoch);
   }
-  release_tickets(ticket);
+  release_tickets(ticket, int(gridDim.x));
 }
 
 // Dependencies of a row program read from the published copy of the factor
@@ -1947,7 +1986,7 @@ __global__ __launch_bounds__(Shape<B>::NT) void k_ilu_factor_resident_df(double 
     }
     if (singular && t == 0) { atomicOr(flags, kErrSingular); }
   }
-  release_tickets(ticket);
+  release_tickets(ticket, int(gridDim.x));
 }
 
 // Largest co-resident grid of a kernel (all CUs x blocks per CU).
@@ -2160,6 +2199,28 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
   }
   return known && ok;
 }
+template <class T> bool launch_group_sweep(bool lower, int b, const GroupTable *tab, int grid, hipStream_t s) {
+  bool ok = false;
+  EG_DISPATCH_B_SMALL(b, ({
+    const int spin = kSpinLimit;
+    if (lower) { k_group_sweep<T, B, 8, true><<<grid, SweepShape<B>::NT, 0, s>>>(tab, spin); }
+    else { k_group_sweep<T, B, 8, false><<<grid, SweepShape<B>::NT, 0, s>>>(tab, spin); }
+    ok = hipGetLastError() == hipSuccess;
+  }));
+  return ok;
+}
+template <class T> int group_sweep_capacity(int b) {
+  int cap = 0;
+  EG_DISPATCH_B_SMALL(b, ({
+    cap = std::min(coresident(reinterpret_cast<const void *>(k_group_sweep<T, B, 8, true>), SweepShape<B>::NT, 0),
+                   coresident(reinterpret_cast<const void *>(k_group_sweep<T, B, 8, false>), SweepShape<B>::NT, 0));
+  }));
+  return cap;
+}
+template bool launch_group_sweep<double>(bool, int, const GroupTable *, int, hipStream_t);
+template bool launch_group_sweep<float>(bool, int, const GroupTable *, int, hipStream_t);
+template int group_sweep_capacity<double>(int);
+template int group_sweep_capacity<float>(int);
 template <class T> void launch_arm(T *v, size_t n, hipStream_t s) {
   k_arm<T><<<int(std::min<size_t>((n + 255) / 256, 4096)), 256, 0, s>>>(v, n);
 }

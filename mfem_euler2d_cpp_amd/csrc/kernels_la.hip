@@ -300,14 +300,12 @@ __global__ __launch_bounds__(256) void k_block_gemv(const T *__restrict__ blocks
 constexpr int kMaxVec = 64;
 
 template <class T>
-__global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ V, long ldv, int nvec,
-                                                  const T *__restrict__ w, const T *__restrict__ metric, long n,
-                                                  double *__restrict__ partials) {
-  __shared__ double red[16 * 8];
-  __shared__ double acc_s[kMaxVec];
+__device__ __forceinline__ void multidot_block(const T *__restrict__ V, long ldv, int nvec, const T *__restrict__ w,
+                                               const T *__restrict__ metric, long n, double *__restrict__ partials,
+                                               int vblock, int vgrid, double *red, double *acc_s) {
   for (int i0 = 0; i0 < nvec; i0 += 8) {
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+    for (long k = vblock * long(blockDim.x) + threadIdx.x; k < n; k += long(vgrid) * blockDim.x) {
       const double wk = double(metric[k] * w[k]);
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
@@ -318,7 +316,15 @@ __global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ V, long 
     if (threadIdx.x < 8 && i0 + threadIdx.x < nvec) { acc_s[i0 + threadIdx.x] = s; }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nvec; i += blockDim.x) { partials[size_t(i) * gridDim.x + blockIdx.x] = acc_s[i]; }
+  for (int i = threadIdx.x; i < nvec; i += blockDim.x) { partials[size_t(i) * vgrid + vblock] = acc_s[i]; }
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_multidot(const T *__restrict__ V, long ldv, int nvec,
+                                                  const T *__restrict__ w, const T *__restrict__ metric, long n,
+                                                  double *__restrict__ partials) {
+  __shared__ double red[16 * 8];
+  __shared__ double acc_s[kMaxVec];
+  multidot_block<T>(V, ldv, nvec, w, metric, n, partials, blockIdx.x, gridDim.x, red, acc_s);
 }
 
 // Reduce partials[nvec][nb] -> proj (LDS) redundantly in every block.
@@ -337,18 +343,16 @@ __device__ void reduce_partials(const double *__restrict__ partials, int nvec, i
 // pass's partial dots with V (norm_mode 0) or the partial metric norm of w (1).
 // Every thread re-reads only the entries it wrote itself.
 template <class T>
-__global__ __launch_bounds__(256) void k_cgs_update(const T *__restrict__ V, long ldv, int nvec,
-                                                    const double *__restrict__ partials_in, T *__restrict__ w,
-                                                    const T *__restrict__ metric, long n, double *__restrict__ proj_out,
-                                                    double *__restrict__ partials_out, int norm_mode) {
-  __shared__ double proj[kMaxVec];
-  __shared__ double red[16 * 8];
-  __shared__ double acc_s[kMaxVec];
+__device__ __forceinline__ void cgs_update_block(const T *__restrict__ V, long ldv, int nvec,
+                                                 const double *__restrict__ partials_in, T *__restrict__ w,
+                                                 const T *__restrict__ metric, long n, double *__restrict__ proj_out,
+                                                 double *__restrict__ partials_out, int norm_mode, int vblock,
+                                                 int vgrid, double *proj, double *red, double *acc_s) {
   reduce_partials(partials_in, nvec, reduce_blocks(n), proj);
-  if (blockIdx.x == 0) {
+  if (vblock == 0) {
     for (int i = threadIdx.x; i < nvec; i += blockDim.x) { proj_out[i] = proj[i]; }
   }
-  const long first = blockIdx.x * long(blockDim.x) + threadIdx.x, stride = long(gridDim.x) * blockDim.x;
+  const long first = vblock * long(blockDim.x) + threadIdx.x, stride = long(vgrid) * blockDim.x;
   double nacc = 0.;
   for (long k = first; k < n; k += stride) {
     T s = T(0);
@@ -377,20 +381,30 @@ __global__ __launch_bounds__(256) void k_cgs_update(const T *__restrict__ V, lon
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nout; i += blockDim.x) { partials_out[size_t(i) * gridDim.x + blockIdx.x] = acc_s[i]; }
+  for (int i = threadIdx.x; i < nout; i += blockDim.x) { partials_out[size_t(i) * vgrid + vblock] = acc_s[i]; }
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_cgs_update(const T *__restrict__ V, long ldv, int nvec,
+                                                    const double *__restrict__ partials_in, T *__restrict__ w,
+                                                    const T *__restrict__ metric, long n, double *__restrict__ proj_out,
+                                                    double *__restrict__ partials_out, int norm_mode) {
+  __shared__ double proj[kMaxVec];
+  __shared__ double red[16 * 8];
+  __shared__ double acc_s[kMaxVec];
+  cgs_update_block<T>(V, ldv, nvec, partials_in, w, metric, n, proj_out, partials_out, norm_mode, blockIdx.x,
+                      gridDim.x, proj, red, acc_s);
 }
 
 // next = sqrt(sum partials); vnext = w * (1/next) if next > 0 (src/linear.cpp:117-122).
 template <class T>
-__global__ __launch_bounds__(256) void k_normalize(const double *__restrict__ partials, T *__restrict__ w,
-                                                   T *__restrict__ vnext, long n, double *__restrict__ norm2_out,
-                                                   const double *__restrict__ proj_in, int nproj,
-                                                   double *__restrict__ host_out) {
-  __shared__ double proj[1];
+__device__ __forceinline__ void normalize_block(const double *__restrict__ partials, T *__restrict__ w,
+                                                T *__restrict__ vnext, long n, double *__restrict__ norm2_out,
+                                                const double *__restrict__ proj_in, int nproj,
+                                                double *__restrict__ host_out, int vblock, int vgrid, double *proj) {
   reduce_partials(partials, 1, reduce_blocks(n), proj);
   const double norm2 = proj[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) { norm2_out[0] = norm2; }
-  if (blockIdx.x == 0 && host_out) {
+  if (vblock == 0 && threadIdx.x == 0) { norm2_out[0] = norm2; }
+  if (vblock == 0 && host_out) {
     // the Hessenberg column: both projection passes and the squared norm
     for (int i = threadIdx.x; i < nproj; i += blockDim.x) {
       host_out[i] = proj_in[i];
@@ -401,8 +415,50 @@ __global__ __launch_bounds__(256) void k_normalize(const double *__restrict__ pa
   const double next = sqrt(norm2);
   if (next > 0) {
     const T inv = T(1 / next);
-    for (long k = blockIdx.x * long(blockDim.x) + threadIdx.x; k < n; k += long(gridDim.x) * blockDim.x) {
+    for (long k = vblock * long(blockDim.x) + threadIdx.x; k < n; k += long(vgrid) * blockDim.x) {
       vnext[k] = w[k] * inv;
+    }
+  }
+}
+template <class T>
+__global__ __launch_bounds__(256) void k_normalize(const double *__restrict__ partials, T *__restrict__ w,
+                                                   T *__restrict__ vnext, long n, double *__restrict__ norm2_out,
+                                                   const double *__restrict__ proj_in, int nproj,
+                                                   double *__restrict__ host_out) {
+  __shared__ double proj[1];
+  normalize_block<T>(partials, w, vnext, n, norm2_out, proj_in, nproj, host_out, blockIdx.x, gridDim.x, proj);
+}
+
+// The CGS2 stages of every slot of a grouped Arnoldi step (stage 0: the
+// first projection pass's partial dots, 1: w -= V h1 and the second pass's
+// dots, 2: w -= V h2 and the norm, 3: the column to pinned memory and
+// v_{j+1} = w / |w|), on each slot's own reduce_blocks(N) partials.
+template <class T>
+__global__ __launch_bounds__(256) void k_group_krylov(const GroupTable *__restrict__ tab, int stage) {
+  __shared__ double proj[kMaxVec];
+  __shared__ double red[16 * 8];
+  __shared__ double acc_s[kMaxVec];
+  const int count = tab->count;
+  const int *off = stage == 3 ? tab->norm_off : tab->red_off;
+  const int total = off[count];
+  for (int v = blockIdx.x; v < total; v += gridDim.x) {
+    const int i = group_slot(off, count, v);
+    const ArnoldiSlot &sl = tab->slot[i];
+    const int vb = v - off[i], vg = off[i + 1] - off[i];
+    const T *V = static_cast<const T *>(sl.V), *metric = static_cast<const T *>(sl.metric);
+    T *w = static_cast<T *>(sl.w);
+    const long N = sl.N;
+    const int nv = sl.j + 1;
+    __syncthreads(); // the shared arrays are free again
+    if (stage == 0) {
+      multidot_block<T>(V, N, nv, w, metric, N, sl.partA, vb, vg, red, acc_s);
+    } else if (stage == 1) {
+      cgs_update_block<T>(V, N, nv, sl.partA, w, metric, N, sl.scal, sl.partB, 0, vb, vg, proj, red, acc_s);
+    } else if (stage == 2) {
+      cgs_update_block<T>(V, N, nv, sl.partB, w, metric, N, sl.scal + 64, sl.partA, 1, vb, vg, proj, red, acc_s);
+    } else {
+      normalize_block<T>(sl.partA, w, static_cast<T *>(sl.V) + size_t(nv) * N, N, sl.scal + 128, sl.scal, nv,
+                         sl.host_col, vb, vg, proj);
     }
   }
 }
@@ -706,6 +762,12 @@ void launch_cell_norm_noalloc(const double *v, const double *metric, long n, dou
   template void launch_axpby<T>(T *, double, const T *, double, long, hipStream_t);
 EG_INST_T(double)
 EG_INST_T(float)
+int elementwise_grid(long n) { return grid_for(n); }
+template <class T> void launch_group_krylov(int stage, const GroupTable *tab, int grid, hipStream_t s) {
+  k_group_krylov<T><<<grid, 256, 0, s>>>(tab, stage);
+}
+template void launch_group_krylov<double>(int, const GroupTable *, int, hipStream_t);
+template void launch_group_krylov<float>(int, const GroupTable *, int, hipStream_t);
 template void launch_convert<double, float>(const double *, float *, long, hipStream_t);
 template void launch_convert<float, double>(const float *, double *, long, hipStream_t);
 template void launch_convert<double, double>(const double *, double *, long, hipStream_t);

@@ -29,6 +29,8 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
   V.resize(size_t(N) * (restart + 1));
   for (hipEvent_t &e : step_done) { hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"); }
   for (hipEvent_t &e : setup_mark) { hip_check(hipEventCreate(&e), "hipEventCreate"); }
+  batcher = p.batcher;
+  if (batcher) { hip_check(hipEventCreateWithFlags(&batch_ready, hipEventDisableTiming), "hipEventCreate"); }
   for (auto *v : {&w, &z, &comb, &corr, &res, &x, &act}) { v->resize(N); }
   partA.resize(size_t(64) * eg::kReduceBlocks);
   partB.resize(size_t(64) * eg::kReduceBlocks);
@@ -363,7 +365,16 @@ LinearSolver::~LinearSolver() {
   for (hipEvent_t e : setup_mark) {
     if (e) { (void)hipEventDestroy(e); }
   }
+  if (batch_ready) { (void)hipEventDestroy(batch_ready); }
 }
+bool LinearSolver::Batchable() const {
+  return batcher && form == 1 && ilu && dataflow && pattern && pattern->max_deps <= 8 && block >= 4 &&
+         block <= 36 && block % 4 == 0;
+}
+template <> const double *LinearSolver::Factors<double>() const { return blocks.p; }
+template <> const float *LinearSolver::Factors<float>() const { return blocks32.p; }
+template <> double *LinearSolver::SweepBuffer<double>() { return sweep.p; }
+template <> float *LinearSolver::SweepBuffer<float>() { return sweep32.p; }
 
 void LinearSolver::Apply(const double *xv, double *yv) {
   if (form == 0) { eg::launch_dg_apply<double>(p.geo, p.frozen, state, xv, yv, p.stream); }
@@ -429,6 +440,10 @@ void LinearSolver::Fail(const char *what) {
 // its completion.
 template <class T>
 void LinearSolver::EnqueueArnoldi(T *Vb, T *zb, T *wb, const T *metric, int j) {
+  if (Batchable()) {
+    batcher->Submit<T>(*this, Vb, zb, wb, metric, j);
+    return;
+  }
   hipStream_t s = p.stream;
   double *scal = p.scalars.p;
   Precondition(Vb + size_t(j) * N, zb);
@@ -461,6 +476,12 @@ double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iter
     return std::sqrt(p.ReduceScalar(p.partial, eg::reduce_blocks(N), 0));
   };
   auto H = [&](int i, int j) -> double & { return h[i + size_t(j) * (restart + 1)]; };
+  // the batcher's count of solves inside a Krylov iteration
+  struct Inside {
+    KrylovBatcher *b;
+    explicit Inside(KrylovBatcher *kb) : b(kb) { if (b) { b->Enter(); } }
+    ~Inside() { if (b) { b->Leave(); } }
+  } inside(Batchable() ? batcher : nullptr);
   if (zero_guess) {
     hip_check(hipMemsetAsync(xv, 0, sizeof(T) * N, s), "memset");
     hip_check(hipMemcpyAsync(rb, rhs, sizeof(T) * N, hipMemcpyDeviceToDevice, s), "copy");

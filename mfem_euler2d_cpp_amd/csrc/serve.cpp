@@ -57,7 +57,8 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
                                 ServeReport *report) {
   const int nreq = int(requests.size());
   std::vector<SolveOutcome> out(nreq);
-  require(cfg.workers >= 1 && cfg.workers <= 64, "Serve needs 1 to 64 workers");
+  require(cfg.workers >= 1 && cfg.workers <= (cfg.batched ? eg::kGroupMax : 64),
+          cfg.batched ? "The batched form needs 1 to 256 workers" : "Serve needs 1 to 64 workers");
   double budget = cfg.hbm_budget;
   if (budget <= 0.) {
     size_t free_b = 0, total_b = 0;
@@ -110,6 +111,8 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
       throw std::runtime_error("Serve could not create its worker streams");
     }
   }
+  std::unique_ptr<KrylovBatcher> batcher;
+  if (cfg.batched) { batcher = std::make_unique<KrylovBatcher>(cfg.window_us); }
   auto worker = [&](int wid) {
     hipStream_t stream = streams[wid];
     for (;;) {
@@ -132,6 +135,7 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
         try {
           EulerGas gas(requests[i].gamma);
           Problem p(opts[i], gas, stream);
+          p.batcher = batcher.get();
           RunResult res = Run(p, requests[i].form ? "hdg" : "dg", "", false, cfg.keep_states);
           r.updates = res.updates;
           r.time_steps = res.time_steps;
@@ -194,6 +198,10 @@ std::vector<SolveOutcome> Serve(const std::vector<SolveRequest> &requests, const
   }
   wake_work.notify_all();
   for (auto &th : pool) { th.join(); }
+  if (batcher) {
+    if (report) { report->batch = batcher->Report(); }
+    batcher.reset();
+  }
   for (auto &st : streams) { (void)hipStreamDestroy(st); }
   // Give the memory the solves cached in the stream-ordered pool back.
   int dev = 0;

@@ -159,6 +159,30 @@ def test_batched_draws_alone_equal_in_mix(gpu):
 
 
 @pytest.mark.gpu
+def test_grouped_arnoldi_steps_same_results(gpu):
+    """The batched form with grouped Arnoldi steps (eg_serve_batched): the
+    six instances in the mix, in mixed precision and in FP64, give bitwise
+    the results of their own launches (eg_serve), and an instance alone in
+    the batched form the same again; steps of several solves shared launches
+    (graph replays)."""
+    cases, _ = _batched_six()
+    for prec in ("mixed", "fp64"):
+        reqs = [(c + f" --precision {prec}", "hdg", 0.) for c in cases]
+        print(f"precision {prec}", flush=True)
+        plain, _, su = eg.serve(reqs, workers=6, keep_states=True)
+        grouped, rep, sg = eg.serve(reqs, workers=6, keep_states=True, batched=True)
+        b = rep["batch"]
+        assert b["flushes"] > 0 and b["graph_launches"] == b["flushes"] and b["max_group"] >= 2, b
+        assert set(b["tile"]) == {int(c.split("--k ")[1].split()[0]) for c in cases}, b
+        for i in range(len(cases)):
+            assert plain[i]["status"] == grouped[i]["status"] == 0, (grouped[i]["error"], plain[i]["error"])
+            assert plain[i]["updates"] == grouped[i]["updates"] == 24
+            assert np.array_equal(su[i], sg[i]), (cases[i], float(np.max(np.abs(su[i] - sg[i]))))
+    alone, _, sa = eg.serve([reqs[0]], workers=1, keep_states=True, batched=True)
+    assert alone[0]["status"] == 0 and np.array_equal(sa[0], sg[0])
+
+
+@pytest.mark.gpu
 def test_batched_draw_matches_oracle(gpu, oracle_lib):
     """The smallest of those instances in plain FP64 against the CPU oracle,
     and a rejected draw rejected by the oracle with the same message."""
