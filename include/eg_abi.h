@@ -184,11 +184,14 @@ int eg_serve(const char *const *args, const int *forms, const double *arrivals, 
  * long the first waiting step waits for the steps of the other solves inside
  * a Krylov iteration. batch (optional) receives the grouping counts. */
 typedef struct {
-  long long flushes;        /* grouped launches */
-  long long steps;          /* Arnoldi steps in them */
-  long long graph_launches; /* of those, graph replays */
-  int max_group;            /* most steps in one launch */
-  int tile[10];             /* per trace order q = k + 1: virtual sweep workgroups per solve, chosen online (0: none) */
+  long long flushes;        /* grouped launches (one or more graph replays each) */
+  long long chunks;         /* chunks of Arnoldi steps the solves handed over */
+  long long graph_launches; /* graph replays, one step of every solve in the flush each */
+  int max_group;            /* most solves in one flush */
+  int tile[10];             /* per trace order q = k + 1: virtual sweep workgroups per row of a solve's widest
+                               dependency level, chosen online (0: none) */
+  double launch_s;          /* host seconds the batcher spent launching */
+  double window_s;          /* host seconds it spent gathering steps */
 } eg_batch_report;
 int eg_serve_batched(const char *const *args, const int *forms, const double *arrivals, int n, int workers,
                      double hbm_budget, double window_us, eg_solve_outcome *out, double *const *states,

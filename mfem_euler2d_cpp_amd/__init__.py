@@ -51,8 +51,9 @@ class ServeReport(ctypes.Structure):
 
 
 class BatchReport(ctypes.Structure):
-    _fields_ = [("flushes", ctypes.c_longlong), ("steps", ctypes.c_longlong), ("graph_launches", ctypes.c_longlong),
-                ("max_group", ctypes.c_int), ("tile", ctypes.c_int * 10)]
+    _fields_ = [("flushes", ctypes.c_longlong), ("chunks", ctypes.c_longlong), ("graph_launches", ctypes.c_longlong),
+                ("max_group", ctypes.c_int), ("tile", ctypes.c_int * 10), ("launch_s", ctypes.c_double),
+                ("window_s", ctypes.c_double)]
 
 
 _SIGNATURES = {
@@ -324,8 +325,8 @@ def serve(requests, workers=8, hbm_budget=0.0, keep_states=False, batched=False,
                     "error": errors.raw[256 * i:256 * (i + 1)].split(b"\0", 1)[0].decode()})
     report = {"wall": rep.wall, "max_reserved": rep.max_reserved, "max_concurrent": rep.max_concurrent}
     if batched:
-        report["batch"] = {"flushes": brep.flushes, "steps": brep.steps, "graph_launches": brep.graph_launches,
-                           "max_group": brep.max_group,
+        report["batch"] = {"flushes": brep.flushes, "chunks": brep.chunks, "graph_launches": brep.graph_launches,
+                           "max_group": brep.max_group, "launch_s": brep.launch_s, "window_s": brep.window_s,
                            "tile": {q - 1: brep.tile[q] for q in range(1, 10) if brep.tile[q]}}
     return res, report, (holders if keep_states else None)
 

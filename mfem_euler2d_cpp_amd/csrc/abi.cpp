@@ -370,9 +370,11 @@ void serve_into(const char *const *args, const int *forms, const double *arrival
   if (batch) {
     *batch = eg_batch_report{};
     batch->flushes = rep.batch.flushes;
-    batch->steps = rep.batch.steps;
+    batch->chunks = rep.batch.chunks;
     batch->graph_launches = rep.batch.graph_launches;
     batch->max_group = rep.batch.max_group;
+    batch->launch_s = rep.batch.launch_s;
+    batch->window_s = rep.batch.window_s;
     for (int q = 0; q <= eg::kMaxQ; ++q) { batch->tile[q] = rep.batch.tile[q]; }
   }
 }

@@ -16,9 +16,10 @@
 
 namespace euler {
 
-// Candidates of the virtual sweep workgroups per solve (the per-order tile)
-// and the timed flushes per candidate before the choice.
-static constexpr int kTileCand[] = {2, 4, 8, 16, 32, 64};
+// Candidates of the virtual sweep workgroups per solve (the per-order tile),
+// per row of the solve's widest dependency level, and the timed flushes per
+// candidate before the choice.
+static constexpr int kTileCand[] = {1, 2, 4};
 static constexpr int kNumCand = int(sizeof(kTileCand) / sizeof(kTileCand[0]));
 static constexpr int kTrialsPerCand = 3;
 // Persistent grids of the Schur apply and the CGS2 stages (workgroups of 256
@@ -28,7 +29,8 @@ static constexpr int kWalkGrid = 2048;
 struct KrylovBatcher::Entry {
   LinearSolver *s = nullptr;
   eg::ArnoldiSlot slot{};
-  int q = 0;
+  int q = 0, chunk = 1, width = 1;
+  hipEvent_t done = nullptr; // recorded on the solve's stream after its steps
   bool f32 = false, launched = false;
   std::string error;
 };
@@ -38,8 +40,9 @@ struct KrylovBatcher::Group {
   bool f32 = false;
   hipStream_t stream = nullptr;
   eg::GroupTable *table = nullptr;        // device
-  eg::GroupTable *staging[2] = {nullptr, nullptr}; // pinned host, alternating
-  hipEvent_t staged[2] = {nullptr, nullptr}, done = nullptr, t0 = nullptr, t1 = nullptr;
+  static constexpr int kRing = 8;
+  eg::GroupTable *staging[kRing] = {};  // pinned host, in turn
+  hipEvent_t staged[kRing] = {}, done = nullptr, t0 = nullptr, t1 = nullptr;
   int next_stage = 0;
   hipGraphExec_t exec = nullptr;
   int sweep_grid = 0;
@@ -50,7 +53,10 @@ struct KrylovBatcher::Group {
   std::vector<double> cost[kNumCand];
   ~Group() {
     if (exec) { (void)hipGraphExecDestroy(exec); }
-    for (hipEvent_t e : {staged[0], staged[1], done, t0, t1}) {
+    for (hipEvent_t e : staged) {
+      if (e) { (void)hipEventDestroy(e); }
+    }
+    for (hipEvent_t e : {done, t0, t1}) {
       if (e) { (void)hipEventDestroy(e); }
     }
     for (auto *h : staging) {
@@ -64,6 +70,10 @@ struct KrylovBatcher::Group {
 KrylovBatcher::KrylovBatcher(double window_us) : window_us_(window_us) {
   const char *g = std::getenv("EG_GROUP_GRAPH");
   use_graph_ = !(g && std::string(g) == "0");
+  const char *a = std::getenv("EG_BATCH_ALIGN_US");
+  align_us_ = a ? std::atof(a) : 2000.;
+  const char *wnd = std::getenv("EG_BATCH_WINDOW_US");
+  if (wnd) { window_us_ = std::atof(wnd); }
   thread_ = std::thread([this] { Loop(); });
 }
 
@@ -78,7 +88,61 @@ KrylovBatcher::~KrylovBatcher() {
     if (g->stream) { (void)hipStreamSynchronize(g->stream); }
   }
   groups_.clear();
+  for (double *m : mirrors_) { (void)hipHostFree(m); }
   (void)hipGetLastError();
+}
+
+double *KrylovBatcher::AcquireMirror() {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (free_mirrors_.empty()) {
+    double *m = nullptr;
+    hip_check(hipHostMalloc(&m, sizeof(double) * eg::kMirrorSize, hipHostMallocDefault), "hipHostMalloc");
+    mirrors_.push_back(m);
+    return m;
+  }
+  double *m = free_mirrors_.back();
+  free_mirrors_.pop_back();
+  return m;
+}
+void KrylovBatcher::ReleaseMirror(double *m) {
+  std::lock_guard<std::mutex> lk(mu_);
+  free_mirrors_.push_back(m);
+}
+
+void KrylovBatcher::Join() {
+  std::lock_guard<std::mutex> lk(mu_);
+  ++members_;
+}
+void KrylovBatcher::Depart() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    --members_;
+  }
+  align_.notify_all();
+}
+void KrylovBatcher::Align() {
+  if (align_us_ <= 0.) { return; }
+  std::unique_lock<std::mutex> lk(mu_);
+  const long gen = align_gen_;
+  if (++arrived_ >= members_) {
+    arrived_ = 0;
+    ++align_gen_;
+    align_.notify_all();
+    return;
+  }
+  const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double, std::micro>(align_us_);
+  while (align_gen_ == gen) {
+    if (arrived_ >= members_) { // a member departed meanwhile
+      arrived_ = 0;
+      ++align_gen_;
+      align_.notify_all();
+      break;
+    }
+    if (align_.wait_until(lk, until) == std::cv_status::timeout) {
+      if (align_gen_ == gen) { --arrived_; }
+      break;
+    }
+  }
 }
 
 void KrylovBatcher::Enter() {
@@ -98,10 +162,14 @@ BatchReport KrylovBatcher::Report() {
   return report_;
 }
 
-template <class T> void KrylovBatcher::Submit(LinearSolver &s, T *V, T *z, T *w, const T *metric, int j) {
+template <class T>
+void KrylovBatcher::Submit(LinearSolver &s, T *V, T *z, T *w, const T *metric, int chunk, hipEvent_t done) {
   Entry e;
   e.s = &s;
+  e.done = done;
+  e.width = std::max(1, s.pattern->max_width);
   e.q = s.p.q;
+  e.chunk = std::max(1, chunk);
   e.f32 = sizeof(T) == 4;
   auto &sl = e.slot;
   const auto &pt = *s.pattern;
@@ -123,9 +191,9 @@ template <class T> void KrylovBatcher::Submit(LinearSolver &s, T *V, T *z, T *w,
   sl.partA = s.partA;
   sl.partB = s.partB;
   sl.scal = s.p.scalars;
-  sl.host_col = s.p.pinned + 256 * (j & 1);
+  sl.cycle = s.cycle;
+  sl.mirror = s.mirror;
   sl.N = s.N;
-  sl.j = j;
   sl.nb = pt.nb;
   sl.scaled = s.scaled_upper ? 1 : 0;
   sl.nx = s.p.opt.nx;
@@ -138,8 +206,9 @@ template <class T> void KrylovBatcher::Submit(LinearSolver &s, T *V, T *z, T *w,
   launched_.wait(lk, [&] { return e.launched; });
   if (!e.error.empty()) { throw std::runtime_error(e.error); }
 }
-template void KrylovBatcher::Submit<double>(LinearSolver &, double *, double *, double *, const double *, int);
-template void KrylovBatcher::Submit<float>(LinearSolver &, float *, float *, float *, const float *, int);
+template void KrylovBatcher::Submit<double>(LinearSolver &, double *, double *, double *, const double *, int,
+                                            hipEvent_t);
+template void KrylovBatcher::Submit<float>(LinearSolver &, float *, float *, float *, const float *, int, hipEvent_t);
 
 void KrylovBatcher::Loop() {
   std::unique_lock<std::mutex> lk(mu_);
@@ -151,6 +220,7 @@ void KrylovBatcher::Loop() {
     }
     // wait, up to the window, for the steps of the other solves inside a
     // Krylov iteration
+    const double w0 = clock_s();
     const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double, std::micro>(window_us_);
     while (!stop_ && int(queue_.size()) < in_krylov_) {
       if (wake_.wait_until(lk, until) == std::cv_status::timeout) { break; }
@@ -158,8 +228,12 @@ void KrylovBatcher::Loop() {
     std::vector<Entry *> batch;
     batch.swap(queue_);
     lk.unlock();
+    const double w1 = clock_s();
     Flush(batch);
+    const double w2 = clock_s();
     lk.lock();
+    report_.window_s += w1 - w0;
+    report_.launch_s += w2 - w1;
     for (Entry *e : batch) { e->launched = true; }
     launched_.notify_all();
   }
@@ -176,9 +250,8 @@ KrylovBatcher::Group &KrylovBatcher::GroupFor(int q, bool f32) {
   hip_check(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipMalloc(&g->table, sizeof(eg::GroupTable)), "hipMalloc");
   for (auto *&h : g->staging) { hip_check(hipHostMalloc(&h, sizeof(eg::GroupTable), hipHostMallocDefault), "hipHostMalloc"); }
-  for (hipEvent_t *e : {&g->staged[0], &g->staged[1], &g->done}) {
-    hip_check(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
-  }
+  for (hipEvent_t &e : g->staged) { hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"); }
+  hip_check(hipEventCreateWithFlags(&g->done, hipEventDisableTiming), "hipEventCreate");
   hip_check(hipEventCreate(&g->t0), "hipEventCreate");
   hip_check(hipEventCreate(&g->t1), "hipEventCreate");
   g->sweep_grid = std::max(1, f32 ? eg::group_sweep_capacity<float>(g->b) : eg::group_sweep_capacity<double>(g->b));
@@ -207,12 +280,19 @@ void KrylovBatcher::Flush(std::vector<Entry *> &batch) {
 }
 
 void KrylovBatcher::Launch(Group &g, Entry *const *entries, int count) {
-  // the tile: the timed trial of a candidate while exploring, else the choice
+  // the tile: the timed trial of a candidate while exploring, else the
+  // choice; a trial whose timing is not in yet is read at a later flush
+  // (this thread never waits for the device)
+  bool timing_free = g.timing_cand < 0;
   if (g.timing_cand >= 0) {
-    hip_check(hipEventSynchronize(g.t1), "hipEventSynchronize");
+    const hipError_t q = hipEventQuery(g.t1);
+    if (q == hipSuccess) { timing_free = true; }
+    else { (void)hipGetLastError(); }
+  }
+  if (g.timing_cand >= 0 && timing_free) {
     float ms = 0.f;
     hip_check(hipEventElapsedTime(&ms, g.t0, g.t1), "hipEventElapsedTime");
-    g.cost[g.timing_cand].push_back(double(ms) / std::max(1., g.timing_rows));
+    g.cost[g.timing_cand].push_back(double(ms) / std::max(1., g.timing_rows)); // per row and step
     g.timing_cand = -1;
     if (g.trials == kNumCand * kTrialsPerCand) {
       double best = 0.;
@@ -227,9 +307,12 @@ void KrylovBatcher::Launch(Group &g, Entry *const *entries, int count) {
       }
     }
   }
-  const bool explore = g.tile == 0;
+  const bool explore = g.tile == 0 && timing_free;
+  int replays = 1;
+  for (int i = 0; i < count; ++i) { replays = std::max(replays, entries[i]->chunk); }
   const int cand = explore ? g.trials % kNumCand : -1;
-  const int tile = explore ? kTileCand[cand] : g.tile;
+  // while a trial's timing is pending: the candidate of that trial
+  const int tile = explore ? kTileCand[cand] : (g.tile ? g.tile : kTileCand[g.timing_cand]);
   for (int i = 0; i < count; ++i) {
     // a solve whose earlier work is already done needs no wait (in the
     // steady state its stream holds only the previous grouped step)
@@ -239,16 +322,21 @@ void KrylovBatcher::Launch(Group &g, Entry *const *entries, int count) {
     hip_check(hipStreamWaitEvent(g.stream, entries[i]->s->batch_ready, 0), "hipStreamWaitEvent");
   }
   const int k = g.next_stage;
-  g.next_stage ^= 1;
+  g.next_stage = (k + 1) % Group::kRing;
   hip_check(hipEventSynchronize(g.staged[k]), "hipEventSynchronize"); // the copy that last read this buffer
   eg::GroupTable &t = *g.staging[k];
   t.count = count;
   t.sweep_off[0] = t.schur_off[0] = t.red_off[0] = t.norm_off[0] = 0;
-  double rows = 0.;
+  double rows = 0., want = 0.;
+  for (int i = 0; i < count; ++i) { want += std::min(entries[i]->slot.nb, tile * entries[i]->width); }
+  // a virtual workgroup holds its physical one for the whole sweep of its
+  // solve: keep them all resident at once (scaled down to the grid)
+  const double scale = want > g.sweep_grid ? g.sweep_grid / want : 1.;
   for (int i = 0; i < count; ++i) {
     const eg::ArnoldiSlot &sl = entries[i]->slot;
     t.slot[i] = sl;
-    t.sweep_off[i + 1] = t.sweep_off[i] + std::max(1, std::min(sl.nb, tile));
+    const int vw = int(scale * std::min(sl.nb, tile * entries[i]->width));
+    t.sweep_off[i + 1] = t.sweep_off[i] + std::max(1, vw);
     t.schur_off[i + 1] = t.schur_off[i] + (sl.nf + 3) / 4;
     t.red_off[i + 1] = t.red_off[i] + eg::reduce_blocks(sl.N);
     t.norm_off[i + 1] = t.norm_off[i] + eg::elementwise_grid(sl.N);
@@ -264,11 +352,13 @@ void KrylovBatcher::Launch(Group &g, Entry *const *entries, int count) {
       ok = ok && eg::launch_group_sweep<float>(false, g.b, g.table, g.sweep_grid, st);
       ok = ok && eg::launch_group_schur<float>(g.q, g.table, kWalkGrid, st);
       for (int stage = 0; stage < 4 && ok; ++stage) { eg::launch_group_krylov<float>(stage, g.table, kWalkGrid, st); }
+      eg::launch_group_tail(g.table, st);
     } else {
       ok = ok && eg::launch_group_sweep<double>(true, g.b, g.table, g.sweep_grid, st);
       ok = ok && eg::launch_group_sweep<double>(false, g.b, g.table, g.sweep_grid, st);
       ok = ok && eg::launch_group_schur<double>(g.q, g.table, kWalkGrid, st);
       for (int stage = 0; stage < 4 && ok; ++stage) { eg::launch_group_krylov<double>(stage, g.table, kWalkGrid, st); }
+      eg::launch_group_tail(g.table, st);
     }
     require(ok, "Launch of a grouped Arnoldi step failed");
   };
@@ -288,27 +378,30 @@ void KrylovBatcher::Launch(Group &g, Entry *const *entries, int count) {
     hip_check(e, "hipGraphInstantiate");
   }
   if (explore) { hip_check(hipEventRecord(g.t0, g.stream), "hipEventRecord"); }
-  if (use_graph_) {
-    hip_check(hipGraphLaunch(g.exec, g.stream), "hipGraphLaunch");
-  } else {
-    record(g.stream);
+  // one replay per step: a solve whose cycle has ended is skipped by every kernel
+  for (int r = 0; r < replays; ++r) {
+    if (use_graph_) {
+      hip_check(hipGraphLaunch(g.exec, g.stream), "hipGraphLaunch");
+    } else {
+      record(g.stream);
+    }
   }
   if (explore) {
     hip_check(hipEventRecord(g.t1, g.stream), "hipEventRecord");
     g.timing_cand = cand;
-    g.timing_rows = rows;
+    g.timing_rows = rows * replays;
     ++g.trials;
   }
   hip_check(hipEventRecord(g.done, g.stream), "hipEventRecord");
   for (int i = 0; i < count; ++i) {
     LinearSolver &s = *entries[i]->s;
     hip_check(hipStreamWaitEvent(s.p.stream, g.done, 0), "hipStreamWaitEvent");
-    hip_check(hipEventRecord(s.step_done[entries[i]->slot.j & 1], s.p.stream), "hipEventRecord");
+    hip_check(hipEventRecord(entries[i]->done, s.p.stream), "hipEventRecord");
   }
   std::lock_guard<std::mutex> lk(mu_);
   ++report_.flushes;
-  report_.steps += count;
-  report_.graph_launches += use_graph_ ? 1 : 0;
+  report_.chunks += count;
+  report_.graph_launches += use_graph_ ? replays : 0;
   report_.max_group = std::max(report_.max_group, count);
   if (g.q <= eg::kMaxQ) { report_.tile[g.q] = tile; }
 }

@@ -58,6 +58,23 @@ struct Frozen {
 // grid walks the virtual blocks (the launches are recorded once in a graph
 // and replayed with a new table).
 constexpr int kGroupMax = 256;
+// GMRES cycle of one solve on the device: the Arnoldi step index (count),
+// whether the cycle goes on, and the Givens rotations of the Hessenberg
+// columns so far (src/linear.cpp:100-131). The grouped step kernels read the
+// step index and skip a solve whose cycle has ended; the last kernel of each
+// step (k_group_tail) rotates the new column, decides whether the cycle goes
+// on and mirrors the column to the solve's pinned host memory.
+constexpr int kCycleMax = 64;
+struct CycleState {
+  int active, count, jmax, breakdown;
+  double target; // tol |rhs|
+  double g[kCycleMax + 1], cs[kCycleMax], sn[kCycleMax];
+};
+// Pinned mirror of a cycle, written by the device: [0] steps done, [1] cycle
+// goes on, [2] breakdown, [kMirrorG + i] rotated residual entries,
+// [kMirrorH + j (kCycleMax + 1) + i] rotated Hessenberg column j.
+constexpr int kMirrorG = 8, kMirrorH = kMirrorG + kCycleMax + 8;
+constexpr int kMirrorSize = kMirrorH + kCycleMax * (kCycleMax + 1);
 struct ArnoldiSlot {
   void *V, *z, *w;        // Krylov basis (v_j at V + j N), z = M^-1 v_j, w = S z
   const void *metric;     // trace metric
@@ -67,9 +84,11 @@ struct ArnoldiSlot {
   int *ticket, *flags;    // the sweeps' ticket counters, the solve's error flags
   const void *panels;     // Schur face panels
   const double *D;        // trace diagonal
-  double *partA, *partB, *scal, *host_col; // reduction partials, projections, pinned column
+  double *partA, *partB, *scal; // reduction partials, projections
+  CycleState *cycle;      // the solve's GMRES cycle (step index j = cycle->count)
+  double *mirror;         // its pinned mirror
   long N;
-  int j, nb, scaled, nx, ny, nf;
+  int nb, scaled, nx, ny, nf, pad;
 };
 struct GroupTable {
   int count;

@@ -130,6 +130,7 @@ private:
 struct BlockPattern {
   int nb = 0, block = 0, nnzb = 0;
   int max_deps = 0; // most off-diagonal blocks in one triangle of a row
+  int max_width = 0; // most rows in one dependency level (lower or upper)
   std::vector<int> row, col, diag, upd_ptr, upd;
   std::vector<int> lower_level_ptr, lower_rows, upper_level_ptr, upper_rows;
   DeviceArray<int> d_row, d_col, d_diag, d_upd_ptr, d_upd, d_lower_rows, d_upper_rows;
@@ -256,6 +257,9 @@ public:
   DeviceArray<double> factor_pub;
   // Ticket counters of the dataflow sweeps (two ints, zero between launches).
   DeviceArray<int> sweep_ticket;
+  // Most workgroups of a dataflow launch: EG_DF_WIDTH (default below) times
+  // the widest dependency level; 0: as many as the device holds.
+  int df_grid = 0;
   // Per-level factor launches recorded once into a graph (EG_FACTOR_GRAPH=0:
   // launched directly).
   bool use_graph = true;
@@ -279,6 +283,9 @@ public:
   // and the event marking this solve's work before a step it hands over.
   KrylovBatcher *batcher = nullptr;
   hipEvent_t batch_ready = nullptr;
+  DeviceArray<eg::CycleState> cycle; // the device GMRES cycle of the grouped steps
+  double *mirror = nullptr;          // its pinned mirror (eg::kMirrorSize doubles, from the batcher)
+  bool joined = false;               // a member of the batcher's rendezvous
   // Whether the Arnoldi steps in T can be handed to the batcher (condensed
   // system, single-launch ILU(0) sweeps of face blocks).
   bool Batchable() const;
@@ -309,6 +316,7 @@ public:
 private:
   friend class KrylovBatcher;
   template <class T> void EnqueueArnoldi(T *Vb, T *zb, T *wb, const T *metric, int j);
+  template <class T> int DeviceCycle(T *Vb, T *zb, T *wb, const T *metric, double beta, double target, int jmax);
   template <class T> double GMRES(const T *rhs, T *xv, double tol, int limit, int &iterations, bool zero_guess,
                                   double *true_norm,
                                     double known_rhs_norm = -1.);
@@ -368,11 +376,13 @@ struct ServeConfig {
   double window_us = 150.;
 };
 // Grouped Arnoldi steps of concurrent solves: counts, and per trace order
-// q (index q) the chosen virtual sweep workgroups per solve (0: none seen).
+// q (index q) the chosen virtual sweep workgroups per row of a solve's
+// widest dependency level (0: none seen).
 struct BatchReport {
-  long flushes = 0, steps = 0, graph_launches = 0;
+  long flushes = 0, chunks = 0, graph_launches = 0; // chunks of steps handed over; graph replays
   int max_group = 0;
   int tile[eg::kMaxQ + 1] = {};
+  double launch_s = 0., window_s = 0.; // the batcher thread's time launching, and gathering steps
 };
 struct ServeReport {
   double wall = 0.;
@@ -400,9 +410,21 @@ public:
   // steps of the solves inside one, up to the window).
   void Enter();
   void Leave();
-  // Hands over Arnoldi step j of s; returns once the step is launched (its
-  // completion is then recorded in s.step_done[j % 2] on s's stream).
-  template <class T> void Submit(LinearSolver &s, T *V, T *z, T *w, const T *metric, int j);
+  // Condensed solves in the batch (Join when made, Depart when done), and
+  // their rendezvous before each linear solve (up to EG_BATCH_ALIGN_US, 2000
+  // by default; 0: none), so that their Krylov iterations overlap and their
+  // steps share launches.
+  void Join();
+  void Depart();
+  void Align();
+  // Hands over at least `chunk` Arnoldi steps of s's device cycle (s.cycle;
+  // a launch may take more, up to the cycle's end); returns once they are
+  // launched, their completion then recorded in `done` on s's stream.
+  template <class T>
+  void Submit(LinearSolver &s, T *V, T *z, T *w, const T *metric, int chunk, hipEvent_t done);
+  // Pinned cycle mirrors (eg::kMirrorSize doubles), kept until the batcher ends.
+  double *AcquireMirror();
+  void ReleaseMirror(double *m);
   BatchReport Report();
 private:
   struct Entry;
@@ -415,7 +437,11 @@ private:
   std::condition_variable wake_, launched_;
   std::vector<Entry *> queue_;
   std::vector<std::unique_ptr<Group>> groups_;
-  int in_krylov_ = 0;
+  std::vector<double *> mirrors_, free_mirrors_;
+  int in_krylov_ = 0, members_ = 0, arrived_ = 0;
+  long align_gen_ = 0;
+  double align_us_ = 2000.;
+  std::condition_variable align_;
   bool stop_ = false;
   double window_us_;
   bool use_graph_ = true;

@@ -101,7 +101,7 @@ void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const i
 // as writing bsr. ticket: two counters, zero before the first launch and
 // zero again after each. False when the kernel cannot be launched.
 bool launch_ilu_factor_resident_df(double *bsr, double *pub, int b, const int *prog_off, const int *prog, int nrows,
-                                   int *ticket, size_t lds, int *flags, hipStream_t s);
+                                   int *ticket, size_t lds, int *flags, int max_grid, hipStream_t s);
 // Row-slab block products of one level for b in {64, 100, 144, 196, 256,
 // 324} (the diagonal inverses follow with launch_block_inverse).
 bool ilu_slab_ok(int b);
@@ -127,15 +127,17 @@ void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *
 // upper level lists of the pattern); ticket: two ints, zero before the first
 // launch and left zero by each launch; launches sharing them must be ordered
 // on one stream.
+// max_grid (> 0): at most that many workgroups (more than the rows that can
+// be in flight at once would only poll).
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
                          const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, bool scaled,
-                         const int *order, int *ticket, hipStream_t s);
+                         const int *order, int *ticket, int max_grid, hipStream_t s);
 template <class T> void launch_arm(T *v, size_t n, hipStream_t s);
 
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
                           const int *upd_ptr, const int *upd, int *done, int epoch, const int *order, int *ticket,
-                          int *flags, hipStream_t s);
+                          int *flags, int max_grid, hipStream_t s);
 
 // Allocation-free weighted norm sqrt(sum metric v^2) into the device double
 // out: its partial sums go to a fixed per-device array of kReduceBlocks
@@ -185,4 +187,10 @@ template <class T> int group_sweep_capacity(int b); // workgroups of the sweeps 
 template <class T> bool launch_group_schur(int q, const GroupTable *tab, int grid, hipStream_t s);
 template <class T> void launch_group_krylov(int stage, const GroupTable *tab, int grid, hipStream_t s);
 int elementwise_grid(long n); // blocks of the elementwise kernels (k_normalize) over n entries
+// The last stage of a grouped Arnoldi step: Givens rotation of each slot's
+// new Hessenberg column, the cycle's continuation and its pinned mirror.
+void launch_group_tail(const GroupTable *tab, hipStream_t s);
+// Starts a device GMRES cycle: g = (beta, 0, ...), at most jmax steps, ends
+// when |g_{j+1}| <= target.
+void launch_cycle_init(CycleState *c, double *mirror, double beta, double target, int jmax, hipStream_t s);
 } // namespace eg

@@ -688,6 +688,7 @@ __global__ __launch_bounds__(256) void k_group_schur(const GroupTable *__restric
   for (int v = blockIdx.x; v < total; v += gridDim.x) {
     const int i = group_slot(tab->schur_off, count, v);
     const ArnoldiSlot &sl = tab->slot[i];
+    if (sl.cycle->active == 0) { continue; } // the same for the whole workgroup
     const FaceGrid g{sl.nx, sl.ny, sl.nf};
     __syncthreads(); // xs is free again
     schur_apply_block<Q, T>(g, v - tab->schur_off[i], static_cast<const T *>(sl.panels), sl.D,

@@ -1717,8 +1717,9 @@ __global__ __launch_bounds__(SweepShape<B>::NT) void k_group_sweep(const GroupTa
   for (int v = blockIdx.x; v < total; v += gridDim.x) {
     const int i = group_slot(tab->sweep_off, count, v);
     const ArnoldiSlot &sl = tab->slot[i];
+    if (sl.cycle->active == 0) { continue; } // the same for the whole workgroup
     const int first = tab->sweep_off[i];
-    T *vj = static_cast<T *>(sl.V) + size_t(sl.j) * sl.N;
+    T *vj = static_cast<T *>(sl.V) + size_t(sl.cycle->count) * sl.N;
     T *mid = static_cast<T *>(sl.mid), *z = static_cast<T *>(sl.z);
     __syncthreads(); // S is free again
     sweep_sn_rows<T, B, KD, LOWER>(static_cast<const T *>(sl.bsr), sl.nb, sl.row_ptr, sl.diag_ptr, sl.col,
@@ -2161,7 +2162,8 @@ void launch_ilu_upper_level(const T *bsr, int b, const int *rows, int nrows, con
 template <class T>
 bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, const int *row_ptr,
                          const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, bool scaled,
-                         const int *order, int *ticket, hipStream_t s) {
+                         const int *order, int *ticket, int max_grid, hipStream_t s) {
+  const int limit = max_grid > 0 ? std::min(nb, max_grid) : nb;
   if (max_deps > 8) { return false; }
   const int kd = max_deps <= 4 ? 4 : 8;
   bool ok = false, known = false;
@@ -2169,7 +2171,7 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
     known = true;
     // more workgroups than the device holds at once would only wait for a slot
     const int cap = coresident(fn, threads, 0);
-    const int grid = std::max(1, std::min(nb, cap > 0 ? cap : nb));
+    const int grid = std::max(1, std::min(limit, cap > 0 ? cap : limit));
     int spin = kSpinLimit, sc = scaled ? 1 : 0;
     void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin, &sc, &order, &ticket};
     ok = hipLaunchKernel(fn, dim3(grid), dim3(threads), args, 0, s) == hipSuccess;
@@ -2190,7 +2192,7 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
                            : reinterpret_cast<const void *>(k_ilu_sweep_stream<T, false>);
     const size_t lds = (size_t(kd) * b + 1024 + b) * sizeof(T);
     const int cap = coresident(fn, 1024, lds);
-    const int grid = std::max(1, std::min(nb, cap > 0 ? cap : nb));
+    const int grid = std::max(1, std::min(limit, cap > 0 ? cap : limit));
     int spin = kSpinLimit, bv = b, kdv = kd;
     void *args[] = {&bsr, &bv, &nb, &kdv, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &flags, &spin, &order,
                     &ticket};
@@ -2226,7 +2228,7 @@ template <class T> void launch_arm(T *v, size_t n, hipStream_t s) {
 }
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
                           const int *upd_ptr, const int *upd, int *done, int epoch, const int *order, int *ticket,
-                          int *flags, hipStream_t s) {
+                          int *flags, int max_grid, hipStream_t s) {
   if (!fast_dense_ok(b)) { return false; }
   bool ok = false;
   EG_DISPATCH_B(b, ({
@@ -2234,7 +2236,7 @@ bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const 
     const size_t lds = factor_lds<B>();
     const int cap = coresident(fn, Shape<B>::NT, lds);
     if (cap > 0) {
-      const int grid = std::min(nb, cap);
+      const int grid = std::min(max_grid > 0 ? std::min(nb, max_grid) : nb, cap);
       int spin = kSpinLimit;
       void *args[] = {&bsr, &nb, &row_ptr, &diag_ptr, &col, &upd_ptr, &upd, &done, &epoch, &flags, &spin, &order,
                       &ticket};
@@ -2246,14 +2248,14 @@ bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const 
 }
 
 bool launch_ilu_factor_resident_df(double *bsr, double *pub, int b, const int *prog_off, const int *prog, int nrows,
-                                   int *ticket, size_t lds, int *flags, hipStream_t s) {
+                                   int *ticket, size_t lds, int *flags, int max_grid, hipStream_t s) {
   if (!ilu_resident_ok(b)) { return false; }
   bool ok = false;
   EG_DISPATCH_B_SMALL(b, ({
     const void *fn = reinterpret_cast<const void *>(k_ilu_factor_resident_df<B>);
     const int cap = coresident(fn, Shape<B>::NT, lds);
     if (cap > 0) {
-      const int grid = std::min(nrows, cap);
+      const int grid = std::min(max_grid > 0 ? std::min(nrows, max_grid) : nrows, cap);
       int spin = kSpinLimit;
       void *args[] = {&bsr, &pub, &prog_off, &prog, &nrows, &ticket, &flags, &spin};
       ok = hipLaunchKernel(fn, dim3(grid), dim3(Shape<B>::NT), args, lds, s) == hipSuccess;
@@ -2272,10 +2274,11 @@ template void launch_ilu_upper_level<double>(const double *, int, const int *, i
 template void launch_ilu_upper_level<float>(const float *, int, const int *, int, const int *, const int *,
                                             const int *, float *, int, bool, hipStream_t);
 template bool launch_ilu_sweep_df<double>(bool, const double *, int, int, int, const int *, const int *, const int *,
-                                          double *, double *, double *, int *, bool, const int *, int *,
+                                          double *, double *, double *, int *, bool, const int *, int *, int,
                                           hipStream_t);
 template bool launch_ilu_sweep_df<float>(bool, const float *, int, int, int, const int *, const int *, const int *,
-                                         float *, float *, float *, int *, bool, const int *, int *, hipStream_t);
+                                         float *, float *, float *, int *, bool, const int *, int *, int,
+                                         hipStream_t);
 template void launch_arm<double>(double *, size_t, hipStream_t);
 template void launch_arm<float>(float *, size_t, hipStream_t);
 } // namespace eg

@@ -444,11 +444,12 @@ __global__ __launch_bounds__(256) void k_group_krylov(const GroupTable *__restri
   for (int v = blockIdx.x; v < total; v += gridDim.x) {
     const int i = group_slot(off, count, v);
     const ArnoldiSlot &sl = tab->slot[i];
+    if (sl.cycle->active == 0) { continue; } // the same for the whole workgroup
     const int vb = v - off[i], vg = off[i + 1] - off[i];
     const T *V = static_cast<const T *>(sl.V), *metric = static_cast<const T *>(sl.metric);
     T *w = static_cast<T *>(sl.w);
     const long N = sl.N;
-    const int nv = sl.j + 1;
+    const int nv = sl.cycle->count + 1;
     __syncthreads(); // the shared arrays are free again
     if (stage == 0) {
       multidot_block<T>(V, N, nv, w, metric, N, sl.partA, vb, vg, red, acc_s);
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(256) void k_group_krylov(const GroupTable *__restri
       cgs_update_block<T>(V, N, nv, sl.partB, w, metric, N, sl.scal + 64, sl.partA, 1, vb, vg, proj, red, acc_s);
     } else {
       normalize_block<T>(sl.partA, w, static_cast<T *>(sl.V) + size_t(nv) * N, N, sl.scal + 128, sl.scal, nv,
-                         sl.host_col, vb, vg, proj);
+                         nullptr, vb, vg, proj);
     }
   }
 }
@@ -763,6 +764,73 @@ void launch_cell_norm_noalloc(const double *v, const double *metric, long n, dou
 EG_INST_T(double)
 EG_INST_T(float)
 int elementwise_grid(long n) { return grid_for(n); }
+
+// The end of a grouped Arnoldi step j for every slot whose cycle goes on
+// (one thread a slot): the Hessenberg column from the two CGS2 passes and
+// the norm, the earlier Givens rotations applied to it, its own rotation,
+// the rotated residual, and whether the cycle goes on (src/linear.cpp:
+// 117-131: |g_{j+1}| above tol |rhs|, a new direction, steps left); the
+// column and the state go to the pinned mirror.
+__global__ __launch_bounds__(64) void k_group_tail(const GroupTable *__restrict__ tab) {
+  const int count = tab->count;
+  for (int i = blockIdx.x * 64 + threadIdx.x; i < count; i += gridDim.x * 64) {
+    const ArnoldiSlot &sl = tab->slot[i];
+    CycleState &c = *sl.cycle;
+    if (c.active == 0) { continue; }
+    const int j = c.count;
+    const double *col = sl.scal;
+    double *m = sl.mirror;
+    double h[kCycleMax + 1];
+    for (int r = 0; r <= j; ++r) { h[r] = (0. + col[r]) + col[64 + r]; }
+    const double next = sqrt(col[128]);
+    h[j + 1] = next;
+    for (int r = 0; r < j; ++r) {
+      const double u = h[r], vr = h[r + 1];
+      h[r] = c.cs[r] * u + c.sn[r] * vr;
+      h[r + 1] = -c.sn[r] * u + c.cs[r] * vr;
+    }
+    const double d = hypot(h[j], h[j + 1]);
+    if (!(isfinite(d) && d > 0)) {
+      c.active = 0;
+      c.breakdown = 1;
+      m[1] = 0.;
+      m[2] = 1.;
+      continue;
+    }
+    c.cs[j] = h[j] / d;
+    c.sn[j] = h[j + 1] / d;
+    h[j] = d;
+    c.g[j + 1] = -c.sn[j] * c.g[j];
+    c.g[j] *= c.cs[j];
+    c.count = j + 1;
+    const bool more = !(fabs(c.g[j + 1]) <= c.target || next <= 2.2250738585072014e-308) && j + 1 < c.jmax;
+    c.active = more ? 1 : 0;
+    double *hc = m + kMirrorH + size_t(j) * (kCycleMax + 1);
+    for (int r = 0; r <= j; ++r) { hc[r] = h[r]; }
+    m[kMirrorG + j] = c.g[j];
+    m[kMirrorG + j + 1] = c.g[j + 1];
+    m[0] = double(j + 1);
+    m[1] = more ? 1. : 0.;
+  }
+}
+__global__ void k_cycle_init(CycleState *c, double *mirror, double beta, double target, int jmax) {
+  if (threadIdx.x == 0) {
+    c->active = jmax > 0 ? 1 : 0;
+    c->count = 0;
+    c->jmax = jmax;
+    c->breakdown = 0;
+    c->target = target;
+    c->g[0] = beta;
+    mirror[0] = 0.;
+    mirror[1] = c->active;
+    mirror[2] = 0.;
+    mirror[kMirrorG] = beta;
+  }
+}
+void launch_group_tail(const GroupTable *tab, hipStream_t s) { k_group_tail<<<kGroupMax / 64, 64, 0, s>>>(tab); }
+void launch_cycle_init(CycleState *c, double *mirror, double beta, double target, int jmax, hipStream_t s) {
+  k_cycle_init<<<1, 64, 0, s>>>(c, mirror, beta, target, jmax);
+}
 template <class T> void launch_group_krylov(int stage, const GroupTable *tab, int grid, hipStream_t s) {
   k_group_krylov<T><<<grid, 256, 0, s>>>(tab, stage);
 }

@@ -17,6 +17,12 @@
 #include "euler.hpp"
 
 namespace euler {
+// Workgroups of a dataflow ILU(0) launch per row of its widest dependency
+// level (0: as many as the device holds). More only poll; fewer leave the
+// device to concurrent solves: measured at 4, +31% updates/s for 12
+// concurrent small condensed solves and -5% / -7% of the face-block ILU(0)
+// application time alone at 16x16 Q3 / 128x128 Q6.
+constexpr int kDataflowWidth = 4;
 
 // src/linear.cpp:27-57,162-166
 LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
@@ -30,7 +36,11 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
   for (hipEvent_t &e : step_done) { hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"); }
   for (hipEvent_t &e : setup_mark) { hip_check(hipEventCreate(&e), "hipEventCreate"); }
   batcher = p.batcher;
-  if (batcher) { hip_check(hipEventCreateWithFlags(&batch_ready, hipEventDisableTiming), "hipEventCreate"); }
+  if (batcher && form == 1) {
+    hip_check(hipEventCreateWithFlags(&batch_ready, hipEventDisableTiming), "hipEventCreate");
+    cycle.resize(1);
+    mirror = batcher->AcquireMirror();
+  }
   for (auto *v : {&w, &z, &comb, &corr, &res, &x, &act}) { v->resize(N); }
   partA.resize(size_t(64) * eg::kReduceBlocks);
   partB.resize(size_t(64) * eg::kReduceBlocks);
@@ -60,6 +70,9 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     if (mixed) { sweep32.resize(N); }
     if (eg::ilu_resident_ok(block)) { PlanResident(); }
     if (resident && dataflow && resident_df) { factor_pub.resize(blocks.n); }
+    const char *width = std::getenv("EG_DF_WIDTH");
+    const int factor = width ? std::atoi(width) : kDataflowWidth;
+    df_grid = factor > 0 ? factor * std::max(1, pattern->max_width) : 0;
     sweep_ticket.resize(2);
     hip_check(hipMemsetAsync(sweep_ticket, 0, sizeof(int) * 2, p.stream), "memset");
     if (eg::big_block(block) && !eg::ilu_slab_ok(block)) { PlanBigFactor(); }
@@ -278,13 +291,13 @@ void LinearSolver::Setup(bool check_flags) {
     const auto &pt = *pattern;
     bool df = dataflow_factor && eg::launch_ilu_factor_df(blocks, b, pt.nb, pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr,
                                                    pt.d_upd, done_factor, ++epoch_factor, pt.d_lower_rows,
-                                                   done_factor.p + pt.nb, p.flags, p.stream);
+                                                   done_factor.p + pt.nb, p.flags, df_grid, p.stream);
     // row-resident programs: one launch for all levels (the rows wait on
     // each other's completion flags) when the sweeps are single-launch too
     if (!df && resident && dataflow && resident_df) {
       eg::launch_arm<double>(factor_pub, factor_pub.n, p.stream);
       df = eg::launch_ilu_factor_resident_df(blocks, factor_pub, b, d_prog_off, d_prog, pt.nb, done_factor.p + pt.nb,
-                                             resident_lds, p.flags, p.stream);
+                                             resident_lds, p.flags, df_grid, p.stream);
     }
     if (!df) {
       // The per-level launches are recorded once into a graph and replayed
@@ -366,9 +379,11 @@ LinearSolver::~LinearSolver() {
     if (e) { (void)hipEventDestroy(e); }
   }
   if (batch_ready) { (void)hipEventDestroy(batch_ready); }
+  if (mirror) { batcher->ReleaseMirror(mirror); }
+  if (joined) { batcher->Depart(); }
 }
 bool LinearSolver::Batchable() const {
-  return batcher && form == 1 && ilu && dataflow && pattern && pattern->max_deps <= 8 && block >= 4 &&
+  return batcher && mirror && form == 1 && restart <= eg::kCycleMax && ilu && dataflow && pattern && pattern->max_deps <= 8 && block >= 4 &&
          block <= 36 && block % 4 == 0;
 }
 template <> const double *LinearSolver::Factors<double>() const { return blocks.p; }
@@ -394,10 +409,10 @@ template <class T> static void precondition(LinearSolver &s, const T *bsr, const
     require(b != y, "ILU apply needs distinct input and output vectors");
     T *mid = sizeof(T) == 8 ? reinterpret_cast<T *>(s.sweep.p) : reinterpret_cast<T *>(s.sweep32.p);
     if (eg::launch_ilu_sweep_df<T>(true, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col,
-                                   const_cast<T *>(b), mid, y, s.p.flags, false, pt.d_lower_rows, s.sweep_ticket,
+                                   const_cast<T *>(b), mid, y, s.p.flags, false, pt.d_lower_rows, s.sweep_ticket, s.df_grid,
                                    s.p.stream)) {
       if (!eg::launch_ilu_sweep_df<T>(false, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col, mid,
-                                      y, y, s.p.flags, s.scaled_upper, pt.d_upper_rows, s.sweep_ticket,
+                                      y, y, s.p.flags, s.scaled_upper, pt.d_upper_rows, s.sweep_ticket, s.df_grid,
                                       s.p.stream)) {
         throw std::runtime_error("Launch of the upper ILU sweep failed");
       }
@@ -440,10 +455,6 @@ void LinearSolver::Fail(const char *what) {
 // its completion.
 template <class T>
 void LinearSolver::EnqueueArnoldi(T *Vb, T *zb, T *wb, const T *metric, int j) {
-  if (Batchable()) {
-    batcher->Submit<T>(*this, Vb, zb, wb, metric, j);
-    return;
-  }
   hipStream_t s = p.stream;
   double *scal = p.scalars.p;
   Precondition(Vb + size_t(j) * N, zb);
@@ -453,6 +464,43 @@ void LinearSolver::EnqueueArnoldi(T *Vb, T *zb, T *wb, const T *metric, int j) {
   eg::launch_cgs_update<T>(Vb, N, j + 1, partB, wb, metric, N, scal + 64, partA, 1, s);
   eg::launch_normalize<T>(partA, wb, Vb + size_t(j + 1) * N, N, scal + 128, scal, j + 1, p.pinned + 256 * (j & 1), s);
   hip_check(hipEventRecord(step_done[j & 1], s), "hipEventRecord");
+}
+
+// One GMRES cycle through the batcher (grouped steps with other solves): the
+// rotations of the Hessenberg columns run on the device (k_group_tail) with
+// the same arithmetic as the host loop of GMRES, so the host waits once per
+// handed-over chunk of steps instead of once per step. Leaves the rotated
+// columns in h and the rotated residual in rotated; returns the steps done.
+template <class T>
+int LinearSolver::DeviceCycle(T *Vb, T *zb, T *wb, const T *metric, double beta, double target, int jmax) {
+  auto H = [&](int i, int j) -> double & { return h[i + size_t(j) * (restart + 1)]; };
+  eg::launch_cycle_init(cycle, mirror, beta, target, jmax, p.stream);
+  int count = 0;
+  // (Handing a second chunk over before the first is read was measured
+  // slower: the extra flushes queue on the group's stream before other
+  // solves' steps.)
+  for (;;) {
+    // a guess of the steps left from the last contraction of |g|, 2 to 8
+    int chunk = 4;
+    if (count > 0) {
+      const double gc = std::abs(mirror[eg::kMirrorG + count]), gp = std::abs(mirror[eg::kMirrorG + count - 1]);
+      const double rate = gp > 0. ? gc / gp : 1.;
+      if (rate > 0. && rate < 1. && gc > target) {
+        chunk = int(std::ceil(std::log(target / gc) / std::log(rate)));
+      }
+    }
+    chunk = std::max(2, std::min({chunk, 8, jmax - count}));
+    batcher->Submit<T>(*this, Vb, zb, wb, metric, chunk, step_done[0]);
+    hip_check(hipEventSynchronize(step_done[0]), "hipEventSynchronize");
+    count = int(mirror[0]);
+    if (mirror[2] != 0.) { Fail("GMRES Arnoldi breakdown"); }
+    if (mirror[1] == 0.) { break; }
+  }
+  for (int j = 0; j < count; ++j) {
+    for (int i = 0; i <= j; ++i) { H(i, j) = mirror[eg::kMirrorH + size_t(j) * (eg::kCycleMax + 1) + i]; }
+  }
+  for (int i = 0; i <= count; ++i) { rotated[i] = mirror[eg::kMirrorG + i]; }
+  return count;
 }
 
 // One restarted GMRES solve in arithmetic T on the buffers of that type.
@@ -506,7 +554,12 @@ double LinearSolver::GMRES(const T *rhs, T *xv, double tol, int limit, int &iter
     rotated[0] = beta;
     int count = 0;
     bool queued = false; // step j already enqueued (ahead of the host)
-    for (int j = 0; j < restart && iterations < limit; ++j) {
+    if (Batchable()) {
+      // the cycle's steps on the device, handed to the batcher in chunks
+      count = DeviceCycle<T>(Vb, zb, wb, metric, beta, tol * rhs_norm, std::min(restart, limit - iterations));
+      iterations += count;
+    }
+    for (int j = 0; j < restart && iterations < limit && !Batchable(); ++j) {
       if (!queued) { EnqueueArnoldi(Vb, zb, wb, metric, j); }
       // Enqueue step j + 1 before waiting for step j's column when the
       // residual history says step j will not end the cycle: the device then
@@ -651,6 +704,13 @@ void LinearSolver::Solve(const double *dev_rhs, LinearResult &out, double tol, b
     out.solve = std::max(0., clock_s() - start - out.setup);
     return;
   }
+  if (Batchable()) {
+    if (!joined) {
+      batcher->Join();
+      joined = true;
+    }
+    batcher->Align();
+  }
   if (mixed) {
     true_norm = MixedSolve(dev_rhs, tol, p.opt.max_krylov, its, outer, refresh, rhs_norm);
   } else {
@@ -669,4 +729,6 @@ template void LinearSolver::EnqueueArnoldi<double>(double *, double *, double *,
 template void LinearSolver::EnqueueArnoldi<float>(float *, float *, float *, const float *, int);
 template double LinearSolver::GMRES<double>(const double *, double *, double, int, int &, bool, double *, double);
 template double LinearSolver::GMRES<float>(const float *, float *, double, int, int &, bool, double *, double);
+template int LinearSolver::DeviceCycle<double>(double *, double *, double *, const double *, double, double, int);
+template int LinearSolver::DeviceCycle<float>(float *, float *, float *, const float *, double, double, int);
 } // namespace euler

@@ -557,6 +557,9 @@ BlockPattern::BlockPattern(const std::vector<std::vector<int>> &graph, int size,
   };
   group(lev, nlev, lower_level_ptr, lower_rows);
   group(ulev, nulev, upper_level_ptr, upper_rows);
+  for (const auto *ptr : {&lower_level_ptr, &upper_level_ptr}) {
+    for (size_t l = 0; l + 1 < ptr->size(); ++l) { max_width = std::max(max_width, (*ptr)[l + 1] - (*ptr)[l]); }
+  }
   auto up = [&](DeviceArray<int> &d, const std::vector<int> &h) {
     d.resize(std::max<size_t>(h.size(), 1));
     if (!h.empty()) {

@@ -160,11 +160,12 @@ def test_batched_draws_alone_equal_in_mix(gpu):
 
 @pytest.mark.gpu
 def test_grouped_arnoldi_steps_same_results(gpu):
-    """The batched form with grouped Arnoldi steps (eg_serve_batched): the
-    six instances in the mix, in mixed precision and in FP64, give bitwise
-    the results of their own launches (eg_serve), and an instance alone in
-    the batched form the same again; steps of several solves shared launches
-    (graph replays)."""
+    """The batched form (eg_serve_batched: the Arnoldi steps of concurrent
+    condensed solves launched together as graph replays of the grouped
+    kernels, the GMRES rotations on the device): the six instances, in mixed
+    precision and in FP64, each bitwise the same alone and in the mix; the
+    same updates as their own launches (eg_serve, rotations on the host) and
+    states within 1e-12 of them; steps of several solves shared launches."""
     cases, _ = _batched_six()
     for prec in ("mixed", "fp64"):
         reqs = [(c + f" --precision {prec}", "hdg", 0.) for c in cases]
@@ -172,14 +173,14 @@ def test_grouped_arnoldi_steps_same_results(gpu):
         plain, _, su = eg.serve(reqs, workers=6, keep_states=True)
         grouped, rep, sg = eg.serve(reqs, workers=6, keep_states=True, batched=True)
         b = rep["batch"]
-        assert b["flushes"] > 0 and b["graph_launches"] == b["flushes"] and b["max_group"] >= 2, b
+        assert 0 < b["flushes"] <= b["graph_launches"] and b["max_group"] >= 2, b
         assert set(b["tile"]) == {int(c.split("--k ")[1].split()[0]) for c in cases}, b
-        for i in range(len(cases)):
+        for i, r in enumerate(reqs):
             assert plain[i]["status"] == grouped[i]["status"] == 0, (grouped[i]["error"], plain[i]["error"])
             assert plain[i]["updates"] == grouped[i]["updates"] == 24
-            assert np.array_equal(su[i], sg[i]), (cases[i], float(np.max(np.abs(su[i] - sg[i]))))
-    alone, _, sa = eg.serve([reqs[0]], workers=1, keep_states=True, batched=True)
-    assert alone[0]["status"] == 0 and np.array_equal(sa[0], sg[0])
+            assert float(np.max(np.abs(su[i] - sg[i])) / np.max(np.abs(su[i]))) < 1e-12, cases[i]
+            alone, _, sa = eg.serve([r], workers=1, keep_states=True, batched=True)
+            assert alone[0]["status"] == 0 and np.array_equal(sa[0], sg[i]), cases[i]
 
 
 @pytest.mark.gpu
