@@ -256,11 +256,14 @@ def batched_cases(count, screen, seed=0xE1E12D, rounds=6):
     return cases, rejected
 
 
-def run_batched(count, workers, seq_sample=32):
+def run_batched(count, workers, seq_sample=32, group_workers=64):
     """The batched form: `count` independent HDG mixed-precision instances
     run concurrently (stream-ordered, admission by HBM footprint), against
     the same instances in plain FP64 concurrently (the precision gain) and a
-    sample of them in plain FP64 one at a time (the batching gain)."""
+    sample of them in plain FP64 one at a time (the batching gain); and the
+    same mixed instances with their Arnoldi steps grouped across solves
+    (eg_serve_batched: graph replays of the grouped kernels over all solves
+    waiting at that moment, `group_workers` solves at once)."""
     import mfem_euler2d_cpp_amd as eg
     kof = lambda c: int(c.split("--k ")[1].split()[0])
     screen_walls = []
@@ -273,9 +276,14 @@ def run_batched(count, workers, seq_sample=32):
     cases, rejected = batched_cases(count, screen)
     out_c, rep_c, _ = eg.serve([(c, "hdg", 0.) for c in cases], workers=workers)
     out_m, rep_m, _ = eg.serve([(c + " --precision mixed", "hdg", 0.) for c in cases], workers=workers)
-    both = [i for i in range(count) if out_c[i]["status"] == 0 and out_m[i]["status"] == 0]
+    out_g, rep_g, _ = eg.serve([(c + " --precision mixed", "hdg", 0.) for c in cases], workers=group_workers,
+                               batched=True)
+    both = [i for i in range(count) if out_c[i]["status"] == 0 and out_m[i]["status"] == 0
+            and out_g[i]["status"] == 0]
     upd = lambda out: sum(out[i]["updates"] for i in both)
-    rate_m, rate_c = upd(out_m) / rep_m["wall"], upd(out_c) / rep_c["wall"]
+    rate_m, rate_c, rate_g = upd(out_m) / rep_m["wall"], upd(out_c) / rep_c["wall"], upd(out_g) / rep_g["wall"]
+    err_g = [max(abs(a - b) / max(abs(b), 1e-300) for a, b in zip(out_g[i]["errors"], out_m[i]["errors"]))
+             for i in both]
     sample = both[:seq_sample]
     out_s, rep_s, _ = eg.serve([(cases[i], "hdg", 0.) for i in sample], workers=1)
     rate_s = sum(o["updates"] for o in out_s if o["status"] == 0) / rep_s["wall"]
@@ -290,6 +298,10 @@ def run_batched(count, workers, seq_sample=32):
             "mixed_concurrent_vs_fp64_sequential": rate_m / max(rate_s, 1e-30),
             "max_rel_diff_L2_error_mixed_vs_fp64": max(err) if err else None,
             "max_concurrent": rep_m["max_concurrent"],
+            "grouped": {"workers": group_workers, "updates_per_s": rate_g, "wall_s": rep_g["wall"],
+                        "vs_own_launches_concurrent": rate_g / max(rate_m, 1e-30),
+                        "max_rel_diff_L2_error_vs_own_launches": max(err_g) if err_g else None,
+                        "max_concurrent": rep_g["max_concurrent"], **rep_g["batch"]},
             "k_histogram": {k: sum(kof(cases[i]) == k for i in both) for k in range(1, 7)},
             "screening_wall_s": screen_walls}
 
@@ -436,7 +448,7 @@ def main() -> int:
     ap.add_argument("--copy-mib", type=float, default=256.0)
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--batched", type=int, default=256, help="instances of the batched form (0: skip)")
-    ap.add_argument("--served", type=int, default=24, help="requests of the served stream sample (0: skip)")
+    ap.add_argument("--served", type=int, default=12, help="requests of the served stream sample (0: skip)")
     ap.add_argument("--workers", type=int, default=12, help="concurrent solves of the batched/served forms")
     ap.add_argument("--extras-only", action="store_true", help="run only the batched/served forms")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 64x64 Q4 / 128x128 Q6 runs")

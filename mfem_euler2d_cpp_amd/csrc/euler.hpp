@@ -244,6 +244,13 @@ public:
   int epoch_factor = 0;
   DeviceArray<double> sweep;
   DeviceArray<float> sweep32;
+  // Row-sliced sweeps (144 / 196 cell blocks with scaled upper blocks):
+  // workgroups a block row, the second intermediate vectors (the two
+  // alternate between applications) and which one is next (FP64, FP32).
+  int slices = 0;
+  DeviceArray<double> sweep_alt;
+  DeviceArray<float> sweep32_alt;
+  int parity[2] = {0, 0};
   DeviceArray<double> factor_scratch; // large blocks: one block per row (or lower block) of the widest level
   // Large-block split factor plan, per dependency level l: lower blocks
   // [a_ptr[l], a_ptr[l+1]) and updated blocks [t_ptr[l], t_ptr[l+1]).

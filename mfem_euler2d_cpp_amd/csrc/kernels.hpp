@@ -134,6 +134,16 @@ bool launch_ilu_sweep_df(bool lower, const T *bsr, int b, int nb, int max_deps, 
                          const int *diag_ptr, const int *col, T *src, T *dst, T *arm, int *flags, bool scaled,
                          const int *order, int *ticket, int max_grid, hipStream_t s);
 template <class T> void launch_arm(T *v, size_t n, hipStream_t s);
+// Row-sliced dataflow sweeps for 144 / 196 cell blocks with scaled upper
+// blocks (k_ilu_sweep_slice): each block row over `slices` workgroups. Lower:
+// src = rhs, dst = intermediate (armed), arm = final output; upper: src =
+// intermediate (complete), dst = final output, arm = the other intermediate
+// vector (narm values each). False when the kernel cannot be launched.
+bool ilu_sweep_slice_ok(int b, int max_deps);
+template <class T>
+bool launch_ilu_sweep_slice(bool lower, const T *bsr, int b, int nb, int slices, const int *row_ptr,
+                            const int *diag_ptr, const int *col, const T *src, T *dst, T *arm, long narm, int *flags,
+                            const int *order, int *ticket, int max_grid, hipStream_t s);
 
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
                           const int *upd_ptr, const int *upd, int *done, int epoch, const int *order, int *ticket,

@@ -1219,6 +1219,98 @@ __global__ __launch_bounds__(256) void k_ilu_scale_upper(double *__restrict__ bs
   }
 }
 
+// The same scaling for 144 x 144 and 196 x 196 blocks on the FP64 matrix
+// cores. Column c of inv(U_ii) U_ij needs only column c of U_ij, so a block
+// is cut into CG column groups, one 1024-thread workgroup each, which
+// overwrites its own columns in place once it has read them: the product is
+// accumulated in registers (16 x 16 output tiles, wavefront w the tiles w,
+// w + 16, ..), its operands staged through LDS in panels of 16 k (A =
+// inv(U_ii) k-major, B = the group's columns of U_ij), the next panel
+// loaded into registers while the current one is multiplied.
+template <int B> struct ScaleM {
+  static constexpr int BP = (B + 15) / 16 * 16, TN = BP / 16;
+  static constexpr int CG = B > 160 ? 2 : 1;        // column groups a block
+  static constexpr int TC = (TN + CG - 1) / CG;     // tile columns a group (at most)
+  static constexpr int TW = (TN * TC + 15) / 16;    // tiles a wavefront (at most)
+  static constexpr int KP = 16, NP = BP / KP;       // k panels
+  static constexpr int PA = KP * BP, PB = KP * 16 * TC; // values of one A / B panel
+  static constexpr int LA = (PA + 1023) / 1024, LB = (PB + 1023) / 1024;
+  static constexpr int KS = KP + 1;                 // B panel column stride (no bank conflicts across columns)
+};
+template <int B>
+__global__ __launch_bounds__(1024) void k_ilu_scale_upper_mfma(double *__restrict__ bsr,
+                                                               const int *__restrict__ slot,
+                                                               const int *__restrict__ dslot) {
+  using M = ScaleM<B>;
+  __shared__ double As[M::PA];           // [k][r]
+  __shared__ double Bs[16 * M::TC * M::KS]; // [c][k], column stride KS
+  constexpr size_t bb = size_t(B) * B;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, g = lane >> 4, j = lane & 15;
+  const int blk = blockIdx.x / M::CG, cg = blockIdx.x % M::CG;
+  const int cstart = 16 * M::TC * cg, tcols = min(M::TC, M::TN - M::TC * cg);
+  const double *D = bsr + size_t(dslot[blk]) * bb;
+  double *U = bsr + size_t(slot[blk]) * bb;
+  auto load = [&](int p, double (&av)[M::LA], double (&bv)[M::LB]) {
+#pragma unroll
+    for (int q = 0; q < M::LA; ++q) {
+      const int e = t + 1024 * q, ka = e / M::BP, ra = e % M::BP, k = p * M::KP + ka;
+      av[q] = (e < M::PA && ra < B && k < B) ? D[ra + size_t(k) * B] : 0.;
+    }
+#pragma unroll
+    for (int q = 0; q < M::LB; ++q) {
+      const int e = t + 1024 * q, cb = e / M::KP, kb = e % M::KP, k = p * M::KP + kb, c = cstart + cb;
+      bv[q] = (e < M::PB && c < B && k < B) ? U[k + size_t(c) * B] : 0.;
+    }
+  };
+  slab_d4 acc[M::TW];
+#pragma unroll
+  for (int tw = 0; tw < M::TW; ++tw) { acc[tw] = slab_d4{0., 0., 0., 0.}; }
+  double av[M::LA], bv[M::LB];
+  load(0, av, bv);
+  for (int p = 0; p < M::NP; ++p) {
+    if (p > 0) { __syncthreads(); } // every read of the previous panel is done
+#pragma unroll
+    for (int q = 0; q < M::LA; ++q) {
+      const int e = t + 1024 * q;
+      if (e < M::PA) { As[e] = av[q]; }
+    }
+#pragma unroll
+    for (int q = 0; q < M::LB; ++q) {
+      const int e = t + 1024 * q;
+      if (e < M::PB) { Bs[(e / M::KP) * M::KS + e % M::KP] = bv[q]; }
+    }
+    __syncthreads();
+    if (p + 1 < M::NP) { load(p + 1, av, bv); }
+#pragma unroll
+    for (int tw = 0; tw < M::TW; ++tw) {
+      const int tile = w + 16 * tw;
+      if (tile < M::TN * tcols) {
+        const int r0 = 16 * (tile % M::TN), c0 = 16 * (tile / M::TN);
+        slab_d4 a4 = acc[tw];
+#pragma unroll
+        for (int s4 = 0; s4 < M::KP / 4; ++s4) {
+          const int k = 4 * s4 + g;
+          a4 = __builtin_amdgcn_mfma_f64_16x16x4f64(As[k * M::BP + r0 + j], Bs[(c0 + j) * M::KS + k], a4, 0, 0, 0);
+        }
+        acc[tw] = a4;
+      }
+    }
+  }
+  __syncthreads(); // every read of this group's columns of U_ij is done
+#pragma unroll
+  for (int tw = 0; tw < M::TW; ++tw) {
+    const int tile = w + 16 * tw;
+    if (tile < M::TN * tcols) {
+      const int r0 = 16 * (tile % M::TN), c = cstart + 16 * (tile / M::TN) + j;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + g + 4 * q;
+        if (r < B && c < B) { U[r + size_t(c) * B] = acc[tw][q]; }
+      }
+    }
+  }
+}
+
 // The same products for rows whose every lower block ik has one update
 // pair, the diagonal (A_ii -= L_ik U_ki; the cell stencil), 64 x 64 blocks
 // (each wavefront one 16-column tile, one chunk): the L products of all
@@ -1875,6 +1967,113 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_stream(const T *__restrict__
   release_tickets(ticket, int(gridDim.x));
 }
 
+// Dataflow sweeps for 144 x 144 and 196 x 196 blocks with scaled upper
+// blocks, each block row split over `slices` workgroups by rows of its
+// blocks (one workgroup streams a block row at a few tens of GB/s, so the
+// level chain, not HBM, bounded the unsplit form): ticket t is slice
+// t % slices of row order[t / slices], which forms rows [r0, r0 + rs) of the
+// row's output from the whole dependency vectors, so the slices of a row
+// never wait on each other and the ticket order argument holds.
+// LOWER: y_i = b_i - sum L_ij y_j (src = b, dst = w; arms `arm`, the
+// upper sweep's output). UPPER (scaled): y_i = c_i - sum M_ij y_j with
+// c_i = inv(U_ii) w_i formed before the wait from the whole w_i (src = w,
+// complete: the lower sweep is an earlier launch), dst = y; it arms `arm`,
+// the next application's intermediate vector (the intermediate vectors
+// alternate: a slice cannot re-arm w_i while the others still read it).
+template <class T, bool LOWER>
+__global__ __launch_bounds__(1024) void k_ilu_sweep_slice(const T *__restrict__ bsr, int b, int nb, int kd,
+                                                          int slices, const int *__restrict__ row_ptr,
+                                                          const int *__restrict__ diag_ptr,
+                                                          const int *__restrict__ col, const T *src, T *dst,
+                                                          T *arm, long narm, int *flags, int spin_limit,
+                                                          const int *__restrict__ order, int *ticket) {
+  constexpr int NT = 1024, PRE = 4, KDM = 4; // cell blocks: at most 4 neighbours a triangle
+  extern __shared__ __attribute__((aligned(16))) unsigned char slice_lds[];
+  __shared__ int next_t;
+  T *xs = reinterpret_cast<T *>(slice_lds); // kd x b dependency values
+  T *red = xs + size_t(kd) * b;             // NT partial sums
+  T *ws = red + NT;                         // b: w_i (upper)
+  const int rs = (b + slices - 1) / slices, total = nb * slices;
+  const int tid = threadIdx.x, P = NT / rs, r = tid % rs, part = tid / rs;
+  const size_t bb = size_t(b) * b;
+  if (tid == 0) { next_t = atomicAdd(ticket, 1); }
+  for (size_t t = size_t(blockIdx.x) * NT + tid; t < size_t(narm); t += size_t(gridDim.x) * NT) {
+    arm[t] = sentinel<T>();
+  }
+  __syncthreads();
+  for (int tk = __builtin_amdgcn_readfirstlane(next_t); tk < total;) {
+    int ahead = total;
+    if (tid == 0 && int(gridDim.x) < total) { ahead = atomicAdd(ticket, 1); }
+    const int i = order[tk / slices], rr = (tk % slices) * rs + r;
+    const bool worker = part < P && rr < b;
+    const int j0 = LOWER ? row_ptr[i] : diag_ptr[i] + 1, j1 = LOWER ? diag_ptr[i] : row_ptr[i + 1];
+    const int nd = j1 - j0;
+    T pre[KDM][PRE];
+#pragma unroll
+    for (int d = 0; d < KDM; ++d) {
+#pragma unroll
+      for (int v = 0; v < PRE; ++v) {
+        const int c = part + v * P;
+        pre[d][v] = (d < nd && worker && c < b) ? bsr[size_t(j0 + d) * bb + size_t(c) * b + rr] : T(0);
+      }
+    }
+    T own = T(0);
+    if (LOWER) {
+      if (part == 0 && rr < b) { own = src[size_t(i) * b + rr]; }
+    } else {
+      for (int c = tid; c < b; c += NT) { ws[c] = src[size_t(i) * b + c]; }
+      __syncthreads();
+      red[tid] = worker ? stream_dot<T>(bsr + size_t(diag_ptr[i]) * bb + rr, ws, b, part, P) : T(0);
+      __syncthreads();
+      if (part == 0 && rr < b) {
+        for (int q = 0; q < P; ++q) { own += red[q * rs + r]; }
+      }
+      // red is written again only after the barrier that ends the wait
+    }
+    for (int t = tid; t < nd * b; t += NT) {
+      const T *p = dst + size_t(col[j0 + t / b]) * b + t % b;
+      T x = ld_coherent(p);
+      int spins = 0;
+      while (armed(x)) {
+        if (++spins > spin_limit) {
+          atomicOr(flags, kErrSpin);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        x = ld_coherent(p);
+      }
+      xs[t] = x;
+    }
+    __syncthreads();
+    T acc = T(0);
+    if (worker) {
+#pragma unroll
+      for (int d = 0; d < KDM; ++d) {
+        if (d < nd) {
+          const T *x = xs + d * b;
+#pragma unroll
+          for (int v = 0; v < PRE; ++v) {
+            const int c = part + v * P;
+            if (c < b) { acc += pre[d][v] * x[c]; }
+          }
+          acc += stream_dot<T>(bsr + size_t(j0 + d) * bb + rr, x, b, part + PRE * P, P);
+        }
+      }
+    }
+    red[tid] = acc;
+    __syncthreads();
+    if (part == 0 && rr < b) {
+      T sum = T(0);
+      for (int q = 0; q < P; ++q) { sum += red[q * rs + r]; }
+      st_coherent(dst + size_t(i) * b + rr, own - sum);
+    }
+    if (tid == 0) { next_t = ahead; } // read by every thread before the barrier after the wait
+    __syncthreads();
+    tk = __builtin_amdgcn_readfirstlane(next_t);
+  }
+  release_tickets(ticket, int(gridDim.x));
+}
+
 // Dataflow ILU(0) factor: factor_row per row, rows taken from the ticket
 // counter in level order (order = the rows by dependency level), waiting on
 // the rows of the lower pattern.
@@ -2093,9 +2292,17 @@ void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const i
   case 324: { constexpr int B = 324; CALL; } break;                                                \
   default: break;                                                                                  \
   }
-bool ilu_scale_upper_ok(int b) { return b % 4 == 0 && b >= 4 && (b <= 36 || b == 64); }
+bool ilu_scale_upper_ok(int b) { return b % 4 == 0 && b >= 4 && (b <= 36 || b == 64 || b == 144 || b == 196); }
 void launch_ilu_scale_upper(double *bsr, int b, const int *slot, const int *dslot, int count, hipStream_t s) {
   if (count <= 0) { return; }
+  if (b == 144) {
+    k_ilu_scale_upper_mfma<144><<<count * ScaleM<144>::CG, 1024, 0, s>>>(bsr, slot, dslot);
+    return;
+  }
+  if (b == 196) {
+    k_ilu_scale_upper_mfma<196><<<count * ScaleM<196>::CG, 1024, 0, s>>>(bsr, slot, dslot);
+    return;
+  }
   const size_t lds = 2 * size_t(b) * b * sizeof(double);
   switch (b) {
 #define EG_SC(BB)                                                                                  \
@@ -2223,6 +2430,32 @@ template bool launch_group_sweep<double>(bool, int, const GroupTable *, int, hip
 template bool launch_group_sweep<float>(bool, int, const GroupTable *, int, hipStream_t);
 template int group_sweep_capacity<double>(int);
 template int group_sweep_capacity<float>(int);
+bool ilu_sweep_slice_ok(int b, int max_deps) { return (b == 144 || b == 196) && max_deps <= 4; }
+template <class T>
+bool launch_ilu_sweep_slice(bool lower, const T *bsr, int b, int nb, int slices, const int *row_ptr,
+                            const int *diag_ptr, const int *col, const T *src, T *dst, T *arm, long narm, int *flags,
+                            const int *order, int *ticket, int max_grid, hipStream_t s) {
+  if (slices < 1 || slices > 8) { return false; }
+  const int kd = 4, total = nb * slices;
+  const void *fn = lower ? reinterpret_cast<const void *>(k_ilu_sweep_slice<T, true>)
+                         : reinterpret_cast<const void *>(k_ilu_sweep_slice<T, false>);
+  const size_t lds = (size_t(kd) * b + 1024 + b) * sizeof(T);
+  const int cap = coresident(fn, 1024, lds);
+  int grid = max_grid > 0 ? std::min(total, max_grid * slices) : total;
+  grid = std::max(1, std::min(grid, cap > 0 ? cap : grid));
+  int spin = kSpinLimit, bv = b, kdv = kd, sl = slices;
+  void *args[] = {&bsr, &bv, &nb, &kdv, &sl, &row_ptr, &diag_ptr, &col, &src, &dst, &arm, &narm, &flags, &spin,
+                  &order, &ticket};
+  const bool ok = hipLaunchKernel(fn, dim3(grid), dim3(1024), args, lds, s) == hipSuccess;
+  if (!ok) { (void)hipGetLastError(); }
+  return ok;
+}
+template bool launch_ilu_sweep_slice<double>(bool, const double *, int, int, int, const int *, const int *,
+                                             const int *, const double *, double *, double *, long, int *,
+                                             const int *, int *, int, hipStream_t);
+template bool launch_ilu_sweep_slice<float>(bool, const float *, int, int, int, const int *, const int *,
+                                            const int *, const float *, float *, float *, long, int *, const int *,
+                                            int *, int, hipStream_t);
 template <class T> void launch_arm(T *v, size_t n, hipStream_t s) {
   k_arm<T><<<int(std::min<size_t>((n + 255) / 256, 4096)), 256, 0, s>>>(v, n);
 }

@@ -23,6 +23,12 @@ namespace euler {
 // concurrent small condensed solves and -5% / -7% of the face-block ILU(0)
 // application time alone at 16x16 Q3 / 128x128 Q6.
 constexpr int kDataflowWidth = 4;
+// Workgroups a block row of the row-sliced sweeps (144 and 196 cell blocks;
+// EG_SWEEP_SLICES overrides, 0: the unsliced streamed sweeps with unscaled
+// upper blocks). Measured at 128x128 Q6 (196 blocks, FP64): one 12.97 ms,
+// two 13.0, four 8.34, eight 8.11 ms an application; 64x64 Q5: 3.19, 2.80,
+// 1.48, 1.47 ms.
+constexpr int kSweepSlices = 8;
 
 // src/linear.cpp:27-57,162-166
 LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
@@ -93,6 +99,13 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     // backward sweep, as the reference writes it).
     const char *su = std::getenv("EG_SCALED_UPPER");
     scaled_upper = eg::ilu_scale_upper_ok(block) && !(su && std::string(su) == "0");
+    if (scaled_upper && dataflow && eg::ilu_sweep_slice_ok(block, pattern->max_deps)) {
+      const char *sl = std::getenv("EG_SWEEP_SLICES");
+      slices = sl ? std::atoi(sl) : kSweepSlices;
+    }
+    // the single-launch sweeps of blocks above 100 take scaled upper blocks
+    // only in their row-sliced form (the streamed form uses U_ij as it is)
+    if (block > 100 && dataflow && slices <= 0) { scaled_upper = false; }
     if (scaled_upper) {
       const auto &pt = *pattern;
       std::vector<int> us, ud;
@@ -111,6 +124,10 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
                   "upload");
       }
       p.Sync(); // the host vectors go out of scope
+    }
+    if (slices > 0) {
+      sweep_alt.resize(N);
+      if (mixed) { sweep32_alt.resize(N); }
     }
     if (form == 0) {
       std::vector<int> ds(p.n), sl(4 * p.n);
@@ -331,6 +348,11 @@ void LinearSolver::Setup(bool check_flags) {
   if (ilu && dataflow) {
     eg::launch_arm<double>(sweep, N, p.stream);
     if (mixed) { eg::launch_arm<float>(sweep32, N, p.stream); }
+    if (slices > 0) {
+      eg::launch_arm<double>(sweep_alt, N, p.stream);
+      if (mixed) { eg::launch_arm<float>(sweep32_alt, N, p.stream); }
+      parity[0] = parity[1] = 0;
+    }
   }
   const char *context = ilu ? "Singular BSR ILU diagonal" : "Singular block Jacobi factor";
   if (check_flags) { p.CheckFlags(context); }
@@ -408,6 +430,22 @@ template <class T> static void precondition(LinearSolver &s, const T *bsr, const
   if (s.dataflow) {
     require(b != y, "ILU apply needs distinct input and output vectors");
     T *mid = sizeof(T) == 8 ? reinterpret_cast<T *>(s.sweep.p) : reinterpret_cast<T *>(s.sweep32.p);
+    if (s.slices > 0) {
+      // the intermediate vectors alternate: the upper sweep arms the next one
+      const int pi = sizeof(T) == 8 ? 0 : 1;
+      T *alt = sizeof(T) == 8 ? reinterpret_cast<T *>(s.sweep_alt.p) : reinterpret_cast<T *>(s.sweep32_alt.p);
+      T *cur = s.parity[pi] ? alt : mid, *next = s.parity[pi] ? mid : alt;
+      if (!eg::launch_ilu_sweep_slice<T>(true, bsr, s.block, pt.nb, s.slices, pt.d_row, pt.d_diag, pt.d_col, b, cur,
+                                         y, s.N, s.p.flags, pt.d_lower_rows, s.sweep_ticket, s.df_grid,
+                                         s.p.stream) ||
+          !eg::launch_ilu_sweep_slice<T>(false, bsr, s.block, pt.nb, s.slices, pt.d_row, pt.d_diag, pt.d_col, cur,
+                                         y, next, s.N, s.p.flags, pt.d_upper_rows, s.sweep_ticket, s.df_grid,
+                                         s.p.stream)) {
+        throw std::runtime_error("Launch of a row-sliced ILU sweep failed");
+      }
+      s.parity[pi] ^= 1;
+      return;
+    }
     if (eg::launch_ilu_sweep_df<T>(true, bsr, s.block, pt.nb, pt.max_deps, pt.d_row, pt.d_diag, pt.d_col,
                                    const_cast<T *>(b), mid, y, s.p.flags, false, pt.d_lower_rows, s.sweep_ticket, s.df_grid,
                                    s.p.stream)) {

@@ -350,6 +350,40 @@ def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
             assert rel(a, b) < tol, mode
 
 
+@pytest.mark.parametrize("args", ["--nx 6 --ny 5 --k 5", "--nx 5 --ny 4 --k 6"])
+@pytest.mark.parametrize("slices", ["0", "1", "3", "8"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_row_sliced_sweeps(gpu, oracle_lib, args, slices, precision, monkeypatch):
+    """DG ILU(0) application with 144 / 196 cell blocks: the row-sliced
+    dataflow sweeps (scaled upper blocks from the MFMA scaling, each block
+    row over 1, 3 or 8 workgroups) and the unsliced streamed sweeps (0,
+    unscaled), three applications in a row (the intermediate vectors
+    alternate), in FP64 and FP32 against the oracle."""
+    import torch
+    monkeypatch.setenv("EG_SWEEP_SLICES", slices)
+    base = f"{args} --precond block-ilu0"
+    print(f"case: {base} --precision {precision}, {slices} slices", flush=True)
+    o = oracle_lib.Oracle(base)  # the oracle has no precision option
+    c = gpu.Context(f"{base} --precision {precision}")
+    u = perturbed(o)
+    du = torch.tensor(u, device="cuda")
+    c.freeze(du, 1e-3)
+    o.freeze(u, 1e-3)
+    c.set_stage(du, 0.05)
+    o.set_stage(u, 0.05)
+    c.call("eg_dg_refresh", gpu._ptr(du))
+    o.dg_refresh(u)
+    c.call("eg_pc_setup", 0)
+    f32 = precision == "mixed"
+    for rep in range(3):
+        b = np.sin((0.11 + 0.07 * rep) * (np.arange(o.nu) + 1))
+        dt = torch.float32 if f32 else torch.float64
+        y = torch.zeros(o.nu, dtype=dt, device="cuda")
+        c.call("eg_pc_apply_f32" if f32 else "eg_pc_apply", 0, gpu._ptr(torch.tensor(b, dtype=dt, device="cuda")),
+               gpu._ptr(y))
+        assert rel(y.double().cpu().numpy(), o.precondition(0, 1, b)) < (2e-4 if f32 else 1e-10), rep
+
+
 @pytest.mark.parametrize("args", ["--nx 16 --ny 16 --k 3", "--nx 5 --ny 3 --k 2", "--nx 4 --ny 4 --k 4",
                                   "--nx 3 --ny 3 --k 6", "--nx 64 --ny 64 --k 1"])
 @pytest.mark.parametrize("env", [("EG_SCALED_UPPER", "0"), ("EG_SLAB", "vector"), ("EG_FACTOR_GRAPH", "0"),
