@@ -2292,9 +2292,15 @@ void launch_ilu_factor_resident(double *bsr, int b, const int *prog_off, const i
   case 324: { constexpr int B = 324; CALL; } break;                                                \
   default: break;                                                                                  \
   }
-bool ilu_scale_upper_ok(int b) { return b % 4 == 0 && b >= 4 && (b <= 36 || b == 64 || b == 144 || b == 196); }
+bool ilu_scale_upper_ok(int b) {
+  return b % 4 == 0 && b >= 4 && (b <= 36 || b == 64 || b == 100 || b == 144 || b == 196);
+}
 void launch_ilu_scale_upper(double *bsr, int b, const int *slot, const int *dslot, int count, hipStream_t s) {
   if (count <= 0) { return; }
+  if (b == 100) {
+    k_ilu_scale_upper_mfma<100><<<count * ScaleM<100>::CG, 1024, 0, s>>>(bsr, slot, dslot);
+    return;
+  }
   if (b == 144) {
     k_ilu_scale_upper_mfma<144><<<count * ScaleM<144>::CG, 1024, 0, s>>>(bsr, slot, dslot);
     return;
@@ -2430,7 +2436,7 @@ template bool launch_group_sweep<double>(bool, int, const GroupTable *, int, hip
 template bool launch_group_sweep<float>(bool, int, const GroupTable *, int, hipStream_t);
 template int group_sweep_capacity<double>(int);
 template int group_sweep_capacity<float>(int);
-bool ilu_sweep_slice_ok(int b, int max_deps) { return (b == 144 || b == 196) && max_deps <= 4; }
+bool ilu_sweep_slice_ok(int b, int max_deps) { return (b == 100 || b == 144 || b == 196) && max_deps <= 4; }
 template <class T>
 bool launch_ilu_sweep_slice(bool lower, const T *bsr, int b, int nb, int slices, const int *row_ptr,
                             const int *diag_ptr, const int *col, const T *src, T *dst, T *arm, long narm, int *flags,

@@ -23,12 +23,13 @@ namespace euler {
 // concurrent small condensed solves and -5% / -7% of the face-block ILU(0)
 // application time alone at 16x16 Q3 / 128x128 Q6.
 constexpr int kDataflowWidth = 4;
-// Workgroups a block row of the row-sliced sweeps (144 and 196 cell blocks;
-// EG_SWEEP_SLICES overrides, 0: the unsliced streamed sweeps with unscaled
-// upper blocks). Measured at 128x128 Q6 (196 blocks, FP64): one 12.97 ms,
-// two 13.0, four 8.34, eight 8.11 ms an application; 64x64 Q5: 3.19, 2.80,
-// 1.48, 1.47 ms.
-constexpr int kSweepSlices = 8;
+// Workgroups a block row of the row-sliced sweeps (100, 144 and 196 cell
+// blocks; EG_SWEEP_SLICES overrides, 0: for 100 the register-resident sweep,
+// above it the streamed sweep with unscaled upper blocks). Measured at
+// 128x128 Q6 (196 blocks, FP64): unsliced 12.97 ms, two 13.0, four 8.34,
+// eight 8.11 ms an application; 64x64 Q5: 3.19, 2.80, 1.48, 1.47 ms; 64x64
+// Q4 (100 blocks): register-resident 0.854, four 0.764, eight 1.41 ms.
+constexpr int kSweepSlices = 8, kSweepSlices100 = 4;
 
 // src/linear.cpp:27-57,162-166
 LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
@@ -101,7 +102,7 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     scaled_upper = eg::ilu_scale_upper_ok(block) && !(su && std::string(su) == "0");
     if (scaled_upper && dataflow && eg::ilu_sweep_slice_ok(block, pattern->max_deps)) {
       const char *sl = std::getenv("EG_SWEEP_SLICES");
-      slices = sl ? std::atoi(sl) : kSweepSlices;
+      slices = sl ? std::atoi(sl) : (block > 100 ? kSweepSlices : kSweepSlices100);
     }
     // the single-launch sweeps of blocks above 100 take scaled upper blocks
     // only in their row-sliced form (the streamed form uses U_ij as it is)

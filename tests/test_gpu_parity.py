@@ -350,15 +350,16 @@ def test_ilu_dataflow_matches_levels(gpu, oracle_lib, args, monkeypatch):
             assert rel(a, b) < tol, mode
 
 
-@pytest.mark.parametrize("args", ["--nx 6 --ny 5 --k 5", "--nx 5 --ny 4 --k 6"])
+@pytest.mark.parametrize("args", ["--nx 6 --ny 6 --k 4", "--nx 6 --ny 5 --k 5", "--nx 5 --ny 4 --k 6"])
 @pytest.mark.parametrize("slices", ["0", "1", "3", "8"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_row_sliced_sweeps(gpu, oracle_lib, args, slices, precision, monkeypatch):
-    """DG ILU(0) application with 144 / 196 cell blocks: the row-sliced
-    dataflow sweeps (scaled upper blocks from the MFMA scaling, each block
-    row over 1, 3 or 8 workgroups) and the unsliced streamed sweeps (0,
-    unscaled), three applications in a row (the intermediate vectors
-    alternate), in FP64 and FP32 against the oracle."""
+    """DG ILU(0) application with 100 / 144 / 196 cell blocks: the
+    row-sliced dataflow sweeps (upper blocks scaled by the MFMA kernel, each
+    block row over 1, 3 or 8 workgroups) and the unsliced ones (0: for 100
+    the register-resident sweep with scaled blocks, above 100 the streamed
+    sweep with unscaled ones), three applications in a row (the intermediate
+    vectors alternate), in FP64 and FP32 against the oracle."""
     import torch
     monkeypatch.setenv("EG_SWEEP_SLICES", slices)
     base = f"{args} --precond block-ilu0"
