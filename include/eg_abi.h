@@ -122,6 +122,13 @@ int eg_recovered_residual(eg_ctx *ctx, const double *dev_eps, double *dev_out, d
 int eg_linear_solve(eg_ctx *ctx, int form, const double *dev_rhs, double *dev_x, double *dev_res, double tol,
                     int refresh, eg_linear_stats *stats);
 
+/* The block row order of the ILU(0) factor of form 0 (cell graph) or 1 (face
+ * graph) that --ilu-ordering selects (FixedBlockILU's permutation,
+ * src/block_ilu.cpp:14-29): order[i] = the graph row factored i-th (n or nf
+ * ints); the natural order for none, the minimum discarded fill order for
+ * mdf. */
+int eg_ilu_order(eg_ctx *ctx, int form, int *order);
+
 /* Run (src/stepper.cpp:28-292) for form 0 (dg) or 1 (hdg). prefix: CSV/solution
  * output prefix or NULL. host_u_final: nu doubles. host_increments (optional):
  * up to max_updates accepted Newton increments (nu doubles each); krylov

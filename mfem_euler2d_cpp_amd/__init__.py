@@ -86,6 +86,7 @@ _SIGNATURES = {
     "eg_recovered_residual": (ctypes.c_int, [_vp, _vp, _vp, _c_double_p]),
     "eg_linear_solve": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int,
                                        ctypes.POINTER(LinearStats)]),
+    "eg_ilu_order": (ctypes.c_int, [_vp, ctypes.c_int, _c_int_p]),
     "eg_kernel_time": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, _c_double_p, _c_double_p, _c_double_p]),
     "eg_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, _c_double_p, ctypes.POINTER(RunStats),
                               _c_double_p, ctypes.c_int, _c_int_p]),
@@ -248,6 +249,14 @@ class Context:
         ms, nbytes, flops = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         self.call("eg_kernel_time", kernel.encode(), reps, ctypes.byref(ms), ctypes.byref(nbytes), ctypes.byref(flops))
         return ms.value, nbytes.value, flops.value
+
+    def ilu_order(self, form):
+        """The ILU(0) factor's block row order (eg_ilu_order)."""
+        import numpy as np
+        count = self.n if form in (0, "dg") else 2 * self.n
+        order = np.zeros(count, dtype=np.int32)
+        self.call("eg_ilu_order", 1 if form in (1, "hdg") else 0, order.ctypes.data_as(_c_int_p))
+        return order
 
     def run(self, form, prefix=None, increments=0):
         import numpy as np

@@ -185,6 +185,16 @@ int eg_recovered_residual(eg_ctx *ctx, const double *dev_eps, double *dev_out, d
     if (norm) { *norm = v; }
   });
 }
+int eg_ilu_order(eg_ctx *ctx, int form, int *order) {
+  return guard(ctx, [&] {
+    require(form == 0 || form == 1, "form must be 0 (dg) or 1 (hdg)");
+    const auto graph = ctx->problem->SolverGraph(form == 1);
+    std::vector<int> o(graph.size());
+    if (ctx->opt.ilu_ordering == "mdf") { o = MdfOrdering(graph); }
+    else { for (size_t i = 0; i < o.size(); ++i) { o[i] = int(i); } }
+    std::copy(o.begin(), o.end(), order);
+  });
+}
 int eg_linear_solve(eg_ctx *ctx, int form, const double *dev_rhs, double *dev_x, double *dev_res, double tol,
                     int refresh, eg_linear_stats *stats) {
   return guard(ctx, [&] {

@@ -135,9 +135,20 @@ struct BlockPattern {
   std::vector<int> lower_level_ptr, lower_rows, upper_level_ptr, upper_rows;
   DeviceArray<int> d_row, d_col, d_diag, d_upd_ptr, d_upd, d_lower_rows, d_upper_rows;
   DeviceArray<int> d_lower_diag; // diagonal block slot of each row of lower_rows
-  BlockPattern(const std::vector<std::vector<int>> &graph, int size, hipStream_t s);
+  // Reordered pattern (src/block_ilu.cpp:10-29): BSR row i is graph row
+  // perm[i], and inv[perm[i]] = i; both empty for the natural order.
+  std::vector<int> perm, inv;
+  DeviceArray<int> d_perm, d_inv;
+  BlockPattern(const std::vector<std::vector<int>> &graph, int size, hipStream_t s,
+               const std::vector<int> &order = {});
+  // BSR slot of block (r, c) of the graph (rows and columns in the graph's
+  // numbering, mapped through the order).
   int Find(int r, int c) const;
 };
+// Minimum discarded fill ordering of an unweighted block graph (the
+// reference's --ilu-ordering mdf, src/block_ilu.cpp:16-26): order[i] is the
+// graph row eliminated i-th.
+std::vector<int> MdfOrdering(const std::vector<std::vector<int>> &graph);
 
 class KrylovBatcher;
 
@@ -237,6 +248,23 @@ public:
   DeviceArray<float> V32, w32, z32, comb32, corr32, res32, x32, b32, blocks32;
   std::unique_ptr<BlockPattern> pattern;
   DeviceArray<int> dslot, slot, contrib;
+  // The reference's diagnostics (src/linear.cpp:46-56,184-213,224-240):
+  // assembled: the operator is the assembled block matrix (--assembly full,
+  // or element for DG; the HDG element form is the Schur panels' own
+  // action), applied by launch_bsr_mult; direct: --solver dense|direct, one
+  // dense LU with partial pivoting of the assembled matrix a set-up and a
+  // solve with it (no preconditioner, no Krylov iterations).
+  bool assembled = false, direct = false;
+  std::unique_ptr<BlockPattern> op_pattern;
+  DeviceArray<double> op_blocks;
+  DeviceArray<float> op_blocks32;
+  DeviceArray<int> op_dslot, op_slot, op_contrib;
+  DeviceArray<double> dense, dense_mult, dense_work;
+  DeviceArray<int> dense_used, dense_piv, dense_step;
+  // Reordered ILU(0) (--ilu-ordering mdf): the preconditioner's input and
+  // output in the factor's row order (src/block_ilu.cpp:87-109).
+  DeviceArray<double> perm_in, perm_out;
+  DeviceArray<float> perm_in32, perm_out32;
   // Dataflow ILU schedules (EG_ILU_SCHEDULE): the factor's per-row
   // completion flags and epoch, and the sweeps' intermediate vector, kept
   // armed with the sentinel between applications.
@@ -303,6 +331,10 @@ public:
   LinearSolver &operator=(const LinearSolver &) = delete;
   void PlanBigFactor();
   void PlanResident();
+  // BSR slots the DG / HDG block fills write for pattern pt (DG: diagonal
+  // and the four face-neighbour slots of each cell; HDG: the eight
+  // contributions of each face row).
+  void FillMaps(const BlockPattern &pt, DeviceArray<int> &ds, DeviceArray<int> &sl, DeviceArray<int> &ct);
   void FactorLevels();
   void Refresh(const double *dev_u) { state = dev_u; }
   // Fill + factor the preconditioner at the current state. check_flags

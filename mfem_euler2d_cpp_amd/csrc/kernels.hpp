@@ -45,8 +45,26 @@ void launch_cell_moments(const Geo &g, const QuadRule &r, const WaveGeo &w, cons
                          double *partials, hipStream_t s);
 // Pressure, fluxes and flux Jacobians of count states (41 doubles each).
 void launch_physics_probe(const double *states, int count, double gamma, double *out, hipStream_t s);
+// row_of (optional): BSR row of face f (a reordered pattern), else f.
 void launch_hdg_fill(const Geo &g, const double *panels, const double *D, const int *contrib, const int *row_ptr,
-                     const int *diag_ptr, double *bsr, hipStream_t s);
+                     const int *diag_ptr, double *bsr, hipStream_t s, const int *row_of = nullptr);
+
+// Assembled and direct diagnostics (kernels_diag.hip): y = A x for a BSR
+// matrix; its dense column-major N x N form; LU with partial pivoting of a
+// dense matrix without row interchanges (piv[k]: pivot row of column k; used,
+// mult: N ints / doubles of scratch; a zero pivot sets kErrSingular) and the
+// solve x = A^{-1} b from it (work: 2N doubles, step: N ints); the block
+// gather (dst(i) = src(perm(i))) or scatter (dst(perm(i)) = src(i)).
+template <class T>
+void launch_bsr_mult(const T *bsr, int b, int nb, const int *row_ptr, const int *col, const T *x, T *y,
+                     hipStream_t s);
+void launch_bsr_to_dense(const double *bsr, int b, int nb, const int *row_ptr, const int *col, double *dense,
+                         hipStream_t s);
+void launch_dense_lu(double *a, int N, int *used, int *piv, double *mult, int *flags, hipStream_t s);
+void launch_dense_solve(const double *a, int N, const int *piv, const double *b, double *x, double *work, int *step,
+                        hipStream_t s);
+template <class T>
+void launch_block_permute(const T *src, T *dst, const int *perm, int b, int nb, bool scatter, hipStream_t s);
 
 // Dense / block linear algebra.
 int la_init(); // nonzero if the large-LDS attribute could not be set (clears the error state)

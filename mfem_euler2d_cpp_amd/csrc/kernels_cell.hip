@@ -775,17 +775,18 @@ __global__ void k_hdg_diag_blocks(Geo g, const T *__restrict__ panels, const dou
 template <int Q>
 __global__ void k_hdg_fill(Geo g, const double *__restrict__ panels, const double *__restrict__ D,
                            const int *__restrict__ contrib, const int *__restrict__ row_ptr,
-                           const int *__restrict__ diag_ptr, double *__restrict__ bsr) {
+                           const int *__restrict__ diag_ptr, double *__restrict__ bsr,
+                           const int *__restrict__ row_of) {
   constexpr int TT = 4 * Q, T4 = 16 * Q;
-  const int f = blockIdx.x;
+  const int f = blockIdx.x, fr = row_of ? row_of[f] : f;
   const int e = f / 2, axis = f & 1, ls = axis ? 3 : 1;
   const int er = neighbour_of(g, e, ls);
   const int first_role = e < er ? 0 : 1;
-  const int b0 = row_ptr[f], nb = row_ptr[f + 1] - b0;
+  const int b0 = row_ptr[fr], nb = row_ptr[fr + 1] - b0;
   for (int idx = threadIdx.x; idx < TT * TT; idx += blockDim.x) {
     const int r = idx % TT, c = idx / TT;
     double acc[8];
-    for (int s = 0; s < nb; ++s) { acc[s] = (b0 + s == diag_ptr[f] && r == c) ? D[size_t(f) * TT + r] : 0.; }
+    for (int s = 0; s < nb; ++s) { acc[s] = (b0 + s == diag_ptr[fr] && r == c) ? D[size_t(f) * TT + r] : 0.; }
     for (int k = 0; k < 8; ++k) {
       const int role = k < 4 ? first_role : 1 - first_role, other = k % 4;
       const int s = contrib[f * 8 + k] - b0;
@@ -899,7 +900,7 @@ void launch_hdg_diag_blocks(const Geo &g, const double *panels, const double *D,
   EG_DISPATCH_Q(g.q, (k_hdg_diag_blocks<Q, double><<<g.nf, 256, 0, s>>>(g, panels, D, blocks)));
 }
 void launch_hdg_fill(const Geo &g, const double *panels, const double *D, const int *contrib, const int *row_ptr,
-                     const int *diag_ptr, double *bsr, hipStream_t s) {
-  EG_DISPATCH_Q(g.q, (k_hdg_fill<Q><<<g.nf, 256, 0, s>>>(g, panels, D, contrib, row_ptr, diag_ptr, bsr)));
+                     const int *diag_ptr, double *bsr, hipStream_t s, const int *row_of) {
+  EG_DISPATCH_Q(g.q, (k_hdg_fill<Q><<<g.nf, 256, 0, s>>>(g, panels, D, contrib, row_ptr, diag_ptr, bsr, row_of)));
 }
 } // namespace eg

@@ -36,7 +36,18 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     : p(prob), form(f), N(f == 0 ? prob.nu : prob.nl), block(f == 0 ? prob.m : prob.t),
       restart(std::min(N, prob.opt.restart)), mixed(prob.opt.precision == "mixed"),
       ilu(prob.opt.precond == "block-ilu0"), hybrid(hyb) {
-  require(p.opt.solver == "gmres", "Invalid solver");
+  direct = p.opt.solver == "dense" || p.opt.solver == "direct";
+  require(p.opt.solver == "gmres" || direct, "Invalid solver");
+  assembled = direct || p.opt.assembly == "full" || (form == 0 && p.opt.assembly == "element");
+  if (direct) {
+    ilu = false;
+    mixed = false;
+    // the reference's dense solver is limited to 5000 unknowns
+    // (src/linear.cpp:46-49); its direct solver (PARDISO) runs the same
+    // dense LU here and keeps the limit
+    require(N <= 5000, p.opt.solver == "dense" ? "Dense direct solve limited to 5000 unknowns"
+                                               : "Direct solve on the device (dense LU) limited to 5000 unknowns");
+  }
   require(restart + 1 <= 64, "GMRES restart above 63 is not supported by the device solver");
   require(form == 0 || hybrid != nullptr, "Schur solver needs its condensed state");
   V.resize(size_t(N) * (restart + 1));
@@ -56,8 +67,32 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     for (auto *v : {&w32, &z32, &comb32, &corr32, &res32, &x32, &b32}) { v->resize(N); }
   }
   const int nblk = N / block;
+  if (assembled) {
+    op_pattern = std::make_unique<BlockPattern>(p.SolverGraph(form == 1), block, p.stream);
+    op_blocks.resize(size_t(op_pattern->nnzb) * block * block);
+    if (mixed) { op_blocks32.resize(op_blocks.n); }
+    FillMaps(*op_pattern, op_dslot, op_slot, op_contrib);
+  }
+  if (direct) {
+    dense.resize(size_t(N) * N);
+    dense_mult.resize(N);
+    dense_work.resize(2 * size_t(N));
+    dense_used.resize(N);
+    dense_piv.resize(N);
+    dense_step.resize(N);
+  }
   if (ilu) {
-    pattern = std::make_unique<BlockPattern>(p.SolverGraph(form == 1), block, p.stream);
+    const auto block_graph = p.SolverGraph(form == 1);
+    pattern = std::make_unique<BlockPattern>(
+        block_graph, block, p.stream, p.opt.ilu_ordering == "mdf" ? MdfOrdering(block_graph) : std::vector<int>{});
+    if (!pattern->perm.empty()) {
+      perm_in.resize(N);
+      perm_out.resize(N);
+      if (mixed) {
+        perm_in32.resize(N);
+        perm_out32.resize(N);
+      }
+    }
     blocks.resize(size_t(pattern->nnzb) * block * block);
     // EG_ILU_SCHEDULE: "levels" (one launch per dependency level for the
     // factor and the sweeps), "dataflow" (default: single-launch sweeps,
@@ -130,45 +165,8 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
       sweep_alt.resize(N);
       if (mixed) { sweep32_alt.resize(N); }
     }
-    if (form == 0) {
-      std::vector<int> ds(p.n), sl(4 * p.n);
-      for (int e = 0; e < p.n; ++e) {
-        const int ix = e % p.opt.nx, iy = e / p.opt.nx, nx = p.opt.nx, ny = p.opt.ny;
-        const int nb[4] = {iy * nx + (ix + nx - 1) % nx, iy * nx + (ix + 1) % nx, ((iy + ny - 1) % ny) * nx + ix,
-                           ((iy + 1) % ny) * nx + ix};
-        ds[e] = pattern->Find(e, e);
-        for (int s = 0; s < 4; ++s) { sl[4 * e + s] = pattern->Find(e, nb[s]); }
-      }
-      dslot.resize(ds.size());
-      slot.resize(sl.size());
-      hip_check(hipMemcpyAsync(dslot, ds.data(), sizeof(int) * ds.size(), hipMemcpyHostToDevice, p.stream), "upload");
-      hip_check(hipMemcpyAsync(slot, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice, p.stream), "upload");
-    } else {
-      const int nx = p.opt.nx, ny = p.opt.ny;
-      auto face = [&](int e, int side) {
-        const int ix = e % nx, iy = e / nx;
-        switch (side) {
-        case 0: return 2 * (iy * nx + (ix + nx - 1) % nx);
-        case 1: return 2 * e;
-        case 2: return 2 * (((iy + ny - 1) % ny) * nx + ix) + 1;
-        default: return 2 * e + 1;
-        }
-      };
-      std::vector<int> ct(8 * p.nf);
-      for (int f = 0; f < p.nf; ++f) {
-        const int e = f / 2, ix = e % nx, iy = e / nx;
-        const int er = (f & 1) ? ((iy + 1) % ny) * nx + ix : iy * nx + (ix + 1) % nx;
-        const int first = std::min(e, er), second = std::max(e, er);
-        for (int o = 0; o < 4; ++o) {
-          ct[8 * f + o] = pattern->Find(f, face(first, o));
-          ct[8 * f + 4 + o] = pattern->Find(f, face(second, o));
-        }
-      }
-      contrib.resize(ct.size());
-      hip_check(hipMemcpyAsync(contrib, ct.data(), sizeof(int) * ct.size(), hipMemcpyHostToDevice, p.stream),
-                "upload");
-    }
-  } else {
+    FillMaps(*pattern, dslot, slot, contrib);
+  } else if (!direct) {
     blocks.resize(size_t(nblk) * block * block);
   }
   if (mixed) { blocks32.resize(blocks.n); }
@@ -178,6 +176,49 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
   rotated.assign(restart + 1, 0.);
   coefficients.assign(restart, 0.);
   p.Sync();
+}
+
+void LinearSolver::FillMaps(const BlockPattern &pt, DeviceArray<int> &ds_dev, DeviceArray<int> &sl_dev,
+                            DeviceArray<int> &ct_dev) {
+  const int nx = p.opt.nx, ny = p.opt.ny;
+  if (form == 0) {
+    std::vector<int> ds(p.n), sl(4 * p.n);
+    for (int e = 0; e < p.n; ++e) {
+      const int ix = e % nx, iy = e / nx;
+      const int nb[4] = {iy * nx + (ix + nx - 1) % nx, iy * nx + (ix + 1) % nx, ((iy + ny - 1) % ny) * nx + ix,
+                         ((iy + 1) % ny) * nx + ix};
+      ds[e] = pt.Find(e, e);
+      for (int s = 0; s < 4; ++s) { sl[4 * e + s] = pt.Find(e, nb[s]); }
+    }
+    ds_dev.resize(ds.size());
+    sl_dev.resize(sl.size());
+    hip_check(hipMemcpyAsync(ds_dev, ds.data(), sizeof(int) * ds.size(), hipMemcpyHostToDevice, p.stream), "upload");
+    hip_check(hipMemcpyAsync(sl_dev, sl.data(), sizeof(int) * sl.size(), hipMemcpyHostToDevice, p.stream), "upload");
+    p.Sync(); // the host vectors go out of scope
+    return;
+  }
+  auto face = [&](int e, int side) {
+    const int ix = e % nx, iy = e / nx;
+    switch (side) {
+    case 0: return 2 * (iy * nx + (ix + nx - 1) % nx);
+    case 1: return 2 * e;
+    case 2: return 2 * (((iy + ny - 1) % ny) * nx + ix) + 1;
+    default: return 2 * e + 1;
+    }
+  };
+  std::vector<int> ct(8 * p.nf);
+  for (int f = 0; f < p.nf; ++f) {
+    const int e = f / 2, ix = e % nx, iy = e / nx;
+    const int er = (f & 1) ? ((iy + 1) % ny) * nx + ix : iy * nx + (ix + 1) % nx;
+    const int first = std::min(e, er), second = std::max(e, er);
+    for (int o = 0; o < 4; ++o) {
+      ct[8 * f + o] = pt.Find(f, face(first, o));
+      ct[8 * f + 4 + o] = pt.Find(f, face(second, o));
+    }
+  }
+  ct_dev.resize(ct.size());
+  hip_check(hipMemcpyAsync(ct_dev, ct.data(), sizeof(int) * ct.size(), hipMemcpyHostToDevice, p.stream), "upload");
+  p.Sync(); // the host vector goes out of scope
 }
 
 // Large blocks: one workgroup per row is too few at a dependency level, so
@@ -290,8 +331,27 @@ void LinearSolver::PlanResident() {
 // src/operators.cpp:48-74,269-303; src/block_ilu.cpp:73-86).
 void LinearSolver::Setup(bool check_flags) {
   const int b = block;
+  if (form == 0) { require(state != nullptr, "DG solver has no refreshed state"); }
+  if (assembled) {
+    // src/linear.cpp:184-187: the assembled operator at the current state
+    const auto &op = *op_pattern;
+    if (form == 0) {
+      eg::launch_dg_blocks(p.geo, p.frozen, state, op_blocks, op_dslot, op_blocks, op_slot, p.stream);
+    } else {
+      eg::launch_hdg_fill(p.geo, hybrid->panels, hybrid->D, op_contrib, op.d_row, op.d_diag, op_blocks, p.stream);
+    }
+    if (mixed) { eg::launch_convert<double, float>(op_blocks, op_blocks32, op_blocks.n, p.stream); }
+  }
+  if (direct) {
+    // src/linear.cpp:204-206: the dense matrix and its LU
+    const auto &op = *op_pattern;
+    eg::launch_bsr_to_dense(op_blocks, b, op.nb, op.d_row, op.d_col, dense, p.stream);
+    eg::launch_dense_lu(dense, N, dense_used, dense_piv, dense_mult, p.flags, p.stream);
+    if (check_flags) { p.CheckFlags("Singular dense LU factor"); }
+    else { pending_check = "Singular dense LU factor"; }
+    return;
+  }
   if (form == 0) {
-    require(state != nullptr, "DG solver has no refreshed state");
     if (ilu) {
       eg::launch_dg_blocks(p.geo, p.frozen, state, blocks, dslot, blocks, slot, p.stream);
     } else {
@@ -300,7 +360,7 @@ void LinearSolver::Setup(bool check_flags) {
   } else {
     if (ilu) {
       eg::launch_hdg_fill(p.geo, hybrid->panels, hybrid->D, contrib, pattern->d_row, pattern->d_diag, blocks,
-                          p.stream);
+                          p.stream, pattern->perm.empty() ? nullptr : pattern->d_inv.p);
     } else {
       eg::launch_hdg_diag_blocks(p.geo, hybrid->panels, hybrid->D, blocks, p.stream);
     }
@@ -406,8 +466,9 @@ LinearSolver::~LinearSolver() {
   if (joined) { batcher->Depart(); }
 }
 bool LinearSolver::Batchable() const {
-  return batcher && mirror && form == 1 && restart <= eg::kCycleMax && ilu && dataflow && pattern && pattern->max_deps <= 8 && block >= 4 &&
-         block <= 36 && block % 4 == 0;
+  return batcher && mirror && form == 1 && restart <= eg::kCycleMax && ilu && dataflow && pattern &&
+         pattern->perm.empty() && !assembled && pattern->max_deps <= 8 && block >= 4 && block <= 36 &&
+         block % 4 == 0;
 }
 template <> const double *LinearSolver::Factors<double>() const { return blocks.p; }
 template <> const float *LinearSolver::Factors<float>() const { return blocks32.p; }
@@ -415,14 +476,40 @@ template <> double *LinearSolver::SweepBuffer<double>() { return sweep.p; }
 template <> float *LinearSolver::SweepBuffer<float>() { return sweep32.p; }
 
 void LinearSolver::Apply(const double *xv, double *yv) {
-  if (form == 0) { eg::launch_dg_apply<double>(p.geo, p.frozen, state, xv, yv, p.stream); }
-  else { hybrid->SchurMult(xv, yv); }
+  if (assembled) {
+    const auto &op = *op_pattern;
+    eg::launch_bsr_mult<double>(op_blocks, block, op.nb, op.d_row, op.d_col, xv, yv, p.stream);
+  } else if (form == 0) {
+    eg::launch_dg_apply<double>(p.geo, p.frozen, state, xv, yv, p.stream);
+  } else {
+    hybrid->SchurMult(xv, yv);
+  }
 }
 void LinearSolver::Apply(const float *xv, float *yv) {
-  if (form == 0) { eg::launch_dg_apply<float>(p.geo, p.frozen, state, xv, yv, p.stream); }
-  else { hybrid->SchurMult(xv, yv); }
+  if (assembled) {
+    const auto &op = *op_pattern;
+    eg::launch_bsr_mult<float>(op_blocks32, block, op.nb, op.d_row, op.d_col, xv, yv, p.stream);
+  } else if (form == 0) {
+    eg::launch_dg_apply<float>(p.geo, p.frozen, state, xv, yv, p.stream);
+  } else {
+    hybrid->SchurMult(xv, yv);
+  }
 }
+template <class T> static void sweeps(LinearSolver &s, const T *bsr, const T *b, T *y);
+// src/block_ilu.cpp:87-109: a reordered factor takes its input in its own
+// row order (b(perm(i))) and returns y(perm(i)).
 template <class T> static void precondition(LinearSolver &s, const T *bsr, const T *b, T *y) {
+  if (s.ilu && !s.pattern->perm.empty()) {
+    T *in = sizeof(T) == 8 ? reinterpret_cast<T *>(s.perm_in.p) : reinterpret_cast<T *>(s.perm_in32.p);
+    T *out = sizeof(T) == 8 ? reinterpret_cast<T *>(s.perm_out.p) : reinterpret_cast<T *>(s.perm_out32.p);
+    eg::launch_block_permute<T>(b, in, s.pattern->d_perm, s.block, s.pattern->nb, false, s.p.stream);
+    sweeps<T>(s, bsr, in, out);
+    eg::launch_block_permute<T>(out, y, s.pattern->d_perm, s.block, s.pattern->nb, true, s.p.stream);
+    return;
+  }
+  sweeps<T>(s, bsr, b, y);
+}
+template <class T> static void sweeps(LinearSolver &s, const T *bsr, const T *b, T *y) {
   if (!s.ilu) {
     eg::launch_block_gemv<T>(bsr, s.block, s.N / s.block, b, y, s.p.stream);
     return;
@@ -741,6 +828,20 @@ void LinearSolver::Solve(const double *dev_rhs, LinearResult &out, double tol, b
     hip_check(hipMemsetAsync(x, 0, sizeof(double) * N, p.stream), "memset");
     hip_check(hipMemsetAsync(res, 0, sizeof(double) * N, p.stream), "memset");
     out.solve = std::max(0., clock_s() - start - out.setup);
+    return;
+  }
+  if (direct) {
+    // src/linear.cpp:224-240: x from the LU, then the residual of the
+    // assembled operator
+    eg::launch_dense_solve(dense, N, dense_piv, dev_rhs, x, dense_work, dense_step, p.stream);
+    Apply(x.p, act.p);
+    eg::launch_sub(dev_rhs, act, res, N, p.stream);
+    true_norm = Norm(res);
+    out.solve = std::max(0., clock_s() - start - out.setup);
+    out.residual_norm = true_norm;
+    out.residual = true_norm / rhs_norm;
+    require(std::isfinite(out.residual) && out.residual <= std::max(10 * tol, 1e-12),
+            "True inverse-mass weighted linear residual too large");
     return;
   }
   if (Batchable()) {

@@ -3,6 +3,7 @@
 // Freeze/Residual/CellNorm/Admissible on the GPU), the BSR ILU(0) topology
 // and the HDG condensed state.
 #include <algorithm>
+#include <set>
 #include <chrono>
 #include <cmath>
 #include <fstream>
@@ -504,9 +505,79 @@ void Problem::Save(const std::vector<double> &u, const std::string &prefix) cons
 // src/block_ilu.cpp:10-59 (natural ordering) plus level sets for the
 // parallel factor and sweeps: a row's level is one more than the deepest
 // level among the rows it depends on.
-BlockPattern::BlockPattern(const std::vector<std::vector<int>> &graph, int size, hipStream_t s)
-    : nb(graph.size()), block(size) {
+// The reference runs MFEM's MinimumDiscardedFillOrdering once on the
+// topology matrix of the block graph (every stored entry 1). MFEM is not
+// available here; this restates the published minimum discarded fill rule
+// (Persson and Peraire, SIAM J. Sci. Comput. 30 (2008) 2709-2733) on that
+// matrix: the weight of an uneliminated row k is the square root of the
+// number of fill entries (i, j) that eliminating k would create and ILU(0)
+// discards (i != j both uneliminated neighbours of k, (i, j) not in the
+// pattern; each product of unit entries counts 1); the row of least weight is
+// eliminated next, the lowest index on ties, and the weights of its
+// uneliminated neighbours are recomputed. MFEM's own tie order is not
+// pinned by anything in the reference.
+std::vector<int> MdfOrdering(const std::vector<std::vector<int>> &graph) {
+  const int n = int(graph.size());
+  std::vector<std::vector<int>> adj(n);
+  for (int i = 0; i < n; ++i) {
+    adj[i] = graph[i];
+    adj[i].push_back(i);
+    std::sort(adj[i].begin(), adj[i].end());
+    adj[i].erase(std::unique(adj[i].begin(), adj[i].end()), adj[i].end());
+  }
+  auto linked = [&](int i, int j) { return std::binary_search(adj[i].begin(), adj[i].end(), j); };
+  std::vector<char> done(n, 0);
+  auto weight = [&](int k) {
+    double w = 0.;
+    for (int i : adj[k]) {
+      if (i == k || done[i]) { continue; }
+      for (int j : adj[k]) {
+        if (j == k || j == i || done[j]) { continue; }
+        if (!linked(i, j)) { w += 1.; }
+      }
+    }
+    return std::sqrt(w);
+  };
+  std::vector<double> w(n);
+  std::set<std::pair<double, int>> heap;
+  for (int k = 0; k < n; ++k) {
+    w[k] = weight(k);
+    heap.insert({w[k], k});
+  }
+  std::vector<int> order;
+  order.reserve(n);
+  while (!heap.empty()) {
+    const int k = heap.begin()->second;
+    heap.erase(heap.begin());
+    done[k] = 1;
+    order.push_back(k);
+    for (int j : adj[k]) {
+      if (j == k || done[j]) { continue; }
+      heap.erase({w[j], j});
+      w[j] = weight(j);
+      heap.insert({w[j], j});
+    }
+  }
+  return order;
+}
+
+BlockPattern::BlockPattern(const std::vector<std::vector<int>> &graph_in, int size, hipStream_t s,
+                           const std::vector<int> &order)
+    : nb(graph_in.size()), block(size) {
   require(block > 0, "Invalid BSR block size");
+  // src/block_ilu.cpp:15-29: row i of the factor is graph row perm[i]
+  std::vector<std::vector<int>> permuted;
+  if (!order.empty()) {
+    require(int(order.size()) == nb, "ILU ordering does not match the block graph");
+    perm = order;
+    inv.assign(nb, -1);
+    for (int i = 0; i < nb; ++i) { inv[perm[i]] = i; }
+    permuted.resize(nb);
+    for (int i = 0; i < nb; ++i) {
+      for (int j : graph_in[perm[i]]) { permuted[i].push_back(inv[j]); }
+    }
+  }
+  const auto &graph = order.empty() ? graph_in : permuted;
   diag.resize(nb);
   row.push_back(0);
   for (int i = 0; i < nb; ++i) {
@@ -576,9 +647,17 @@ BlockPattern::BlockPattern(const std::vector<std::vector<int>> &graph, int size,
   std::vector<int> lower_diag(lower_rows.size());
   for (size_t t = 0; t < lower_rows.size(); ++t) { lower_diag[t] = diag[lower_rows[t]]; }
   up(d_lower_diag, lower_diag);
+  if (!perm.empty()) {
+    up(d_perm, perm);
+    up(d_inv, inv);
+  }
   hip_check(hipStreamSynchronize(s), "sync");
 }
 int BlockPattern::Find(int r, int c) const {
+  if (!inv.empty()) {
+    r = inv[r];
+    c = inv[c];
+  }
   const auto b = col.begin() + row[r], e = col.begin() + row[r + 1];
   const auto it = std::lower_bound(b, e, c);
   require(it != e && *it == c, "Entry outside fixed BSR graph");

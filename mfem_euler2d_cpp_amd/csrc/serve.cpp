@@ -50,6 +50,13 @@ size_t SolveFootprint(const Options &o, int form) {
     if (b > 100) { bytes += 2 * std::max(o.nx, o.ny) * 2 * b * b * 8; }         // large-block scratch
   }
   if (o.check_update && form == 1) { bytes += (o.restart + 9) * nu * 8 + 5 * n * m * m * 8; }
+  // assembled operator (--assembly full, element for DG), dense LU (--solver dense|direct)
+  const bool direct = o.solver != "gmres";
+  if (direct || o.assembly == "full" || (form == 0 && o.assembly == "element")) {
+    bytes += (form == 0 ? 5 * n : 8 * nf) * b * b * (mixed ? 12 : 8);
+  }
+  if (direct) { bytes += N * N * 8 + 6 * N * 8; }
+  if (o.ilu_ordering == "mdf") { bytes += 2 * N * (mixed ? 12 : 8); }
   return size_t(bytes * 1.05) + (size_t(16) << 20);
 }
 

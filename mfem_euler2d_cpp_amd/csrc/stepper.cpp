@@ -259,10 +259,12 @@ RunResult Run(Problem &p, const std::string &form, const std::string &prefix, bo
   return out;
 }
 
-// src/options.cpp:8-150. --solver direct|dense, --assembly full|element and
-// --ilu-ordering mdf (reference diagnostics) are rejected with a message.
+// src/options.cpp:8-150, with the reference's diagnostics: --solver
+// dense|direct (a dense LU on the device; the reference's direct solver is
+// MKL PARDISO), --assembly element|full and --ilu-ordering mdf.
 Options ParseOptions(int argc, char **argv) {
   Options o;
+  bool requested_matrix_free = false, assembly_selected = false;
   for (int i = 1; i < argc; i++) {
     const std::string s = argv[i];
     auto value = [&]() {
@@ -287,13 +289,16 @@ Options ParseOptions(int argc, char **argv) {
     else if (s == "--alpha-deg") { o.alpha = std::stod(value()); }
     else if (s == "--safety") { o.safety = std::stod(value()); }
     else if (s == "--volume-order") { o.volume_order = std::stoi(value()); }
-    else if (s == "--assembly") { o.assembly = value(); }
+    else if (s == "--assembly") {
+      o.assembly = value();
+      assembly_selected = true;
+    }
     else if (s == "--form") { o.form = value(); }
     else if (s == "--solver") { o.solver = value(); }
     else if (s == "--integrator") { o.integrator = value(); }
     else if (s == "--precond") { o.precond = value(); }
     else if (s == "--output") { o.output = value(); }
-    else if (s == "--matrix-free") { o.assembly = "partial"; }
+    else if (s == "--matrix-free") { requested_matrix_free = true; }
     else if (s == "--no-save-solution") { o.save_solution = false; }
     else if (s == "--linear-tol") { o.linear_tol = std::stod(value()); }
     else if (s == "--newton-tol") { o.tol = std::stod(value()); }
@@ -311,8 +316,9 @@ Options ParseOptions(int argc, char **argv) {
                    "--T T (default .2) | --steps N (steps wins; no T clipping)\n"
                    "--max-newton N --newton-tol E | --newton-steps N (fixed updates/stage)\n"
                    "--form dg|hdg|both --integrator ssp-dirk3|be\n"
-                   "--solver gmres --precond block-jacobi|block-ilu0 --ilu-ordering none\n"
-                   "--assembly partial --matrix-free --safety 1.1\n"
+                   "--solver gmres|direct|dense (direct and dense: dense LU on the device)\n"
+                   "--precond block-jacobi|block-ilu0 --ilu-ordering none|mdf\n"
+                   "--assembly full|element|partial --matrix-free --safety 1.1\n"
                    "--linear-tol E --restart N --max-krylov N --output PREFIX\n"
                    "--precision fp64|mixed --inner-tol E --ilu-schedule levels|dataflow|dataflow-all --sweep-threads 0|256|1024\n"
                    "--check-update --timing-only --no-save-solution --hdg-first\n";
@@ -332,21 +338,31 @@ Options ParseOptions(int argc, char **argv) {
           "Nonfinite option");
   require(o.form == "dg" || o.form == "hdg" || o.form == "both", "Invalid form");
   require(o.integrator == "ssp-dirk3" || o.integrator == "be", "Invalid time integrator");
-  require(o.solver == "gmres", "Invalid solver");
+  require(o.solver == "gmres" || o.solver == "direct" || o.solver == "dense", "Invalid solver");
   require(o.restart > 0 && o.max_krylov > 0 && o.max_newton > 0, "Iteration limits must be positive");
   require(o.alpha > 0 && o.alpha < 90, "Require 0 < alpha-deg < 90");
-  require(o.ilu_ordering == "none", "Invalid ILU ordering");
+  require(o.ilu_ordering == "none" || o.ilu_ordering == "mdf", "Invalid ILU ordering");
   require(o.volume_order == -1 || o.volume_order == 2 * o.k + 1, "Volume rule must be (k+1)^2 GL points (order 2k+1)");
   o.volume_order = -1;
-  require(o.precond == "block-jacobi" || o.precond == "block-ilu0", "Invalid preconditioner");
-  require(o.assembly == "partial", "Invalid assembly");
+  if (o.solver == "gmres") {
+    require(o.precond == "block-jacobi" || o.precond == "block-ilu0", "Invalid preconditioner");
+    if (requested_matrix_free) { o.assembly = "partial"; }
+    o.matrix_free = o.assembly != "full";
+  } else {
+    require(!requested_matrix_free, "Direct LU requires an assembled matrix");
+    if (!assembly_selected) { o.assembly = "full"; }
+    require(o.assembly == "full", "Direct LU requires --assembly full");
+    o.matrix_free = false;
+    o.precond = "none";
+  }
+  require(o.assembly == "full" || o.assembly == "element" || o.assembly == "partial", "Invalid assembly");
   require(o.precision == "fp64" || o.precision == "mixed", "Invalid precision");
+  require(o.solver == "gmres" || o.precision == "fp64", "--precision mixed needs --solver gmres");
   require(o.inner_tol > 0 && o.inner_tol < 1, "Invalid inner tolerance");
   require(o.ilu_schedule.empty() || o.ilu_schedule == "levels" || o.ilu_schedule == "dataflow" ||
               o.ilu_schedule == "dataflow-all",
           "Invalid ILU schedule");
   require(o.sweep_threads == 0 || o.sweep_threads == 256 || o.sweep_threads == 1024, "Invalid sweep threads");
-  o.matrix_free = true;
   if (o.timing_only) { o.save_solution = false; }
   return o;
 }

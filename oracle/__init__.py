@@ -160,6 +160,12 @@ class Oracle:
                   col.ctypes.data_as(_ip))
         return row, col
 
+    def ilu_order(self, form):
+        """The ILU(0) factor's block row order for this handle's --ilu-ordering."""
+        order = np.zeros(self.n if form == 0 else self.nf, dtype=np.int32)
+        self.call("or_ilu_order", ctypes.c_int(form), order.ctypes.data_as(_ip))
+        return order
+
     def ilu_blocks(self, form, factored):
         row, col = self.ilu_pattern(form)
         b = self.m if form == 0 else self.t
@@ -206,6 +212,20 @@ class Oracle:
                   d(u), d(stats), inc.ctypes.data_as(_dp) if inc is not None else None, ctypes.c_int(increments),
                   kry if increments else None)
         return u, stats, inc, list(kry)[:increments]
+
+
+def mdf_ordering(graph):
+    """The oracle's minimum discarded fill order of a graph (adjacency lists)."""
+    lib = load()
+    ptr = np.zeros(len(graph) + 1, dtype=np.int32)
+    for i, g in enumerate(graph):
+        ptr[i + 1] = ptr[i] + len(g)
+    adj = np.array([j for g in graph for j in g] or [0], dtype=np.int32)
+    order = np.zeros(len(graph), dtype=np.int32)
+    if lib.or_mdf_ordering(ctypes.c_int(len(graph)), ptr.ctypes.data_as(_ip), adj.ctypes.data_as(_ip),
+                           order.ctypes.data_as(_ip)):
+        raise OracleError(lib.or_last_error().decode())
+    return order
 
 
 def gas(gamma, u, axis, safety, r):

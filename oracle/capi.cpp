@@ -211,6 +211,23 @@ int or_ilu_pattern(void *hv, int form, int *nnzb, int *row, int *col) {
     if (col) { std::memcpy(col, ilu.col.data(), sizeof(int) * ilu.col.size()); }
   });
 }
+// The ILU(0) factor's block row order for the handle's --ilu-ordering.
+int or_ilu_order(void *hv, int form, int *order) {
+  return guard([&] {
+    auto &h = *static_cast<Handle *>(hv);
+    FixedBlockILU ilu(h.p.SolverGraph(form == 1), form == 0 ? h.p.m : h.p.t, h.opt.ilu_ordering);
+    std::memcpy(order, ilu.permutation.data(), sizeof(int) * ilu.permutation.size());
+  });
+}
+// The minimum discarded fill order of a graph given as adjacency lists.
+int or_mdf_ordering(int n, const int *ptr, const int *adj, int *order) {
+  return guard([&] {
+    std::vector<std::vector<int>> g(n);
+    for (int i = 0; i < n; ++i) { g[i].assign(adj + ptr[i], adj + ptr[i + 1]); }
+    const auto o = mdf_ordering(g);
+    std::memcpy(order, o.data(), sizeof(int) * o.size());
+  });
+}
 int or_ilu_blocks(void *hv, int form, int factored, double *values) {
   return guard([&] {
     auto &h = *static_cast<Handle *>(hv);

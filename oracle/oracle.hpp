@@ -113,9 +113,10 @@ public:
   std::vector<int> row, col, diag;
   std::vector<Dense> data;
   std::vector<std::vector<int>> pivots;
+  std::vector<int> permutation, inverse; // factor row i is graph row permutation[i]
   mutable std::vector<double> work;
   int block;
-  FixedBlockILU(const std::vector<std::vector<int>> &graph, int size);
+  FixedBlockILU(const std::vector<std::vector<int>> &graph, int size, const std::string &ordering = "none");
   void Zero(bool off_diagonal_only = false);
   Dense &At(int r, int c);
   void Factor();
@@ -228,6 +229,9 @@ public:
 
 // Dense LU with partial pivoting (MFEM LUFactors semantics).
 bool lu_factor(double *a, int *ipiv, int m);
+// Minimum discarded fill ordering of an unweighted block graph (the
+// reference's --ilu-ordering mdf, src/block_ilu.cpp:16-26).
+std::vector<int> mdf_ordering(const std::vector<std::vector<int>> &graph);
 void lu_solve(const double *a, const int *ipiv, int m, int nrhs, double *x);
 void lu_right_solve(const double *a, const int *ipiv, int m, int rows, double *x);
 
@@ -245,6 +249,8 @@ public:
   std::vector<Dense> diagonal;
   std::vector<std::vector<int>> diagonal_pivots;
   std::unique_ptr<FixedBlockILU> ilu;
+  std::vector<double> dense_matrix; // --solver dense|direct: the assembled matrix and its LU
+  std::vector<int> dense_pivots;
   LinearSolver(int n, int block, const Options &o, const std::vector<double> &metric);
   void Prepare(const std::vector<std::vector<int>> &graph, const Options &o);
   double Dot(const double *a, const double *b) const;

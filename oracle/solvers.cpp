@@ -3,6 +3,7 @@
 // block ILU(0) and weighted restarted GMRES.
 #include "oracle.hpp"
 #include <algorithm>
+#include <cmath>
 #include <limits>
 
 namespace oracle {
@@ -527,15 +528,67 @@ void SchurOperator::FillBlocks(FixedBlockILU &target) const {
 }
 
 // ------------------------------------------------------------- block ILU(0)
-// src/block_ilu.cpp:10-59 (ordering "none")
-FixedBlockILU::FixedBlockILU(const std::vector<std::vector<int>> &graph, int size) : block(size) {
+// src/block_ilu.cpp:16-26 calls MFEM's MinimumDiscardedFillOrdering on the
+// topology matrix of the graph (every entry 1); MFEM is absent, so this is
+// the published rule (Persson and Peraire, SIAM J. Sci. Comput. 30 (2008)
+// 2709-2733) on that matrix: eliminate next the uneliminated row whose
+// elimination discards the least fill, the weight being the square root of
+// the number of ordered pairs (i, j), i != j, of its uneliminated neighbours
+// with (i, j) outside the pattern; ties go to the lowest row; the weights of
+// the eliminated row's neighbours are then recomputed. Parity with MFEM's
+// own tie order is unpinned.
+std::vector<int> mdf_ordering(const std::vector<std::vector<int>> &graph) {
+  const int n = graph.size();
+  std::vector<std::vector<int>> adj(n);
+  for (int i = 0; i < n; ++i) {
+    adj[i] = graph[i];
+    adj[i].push_back(i);
+    std::sort(adj[i].begin(), adj[i].end());
+    adj[i].erase(std::unique(adj[i].begin(), adj[i].end()), adj[i].end());
+  }
+  std::vector<char> eliminated(n, 0);
+  auto discarded = [&](int k) {
+    int count = 0;
+    for (int i : adj[k]) {
+      if (i == k || eliminated[i]) { continue; }
+      for (int j : adj[k]) {
+        if (j == k || j == i || eliminated[j]) { continue; }
+        if (!std::binary_search(adj[i].begin(), adj[i].end(), j)) { ++count; }
+      }
+    }
+    return std::sqrt(double(count));
+  };
+  std::vector<double> w(n);
+  for (int k = 0; k < n; ++k) { w[k] = discarded(k); }
+  std::vector<int> order;
+  for (int step = 0; step < n; ++step) {
+    int best = -1;
+    for (int k = 0; k < n; ++k) {
+      if (!eliminated[k] && (best < 0 || w[k] < w[best])) { best = k; }
+    }
+    eliminated[best] = 1;
+    order.push_back(best);
+    for (int j : adj[best]) {
+      if (!eliminated[j]) { w[j] = discarded(j); }
+    }
+  }
+  return order;
+}
+// src/block_ilu.cpp:10-59
+FixedBlockILU::FixedBlockILU(const std::vector<std::vector<int>> &graph, int size, const std::string &ordering)
+    : block(size) {
   const int nb = graph.size();
   require(block > 0, "Invalid BSR block size");
+  permutation.resize(nb);
+  inverse.resize(nb);
+  for (int i = 0; i < nb; ++i) { permutation[i] = i; }
+  if (ordering == "mdf") { permutation = mdf_ordering(graph); }
+  for (int i = 0; i < nb; ++i) { inverse[permutation[i]] = i; }
   diag.resize(nb);
   row.push_back(0);
   for (int i = 0; i < nb; ++i) {
     std::vector<int> nbrs{i};
-    for (int j : graph[i]) { nbrs.push_back(j); }
+    for (int j : graph[permutation[i]]) { nbrs.push_back(inverse[j]); }
     std::sort(nbrs.begin(), nbrs.end());
     nbrs.erase(std::unique(nbrs.begin(), nbrs.end()), nbrs.end());
     for (int j : nbrs) {
@@ -570,7 +623,8 @@ void FixedBlockILU::Zero(bool off_diagonal_only) {
   }
 }
 // src/block_ilu.cpp:67-72
-Dense &FixedBlockILU::At(int r, int c) {
+Dense &FixedBlockILU::At(int original_r, int original_c) {
+  const int r = inverse[original_r], c = inverse[original_c];
   auto it = std::lower_bound(col.begin() + row[r], col.begin() + row[r + 1], c);
   require(it != col.begin() + row[r + 1] && *it == c, "Entry outside fixed BSR graph");
   return data[it - col.begin()];
@@ -591,7 +645,7 @@ void FixedBlockILU::Mult(const double *b, double *x) const {
   const int nb = diag.size();
   for (int i = 0; i < nb; ++i) {
     double *wi = work.data() + size_t(i) * block;
-    std::copy_n(b + size_t(i) * block, block, wi);
+    std::copy_n(b + size_t(permutation[i]) * block, block, wi);
     for (int ij = row[i]; ij < diag[i]; ++ij) {
       const double *wj = work.data() + size_t(col[ij]) * block;
       const Dense &a = data[ij];
@@ -610,7 +664,7 @@ void FixedBlockILU::Mult(const double *b, double *x) const {
       }
     }
     lu_solve(data[diag[i]].data(), pivots[i].data(), block, 1, wi);
-    std::copy_n(wi, block, x + size_t(i) * block);
+    std::copy_n(wi, block, x + size_t(permutation[i]) * block);
   }
 }
 
@@ -627,15 +681,23 @@ LinearSolver::LinearSolver(int size, int b, const Options &o, const std::vector<
   h.assign(size_t(restart + 1) * restart, 0.);
   cosine.assign(restart, 0.); sine.assign(restart, 0.);
   rotated.assign(restart + 1, 0.); coefficients.assign(restart, 0.);
-  require(o.solver == "gmres", "Only --solver gmres is restated by the oracle");
-  if (o.precond == "block-jacobi") {
+  if (o.solver == "dense" || o.solver == "direct") {
+    // src/linear.cpp:46-49 (the reference's direct solver, MKL PARDISO, is
+    // an exact solve as well: the same dense LU here)
+    require(n <= 5000, "Dense direct solve limited to 5000 unknowns");
+    dense_matrix.assign(size_t(n) * n, 0.);
+    dense_pivots.assign(n, 0);
+  }
+  if (o.solver == "gmres" && o.precond == "block-jacobi") {
     diagonal.resize(n / block);
     diagonal_pivots.assign(n / block, std::vector<int>(block));
   }
 }
 // src/linear.cpp:162-166
 void LinearSolver::Prepare(const std::vector<std::vector<int>> &graph, const Options &o) {
-  if (!ilu && o.precond == "block-ilu0") { ilu = std::make_unique<FixedBlockILU>(graph, block); }
+  if (!ilu && o.solver == "gmres" && o.precond == "block-ilu0") {
+    ilu = std::make_unique<FixedBlockILU>(graph, block, o.ilu_ordering);
+  }
 }
 // src/linear.cpp:58-65
 double LinearSolver::Dot(const double *a, const double *b) const {
@@ -754,6 +816,35 @@ void LinearSolver::Solve(const Operator &op, const double *rhs, const Options &o
     std::fill(out.equation_residual.begin(), out.equation_residual.end(), 0.);
     return;
   }
+  if (o.solver != "gmres") {
+    // src/linear.cpp:184-213,224-240: the assembled matrix (column j = A e_j;
+    // the reference assembles the same entries from its element blocks),
+    // its LU with partial pivoting, the solve and the operator's residual
+    double start = clock_s();
+    if (refresh) {
+      std::vector<double> unit(n, 0.), column(n);
+      for (int j = 0; j < n; ++j) {
+        unit[j] = 1.;
+        op.Mult(unit.data(), column.data());
+        unit[j] = 0.;
+        std::copy(column.begin(), column.end(), dense_matrix.begin() + size_t(j) * n);
+      }
+    }
+    out.assembly = clock_s() - start;
+    start = clock_s();
+    if (refresh) { require(lu_factor(dense_matrix.data(), dense_pivots.data(), n), "Singular dense LU factor"); }
+    out.setup = clock_s() - start;
+    start = clock_s();
+    std::copy(rhs, rhs + n, out.x.begin());
+    lu_solve(dense_matrix.data(), dense_pivots.data(), n, 1, out.x.data());
+    out.solve = clock_s() - start;
+    op.Mult(out.x.data(), out.equation_residual.data());
+    for (int i = 0; i < n; ++i) { out.equation_residual[i] -= rhs[i]; }
+    out.residual = std::sqrt(Dot(out.equation_residual.data(), out.equation_residual.data())) / rhs_norm;
+    require(std::isfinite(out.residual) && out.residual <= std::max(10 * o.linear_tol, 1e-12),
+            "True inverse-mass weighted linear residual too large");
+    return;
+  }
   if (o.precond == "block-ilu0" && !ilu) { Prepare(op.BlockGraph(block), o); }
   double start = clock_s();
   if (refresh && ilu) { op.FillBlocks(*ilu); }
@@ -774,8 +865,15 @@ void LinearSolver::Solve(const Operator &op, const double *rhs, const Options &o
   start = clock_s();
   const double gmres_norm = GMRES(op, rhs, o.linear_tol, o.max_krylov, out, refresh);
   out.solve = clock_s() - start;
-  for (int i = 0; i < n; ++i) { out.equation_residual[i] = -residual[i]; }
-  out.residual = gmres_norm / rhs_norm;
+  if (o.matrix_free) {
+    for (int i = 0; i < n; ++i) { out.equation_residual[i] = -residual[i]; }
+    out.residual = gmres_norm / rhs_norm;
+  } else {
+    // src/linear.cpp:236-240: an assembled run keeps the operator's own residual
+    op.Mult(out.x.data(), out.equation_residual.data());
+    for (int i = 0; i < n; ++i) { out.equation_residual[i] -= rhs[i]; }
+    out.residual = std::sqrt(Dot(out.equation_residual.data(), out.equation_residual.data())) / rhs_norm;
+  }
   require(std::isfinite(out.residual) && out.residual <= std::max(10 * o.linear_tol, 1e-12),
           "True inverse-mass weighted linear residual too large");
 }

@@ -248,10 +248,11 @@ RunResult Run(Problem &p, const std::string &form, const std::string &prefix,
   return out;
 }
 
-// src/options.cpp:8-150 (the --solver dense|direct, --assembly full|element
-// and --ilu-ordering mdf diagnostics are rejected: they are not restated)
+// src/options.cpp:8-150 (--solver direct, MKL PARDISO in the reference, is
+// restated as the same dense LU as --solver dense)
 Options ParseOptions(int argc, char **argv) {
   Options o;
+  bool requested_matrix_free = false, assembly_selected = false;
   for (int i = 1; i < argc; i++) {
     const std::string s = argv[i];
     auto value = [&]() {
@@ -276,13 +277,16 @@ Options ParseOptions(int argc, char **argv) {
     else if (s == "--alpha-deg") { o.alpha = std::stod(value()); }
     else if (s == "--safety") { o.safety = std::stod(value()); }
     else if (s == "--volume-order") { o.volume_order = std::stoi(value()); }
-    else if (s == "--assembly") { o.assembly = value(); }
+    else if (s == "--assembly") {
+      o.assembly = value();
+      assembly_selected = true;
+    }
     else if (s == "--form") { o.form = value(); }
     else if (s == "--solver") { o.solver = value(); }
     else if (s == "--integrator") { o.integrator = value(); }
     else if (s == "--precond") { o.precond = value(); }
     else if (s == "--output") { o.output = value(); }
-    else if (s == "--matrix-free") { o.assembly = "partial"; }
+    else if (s == "--matrix-free") { requested_matrix_free = true; }
     else if (s == "--no-save-solution") { o.save_solution = false; }
     else if (s == "--linear-tol") { o.linear_tol = std::stod(value()); }
     else if (s == "--newton-tol") { o.tol = std::stod(value()); }
@@ -304,16 +308,25 @@ Options ParseOptions(int argc, char **argv) {
           "Nonfinite option");
   require(o.form == "dg" || o.form == "hdg" || o.form == "both", "Invalid form");
   require(o.integrator == "ssp-dirk3" || o.integrator == "be", "Invalid time integrator");
-  require(o.solver == "gmres", "Invalid solver");
+  require(o.solver == "gmres" || o.solver == "direct" || o.solver == "dense", "Invalid solver");
   require(o.restart > 0 && o.max_krylov > 0 && o.max_newton > 0, "Iteration limits must be positive");
   require(o.alpha > 0 && o.alpha < 90, "Require 0 < alpha-deg < 90");
-  require(o.ilu_ordering == "none", "Invalid ILU ordering");
+  require(o.ilu_ordering == "none" || o.ilu_ordering == "mdf", "Invalid ILU ordering");
   require(o.volume_order == -1 || o.volume_order == 2 * o.k + 1,
           "Volume rule must be (k+1)^2 GL points (order 2k+1)");
   o.volume_order = -1;
-  require(o.precond == "block-jacobi" || o.precond == "block-ilu0", "Invalid preconditioner");
-  require(o.assembly == "partial", "Invalid assembly");
-  o.matrix_free = true;
+  if (o.solver == "gmres") {
+    require(o.precond == "block-jacobi" || o.precond == "block-ilu0", "Invalid preconditioner");
+    if (requested_matrix_free) { o.assembly = "partial"; }
+    o.matrix_free = o.assembly != "full";
+  } else {
+    require(!requested_matrix_free, "Direct LU requires an assembled matrix");
+    if (!assembly_selected) { o.assembly = "full"; }
+    require(o.assembly == "full", "Direct LU requires --assembly full");
+    o.matrix_free = false;
+    o.precond = "none";
+  }
+  require(o.assembly == "full" || o.assembly == "element" || o.assembly == "partial", "Invalid assembly");
   if (o.timing_only) { o.save_solution = false; }
   return o;
 }

@@ -41,10 +41,13 @@ def test_loader_binds_every_declared_symbol():
     (["--nx", "0"], "Invalid mesh, degree, time or safety"),
     (["--form", "cg"], "Invalid form"),
     (["--integrator", "rk4"], "Invalid time integrator"),
-    (["--solver", "direct"], "Invalid solver"),
+    (["--solver", "pardiso"], "Invalid solver"),
     (["--precond", "amg"], "Invalid preconditioner"),
-    (["--assembly", "full"], "Invalid assembly"),
-    (["--ilu-ordering", "mdf"], "Invalid ILU ordering"),
+    (["--assembly", "csr"], "Invalid assembly"),
+    (["--ilu-ordering", "rcm"], "Invalid ILU ordering"),
+    (["--solver", "dense", "--matrix-free"], "Direct LU requires an assembled matrix"),
+    (["--solver", "direct", "--assembly", "partial"], "Direct LU requires --assembly full"),
+    (["--solver", "dense", "--precision", "mixed"], "--precision mixed needs --solver gmres"),
     (["--precision", "half"], "Invalid precision"),
     (["--nx"], "Missing option value"),
 ])

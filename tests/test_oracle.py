@@ -395,3 +395,92 @@ def test_line_search_damping_branch(oracle_lib, tmp_path):
         assert len(damp) == int(s[0]) and min(damp) == 0.5 and damp.count(0.5) >= 1, damp
         assert all(d in (1.0, 0.5) for d in damp)
     assert np.max(np.abs(finals["dg"] - finals["hdg"])) < 1e-7  # src/main.cpp:47
+
+
+# --------------------------------------------------------------- diagnostics
+def cell_graph(nx, ny):
+    g = []
+    for e in range(nx * ny):
+        ix, iy = e % nx, e // nx
+        g.append(sorted({e, iy * nx + (ix + 1) % nx, iy * nx + (ix - 1) % nx, ((iy + 1) % ny) * nx + ix,
+                         ((iy - 1) % ny) * nx + ix}))
+    return g
+
+
+def face_graph(nx, ny):
+    def face(e, side):
+        ix, iy = e % nx, e // nx
+        return [2 * (iy * nx + (ix - 1) % nx), 2 * e, 2 * (((iy - 1) % ny) * nx + ix) + 1, 2 * e + 1][side]
+    g = [set() for _ in range(2 * nx * ny)]
+    for e in range(nx * ny):
+        for a in range(4):
+            g[face(e, a)].update(face(e, b) for b in range(4))
+    return [sorted(s) for s in g]
+
+
+def mdf_by_definition(graph):
+    """The minimum discarded fill rule written out directly: repeatedly take
+    the uneliminated row whose elimination would create the fewest fill
+    entries (i, j) outside the pattern among its uneliminated neighbours
+    (ordered pairs, i != j; lowest row on ties)."""
+    n = len(graph)
+    nbr = [set(g) | {i} for i, g in enumerate(graph)]
+    left = set(range(n))
+    order = []
+    while left:
+        def fill(k):
+            live = [i for i in nbr[k] if i != k and i in left]
+            return sum(1 for i in live for j in live if i != j and j not in nbr[i])
+        k = min(sorted(left), key=fill)
+        order.append(k)
+        left.remove(k)
+    return order
+
+
+@pytest.mark.parametrize("graph", [cell_graph(4, 3), face_graph(3, 3), face_graph(2, 3), face_graph(4, 2)],
+                         ids=["cells 4x3", "faces 3x3", "faces 2x3", "faces 4x2"])
+def test_mdf_ordering_rule(oracle_lib, graph):
+    """--ilu-ordering mdf (src/block_ilu.cpp:16-26; MFEM's routine is absent,
+    its tie order unpinned): the oracle's order is the stated rule, and a
+    permutation."""
+    got = list(oracle_lib.mdf_ordering(graph))
+    assert sorted(got) == list(range(len(graph)))
+    assert got == mdf_by_definition(graph)
+
+
+def test_mdf_permuted_ilu_brute_force(oracle_lib):
+    """A reordered block ILU(0) (src/block_ilu.cpp:14-29,87-109) is the
+    scalar ILU(0) of the permuted Schur matrix P S P^T on the permuted
+    pattern, applied to P b and permuted back."""
+    o = oracle_lib.Oracle("--nx 3 --ny 3 --k 1 --precond block-ilu0 --ilu-ordering mdf")
+    u = state(o)
+    o.freeze(u, .001)
+    r = o.residual(u)
+    *_, S = condensed(o, u, r)
+    order = o.ilu_order(1)
+    assert list(order) != list(range(o.nf))  # a nontrivial reordering of the face graph
+    idx = np.concatenate([np.arange(f * o.t, (f + 1) * o.t) for f in order])
+    row, col = o.ilu_pattern(1)  # the natural pattern
+    mask = expand_mask(row, col, o.t, o.nf)
+    b = np.sin(.19 * (np.arange(o.nl) + 1))
+    yp = scalar_ilu0_apply(S[np.ix_(idx, idx)], mask[np.ix_(idx, idx)], b[idx])
+    y = np.zeros(o.nl)
+    y[idx] = yp
+    assert relerr(o.precondition(1, 1, b), y) < 1e-10
+
+
+@pytest.mark.parametrize("form", ["dg", "hdg"])
+def test_diagnostic_runs_solve_the_same_systems(oracle_lib, form):
+    """--solver dense|direct, --assembly element|full and --ilu-ordering mdf
+    change how each Newton system is solved, not the system: the final states
+    agree with the default run to the GMRES tolerance, and the dense solver
+    runs no Krylov iterations."""
+    base = "--nx 5 --ny 5 --k 2 --alpha-deg 30 --cfl .5 --steps 1 --newton-steps 2"
+    u0, s0, _, _ = oracle_lib.Oracle(base).run(form)
+    for extra in ("--solver dense", "--solver direct", "--assembly full --precond block-ilu0",
+                  "--assembly element", "--precond block-ilu0 --ilu-ordering mdf"):
+        u, s, _, kry = oracle_lib.Oracle(f"{base} {extra}").run(form, increments=8)
+        assert int(s[0]) == int(s0[0]), extra
+        assert relerr(u, u0) < 1e-10, extra
+        if "--solver" in extra:
+            assert all(k == 0 for k in kry[:int(s[0])])
