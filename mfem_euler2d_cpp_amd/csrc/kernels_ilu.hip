@@ -1987,7 +1987,10 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_slice(const T *__restrict__ 
                                                           const int *__restrict__ col, const T *src, T *dst,
                                                           T *arm, long narm, int *flags, int spin_limit,
                                                           const int *__restrict__ order, int *ticket) {
-  constexpr int NT = 1024, PRE = 4, KDM = 4; // cell blocks: at most 4 neighbours a triangle
+  // cell blocks: at most 4 neighbours a triangle. PRE: columns of each
+  // dependency block loaded before the wait (5 covers a 196-row slice of 8:
+  // no load after the wait; the FP64 upper sweep keeps 4, as 5 spills there)
+  constexpr int NT = 1024, PRE = (sizeof(T) == 8 && !LOWER) ? 4 : 5, KDM = 4;
   extern __shared__ __attribute__((aligned(16))) unsigned char slice_lds[];
   __shared__ int next_t;
   T *xs = reinterpret_cast<T *>(slice_lds); // kd x b dependency values
