@@ -899,6 +899,15 @@ template <int B> constexpr size_t resident_gj_lds() { return 0; }
 // stride), prefetched into registers one panel ahead.
 // Workgroup barrier that orders LDS accesses only: outstanding global loads
 // (operand prefetches) and stores stay in flight across it.
+// Workgroup w of a launch is dispatched to XCD w % 8. This maps the n
+// workgroups to logical ids so that each XCD receives one run of consecutive
+// ids (n / 8 or n / 8 + 1 of them): the slabs of a block row, which read the
+// same operand blocks, then share one XCD's L2.
+__device__ __forceinline__ int xcd_run_id(int w, int n) {
+  constexpr int X = 8;
+  const int per = n / X, rem = n % X, x = w % X, q = w / X;
+  return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + q;
+}
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
@@ -977,7 +986,8 @@ __global__ __launch_bounds__(256) void k_ilu_slab(double *__restrict__ bsr, cons
   extern __shared__ double sm[];
   constexpr size_t bb = size_t(B) * B;
   double *Xs = sm, *Ys = sm + B * S::LDX;
-  const int i = rows[blockIdx.x / S::NS], r0 = (blockIdx.x % S::NS) * S::R;
+  const int bid = xcd_run_id(blockIdx.x, gridDim.x);
+  const int i = rows[bid / S::NS], r0 = (bid % S::NS) * S::R;
   const int t = threadIdx.x, tc = t & 63, tr = t >> 6;
   double acc[S::RT][S::CJ];
   for (int ik = row_ptr[i]; ik < diag_ptr[i]; ++ik) {
@@ -1106,7 +1116,8 @@ __global__ __launch_bounds__(256) void k_ilu_slab_m(double *__restrict__ bsr, co
   using M = SlabM<B>;
   extern __shared__ double Xs[]; // [k][16], k < 4 KS
   constexpr size_t bb = size_t(B) * B;
-  const int i = rows[blockIdx.x / M::NS], r0 = (blockIdx.x % M::NS) * 16;
+  const int bid = xcd_run_id(blockIdx.x, gridDim.x);
+  const int i = rows[bid / M::NS], r0 = (bid % M::NS) * 16;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63, g = lane >> 4, j = lane & 15;
   for (int e = B * 16 + t; e < 4 * M::KS * 16; e += 256) { Xs[e] = 0.; } // k >= B
   const int ik0 = row_ptr[i], ik1 = diag_ptr[i];
@@ -1418,7 +1429,8 @@ __global__ __launch_bounds__(256) void k_ilu_slab_star(double *__restrict__ bsr,
   using M = SlabM<B>;
   static_assert(M::TW == 1 && M::NH == 1, "one tile a wavefront, one chunk a tile");
   extern __shared__ double Xs[]; // [l][k][16]
-  const int i = rows[blockIdx.x / M::NS], r0 = (blockIdx.x % M::NS) * 16;
+  const int bid = xcd_run_id(blockIdx.x, gridDim.x);
+  const int i = rows[bid / M::NS], r0 = (bid % M::NS) * 16;
   const int j0 = row_ptr[i], nl = diag_ptr[i] - j0;
   switch (nl) { // the number of lower blocks as a constant: no guards in the MFMA chains
   case 1: slab_star_row<B, 1>(bsr, i, r0, j0, diag_ptr, col, upd_ptr, upd, Xs); break;
