@@ -299,6 +299,13 @@ public:
   // launched directly).
   bool use_graph = true;
   hipGraphExec_t factor_exec = nullptr;
+  // Mixed precision with 100, 144 or 196 cell blocks: the factor is computed
+  // in FP32 on blocks32 (factor32, per-level launches in their own graph);
+  // the FP64 blocks keep the unfactored Jacobian blocks, factored only when
+  // an FP64 preconditioner apply asks for them (factor64_pending).
+  bool factor32 = false;
+  bool factor64_pending = false;
+  hipGraphExec_t factor_exec32 = nullptr;
   // Row-resident factor of small blocks: per-row programs in level order.
   bool resident = false;
   size_t resident_lds = 0;
@@ -336,6 +343,11 @@ public:
   // contributions of each face row).
   void FillMaps(const BlockPattern &pt, DeviceArray<int> &ds, DeviceArray<int> &sl, DeviceArray<int> &ct);
   void FactorLevels();
+  void FactorLevels32();
+  // The per-level launches of levels, recorded once into exec and replayed.
+  void ReplayLevels(hipGraphExec_t &exec, void (LinearSolver::*levels)());
+  // The FP64 factor (or block-Jacobi inverses) of the filled blocks.
+  void Factor64();
   void Refresh(const double *dev_u) { state = dev_u; }
   // Fill + factor the preconditioner at the current state. check_flags
   // false leaves the singular-factor check to the next readback of Solve

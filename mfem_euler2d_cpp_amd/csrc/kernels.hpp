@@ -134,6 +134,14 @@ void launch_ilu_scale_upper(double *bsr, int b, const int *slot, const int *dslo
 
 void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
                      const int *col, const int *upd_ptr, const int *upd, bool star, hipStream_t s);
+// The single-precision factor of the mixed form, for b in {100, 144, 196}
+// (ilu_factor32_ok): the MFMA slab products of one level, the diagonal
+// inverses of its rows, and the upper scaling, on FP32 blocks.
+bool ilu_factor32_ok(int b);
+void launch_ilu_slab32(float *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                       const int *col, const int *upd_ptr, const int *upd, hipStream_t s);
+void launch_block_inverse32(float *blocks, int b, int count, const int *index, int *flags, hipStream_t s);
+void launch_ilu_scale_upper32(float *bsr, int b, const int *slot, const int *dslot, int count, hipStream_t s);
 // Single-launch dataflow forms: rows taken from a ticket counter in
 // dependency order (per-row completion flags or sentinel-armed values), so
 // they finish whether or not the whole grid is resident; return false when

@@ -263,11 +263,11 @@ template <int B, int NCG> struct Gj2 {
   static constexpr int NT = 64 * NCG;
   static_assert(RW <= 4 && NT <= 1024, "register-resident block");
 };
-template <int B, int NCG> struct Gj2Lds {
-  double col[2][64 * Gj2<B, NCG>::RW];
-  double inv[2];
+template <int B, int NCG, class T> struct Gj2Lds {
+  T col[2][64 * Gj2<B, NCG>::RW];
+  T inv[2];
   int prow[2];
-  double scale[B];
+  T scale[B];
   int piv[B], pinv[B];
 };
 // 1 / a from the hardware reciprocal and two Newton steps (about 1 ulp;
@@ -279,6 +279,14 @@ __device__ __forceinline__ double recip(double a) {
   x = fma(x, e, x);
   e = fma(-a, x, 1.);
   return fma(x, e, x);
+}
+// 1 / a in single precision: the hardware reciprocal and one Newton step.
+__device__ __forceinline__ float recip(float a) {
+  const float x = __builtin_amdgcn_rcpf(a);
+  return fmaf(x, fmaf(-a, x, 1.f), x);
+}
+__device__ __forceinline__ float read_lane(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 __device__ __forceinline__ double read_lane(double v, int lane) {
   const unsigned long long u = __double_as_longlong(v);
@@ -310,13 +318,13 @@ __device__ __forceinline__ int wave_argmax(float mag, bool used, int lane) {
 }
 // Pivot search of the owning wave over the register column x (rows lane +
 // 64 i not yet used); publishes the pivot row, its reciprocal and the column.
-template <int B, int NCG>
-__device__ __forceinline__ void gj2_publish(const double (&x)[Gj2<B, NCG>::RW], unsigned used, int lane,
-                                            Gj2Lds<B, NCG> &L, int buf) {
+template <int B, int NCG, class T>
+__device__ __forceinline__ void gj2_publish(const T (&x)[Gj2<B, NCG>::RW], unsigned used, int lane,
+                                            Gj2Lds<B, NCG, T> &L, int buf) {
   constexpr int RW = Gj2<B, NCG>::RW;
   float key = -1.f;
   int bi = 0;
-  double bv = 0.;
+  T bv = T(0);
 #pragma unroll
   for (int i = 0; i < RW; ++i) {
     const int r = lane + 64 * i;
@@ -327,33 +335,33 @@ __device__ __forceinline__ void gj2_publish(const double (&x)[Gj2<B, NCG>::RW], 
   }
   const int pl = wave_argmax(key, key < 0.f, lane);
   const int pi = __builtin_amdgcn_readlane(bi, pl);
-  const double ap = read_lane(bv, pl);
+  const T ap = read_lane(bv, pl);
   if (lane == 0) {
     L.prow[buf] = pl + 64 * pi;
     L.inv[buf] = recip(ap);
   }
 }
-template <int B, int NCG>
-__device__ __forceinline__ bool gj2_inverse(double *__restrict__ A, Gj2Lds<B, NCG> &L) {
+template <int B, int NCG, class T>
+__device__ __forceinline__ bool gj2_inverse(T *__restrict__ A, Gj2Lds<B, NCG, T> &L) {
   using G = Gj2<B, NCG>;
   constexpr int RW = G::RW, NE = G::NE;
   const int t = threadIdx.x, lane = t & 63, cg = t >> 6;
-  double v[RW][NE];
+  T v[RW][NE];
 #pragma unroll
   for (int i = 0; i < RW; ++i) {
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
       const int r = lane + 64 * i, c = cg + NCG * j;
-      v[i][j] = (r < B && c < B) ? A[r + size_t(c) * B] : 0.;
+      v[i][j] = (r < B && c < B) ? A[r + size_t(c) * B] : T(0);
     }
   }
   unsigned used = 0u; // bit i: row lane + 64 i is a pivot row already
   bool singular = false;
   if (cg == 0) {
-    double x[RW];
+    T x[RW];
 #pragma unroll
     for (int i = 0; i < RW; ++i) { x[i] = v[i][0]; }
-    gj2_publish<B, NCG>(x, used, lane, L, 0);
+    gj2_publish<B, NCG, T>(x, used, lane, L, 0);
   }
   for (int jb = 0; jb < NE; ++jb) {
 #pragma unroll
@@ -365,32 +373,32 @@ __device__ __forceinline__ bool gj2_inverse(double *__restrict__ A, Gj2Lds<B, NC
       const int j1 = s + 1 < NCG ? 0 : 1; // register of column k + 1 in its owner (unrolled: a constant)
       __syncthreads();
       const int p = __builtin_amdgcn_readfirstlane(L.prow[buf]);
-      const double inv = L.inv[buf];
-      if (!(isfinite(inv) && inv != 0.)) { singular = true; }
+      const T inv = L.inv[buf];
+      if (!(isfinite(inv) && inv != T(0))) { singular = true; }
       const int ip = p >> 6, lp = p & 63;
-      double m[RW];
+      T m[RW];
 #pragma unroll
-      for (int i = 0; i < RW; ++i) { m[i] = (lane + 64 * i == p) ? 0. : L.col[buf][lane + 64 * i] * inv; }
+      for (int i = 0; i < RW; ++i) { m[i] = (lane + 64 * i == p) ? T(0) : L.col[buf][lane + 64 * i] * inv; }
       const unsigned used_next = used | (lane == lp ? 1u << ip : 0u);
       const bool ahead = cg == s1 && k + 1 < B;
       if (ahead && j1 < NE) {
         __builtin_amdgcn_s_setprio(3); // the next pivot is the critical path
-        double pr = 0.;
+        T pr = T(0);
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
           if (ip == i) { pr = read_lane(v[i][j1], lp); }
         }
-        double x[RW];
+        T x[RW];
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
           v[i][j1] = fma(-m[i], pr, v[i][j1]);
           x[i] = v[i][j1];
         }
-        gj2_publish<B, NCG>(x, used_next, lane, L, buf ^ 1);
+        gj2_publish<B, NCG, T>(x, used_next, lane, L, buf ^ 1);
         __builtin_amdgcn_s_setprio(0);
       }
       // the pivot row's values of this wave's columns (scalar registers)
-      double prow[NE];
+      T prow[NE];
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
         if (ip == i) {
@@ -406,7 +414,7 @@ __device__ __forceinline__ bool gj2_inverse(double *__restrict__ A, Gj2Lds<B, NC
       }
       if (cg == s) {
 #pragma unroll
-        for (int i = 0; i < RW; ++i) { v[i][0] = (lane + 64 * i == p) ? 1. : -m[i]; }
+        for (int i = 0; i < RW; ++i) { v[i][0] = (lane + 64 * i == p) ? T(1) : -m[i]; }
       }
       used = used_next;
       if (t == 0) {
@@ -417,7 +425,7 @@ __device__ __forceinline__ bool gj2_inverse(double *__restrict__ A, Gj2Lds<B, NC
     }
 #pragma unroll
     for (int i = 0; i < RW; ++i) {
-      const double first = v[i][0];
+      const T first = v[i][0];
 #pragma unroll
       for (int j = 0; j + 1 < NE; ++j) { v[i][j] = v[i][j + 1]; }
       v[i][NE - 1] = first;
@@ -642,14 +650,14 @@ template <> struct Gj2Pick<100> { static constexpr int NCG = 4; };
 template <> struct Gj2Pick<144> { static constexpr int NCG = 8; };
 template <> struct Gj2Pick<196> { static constexpr int NCG = 8; };
 template <int B> constexpr int gj2_threads() { return 64 * Gj2Pick<B>::NCG; }
-template <int B>
-__global__ __launch_bounds__(gj2_threads<B>()) void k_block_inverse_w(double *__restrict__ blocks,
-                                                                                  const int *__restrict__ index,
-                                                                                  int *__restrict__ flags) {
+template <int B, class T = double>
+__global__ __launch_bounds__(gj2_threads<B>()) void k_block_inverse_w(T *__restrict__ blocks,
+                                                                      const int *__restrict__ index,
+                                                                      int *__restrict__ flags) {
   constexpr int NCG = Gj2Pick<B>::NCG;
-  __shared__ Gj2Lds<B, NCG> L;
+  __shared__ Gj2Lds<B, NCG, T> L;
   const int blk = index ? index[blockIdx.x] : blockIdx.x;
-  if (gj2_inverse<B, NCG>(blocks + size_t(blk) * B * B, L) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
+  if (gj2_inverse<B, NCG, T>(blocks + size_t(blk) * B * B, L) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
 }
 
 template <int B>
@@ -1030,14 +1038,16 @@ __global__ __launch_bounds__(256) void k_ilu_slab(double *__restrict__ bsr, cons
   }
 }
 
-// The same slab products on the FP64 matrix cores (v_mfma_f64_16x16x4f64):
-// the 16-row slab is the M dimension of the tile, each wavefront owns the
-// 16-column output tiles w, w + 4, ... Lane (g, j) = (lane / 16, lane % 16)
-// supplies k = g KS + s at step s (s < KS, 4 KS >= B, k >= B zero), so the
-// right operand of a lane is the contiguous run Y[g KS .. g KS + KS) of
-// column 16 ct + j, read straight from global memory in chunks of S steps,
-// one chunk ahead; the left operand (the slab, k-major) sits in LDS. The
-// accumulator of lane (g, j) holds rows g + 4 r (r < 4) of column 16 ct + j.
+// The same slab products on the matrix cores (v_mfma_f64_16x16x4f64, and
+// v_mfma_f32_16x16x4f32 for the single-precision factor): the 16-row slab
+// is the M dimension of the tile, each wavefront owns the 16-column output
+// tiles w, w + 4, ...; the right operand is read straight from global
+// memory, one chunk of loads ahead of the MFMAs, the left operand (the
+// slab, k-major) sits in LDS. SlabM: the layout of the two-stage kernel of
+// 64 x 64 blocks (k_ilu_slab_star), lane (g, j) = (lane / 16, lane % 16)
+// supplying k = g KS + s at step s (s < KS, 4 KS >= B, k >= B zero), the
+// contiguous run Y[g KS .. g KS + KS) of column 16 ct + j. SlabV (below):
+// the layout of k_ilu_slab_m, 16-byte loads.
 constexpr int slab_chunk(int k4) {
   int best = 16, waste = 1 << 30;
   for (int c = 16; c >= 8; --c) {
@@ -1057,47 +1067,104 @@ template <int B> struct SlabM {
   static constexpr size_t lds() { return size_t(4) * KS * 16 * sizeof(double); }
 };
 typedef double slab_d4 __attribute__((ext_vector_type(4)));
-template <int B>
-__device__ __forceinline__ void slab_chunk_load(const double *__restrict__ Y, int ct, int h, double (&y)[SlabM<B>::S]) {
-  using M = SlabM<B>;
+typedef float slab_f4 __attribute__((ext_vector_type(4)));
+// One 16 x 16 x 4 MFMA step per value type. Lane (g, j) = (lane / 16,
+// lane % 16) supplies A(j, g) and B(g, j); entry q of its accumulator is
+// D(row(g, q), j): row g + 4 q for v_mfma_f64_16x16x4f64, 4 g + q for
+// v_mfma_f32_16x16x4f32.
+template <class T> struct Mfma16;
+template <> struct Mfma16<double> {
+  typedef slab_d4 acc;
+  static __device__ __forceinline__ acc mac(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int q) { return g + 4 * q; }
+};
+template <> struct Mfma16<float> {
+  typedef slab_f4 acc;
+  static __device__ __forceinline__ acc mac(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int q) { return 4 * g + q; }
+};
+// The operand layout of k_ilu_slab_m: lane (g, j) loads V = 16 / sizeof(T)
+// consecutive values of its column with one 16-byte load, k = 4 V t + V g +
+// e (e < V) at load t, so the four lane groups read one 64-byte run of each
+// column (16 cache lines a load instruction, a load every V MFMAs). The
+// MFMA step (t, e) reduces over k = 4 V t + V g + e, g < 4; k >= B is zero
+// (B is a multiple of V: a load lies wholly inside or outside the block).
+// The loads come in chunks of S, one chunk ahead of the MFMAs.
+constexpr int slabv_chunk(int kl, int cap) {
+  int best = cap, waste = 1 << 30;
+  for (int c = cap; c >= 4; --c) {
+    const int w = (kl + c - 1) / c * c - kl;
+    if (w < waste) { waste = w; best = c; }
+  }
+  return kl < 4 ? kl : best;
+}
+template <int B, class T> struct SlabV {
+  static constexpr int V = 16 / int(sizeof(T));
+  static_assert(B % V == 0, "whole 16-byte loads");
+  static constexpr int NS = (B + 15) / 16;                          // slabs of a block row
+  static constexpr int CT = (B + 15) / 16;                          // 16-column output tiles
+  static constexpr int TW = (CT + 3) / 4;                           // tiles per wavefront (at most)
+  static constexpr int KL = (B + 4 * V - 1) / (4 * V);              // loads per tile
+  static constexpr int S = slabv_chunk(KL, sizeof(T) == 8 ? 6 : 8); // loads per chunk
+  static constexpr int NH = (KL + S - 1) / S;                       // chunks per tile
+  static constexpr int KP = 4 * V * S * NH;                         // k extent of the slab in LDS
+  static constexpr size_t lds() { return size_t(KP) * 16 * sizeof(T); }
+};
+template <class T> struct Vec16;
+template <> struct Vec16<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct Vec16<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <int B, class T>
+__device__ __forceinline__ void slabv_load(const T *__restrict__ Y, int ct, int h,
+                                           typename Vec16<T>::type (&y)[SlabV<B, T>::S]) {
+  using M = SlabV<B, T>;
+  using vec = typename Vec16<T>::type;
   const int lane = threadIdx.x & 63, g = lane >> 4, c = 16 * ct + (lane & 15);
 #pragma unroll
   for (int s = 0; s < M::S; ++s) {
-    const int k = g * M::KS + h * M::S + s;
-    y[s] = (k < B && c < B) ? Y[k + size_t(c) * B] : 0.;
+    const int k = M::V * (4 * (h * M::S + s) + g);
+    y[s] = (k < B && c < B) ? *reinterpret_cast<const vec *>(Y + k + size_t(c) * B) : vec(T(0));
   }
 }
 // acc[tw] = Xs Y for the tiles of this wavefront; pre holds the first chunk
 // of Y on entry and the first chunk of Ynext (when given) on return.
-template <int B>
-__device__ __forceinline__ void slab_mfma(const double *__restrict__ Y, const double *__restrict__ Ynext,
-                                          const double *Xs, slab_d4 (&acc)[SlabM<B>::TW],
-                                          double (&pre)[SlabM<B>::S]) {
-  using M = SlabM<B>;
+template <int B, class T>
+__device__ __forceinline__ void slabv_mfma(const T *__restrict__ Y, const T *__restrict__ Ynext, const T *Xs,
+                                           typename Mfma16<T>::acc (&acc)[SlabV<B, T>::TW],
+                                           typename Vec16<T>::type (&pre)[SlabV<B, T>::S]) {
+  using M = SlabV<B, T>;
+  using F = Mfma16<T>;
+  using vec = typename Vec16<T>::type;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
-  double cur[M::S];
+  vec cur[M::S];
 #pragma unroll
   for (int s = 0; s < M::S; ++s) { cur[s] = pre[s]; }
 #pragma unroll
   for (int tw = 0; tw < M::TW; ++tw) {
     const int ct = w + 4 * tw;
-    acc[tw] = slab_d4{0., 0., 0., 0.};
+    acc[tw] = typename F::acc{T(0), T(0), T(0), T(0)};
     if (ct < M::CT) {
 #pragma unroll
       for (int h = 0; h < M::NH; ++h) {
-        double nxt[M::S];
-        if (h + 1 < M::NH) { slab_chunk_load<B>(Y, ct, h + 1, nxt); }
-        else if (tw + 1 < M::TW && ct + 4 < M::CT) { slab_chunk_load<B>(Y, ct + 4, 0, nxt); }
-        else if (Ynext) { slab_chunk_load<B>(Ynext, w, 0, nxt); }
+        vec nxt[M::S];
+        if (h + 1 < M::NH) { slabv_load<B, T>(Y, ct, h + 1, nxt); }
+        else if (tw + 1 < M::TW && ct + 4 < M::CT) { slabv_load<B, T>(Y, ct + 4, 0, nxt); }
+        else if (Ynext) { slabv_load<B, T>(Ynext, w, 0, nxt); }
         else {
 #pragma unroll
-          for (int s = 0; s < M::S; ++s) { nxt[s] = 0.; }
+          for (int s = 0; s < M::S; ++s) { nxt[s] = vec(T(0)); }
         }
-        slab_d4 a4 = acc[tw];
+        typename F::acc a4 = acc[tw];
 #pragma unroll
         for (int s = 0; s < M::S; ++s) {
-          const double a = Xs[(g * M::KS + h * M::S + s) * 16 + j];
-          a4 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, cur[s], a4, 0, 0, 0);
+#pragma unroll
+          for (int e = 0; e < M::V; ++e) {
+            const T a = Xs[(M::V * (4 * (h * M::S + s) + g) + e) * 16 + j];
+            a4 = F::mac(a, cur[s][e], a4);
+          }
         }
         acc[tw] = a4;
 #pragma unroll
@@ -1108,34 +1175,46 @@ __device__ __forceinline__ void slab_mfma(const double *__restrict__ Y, const do
 #pragma unroll
   for (int s = 0; s < M::S; ++s) { pre[s] = cur[s]; }
 }
-template <int B>
-__global__ __launch_bounds__(256) void k_ilu_slab_m(double *__restrict__ bsr, const int *__restrict__ rows,
+template <int B, class T>
+__device__ __forceinline__ void slab_chunk_load(const T *__restrict__ Y, int ct, int h, T (&y)[SlabM<B>::S]) {
+  using M = SlabM<B>;
+  const int lane = threadIdx.x & 63, g = lane >> 4, c = 16 * ct + (lane & 15);
+#pragma unroll
+  for (int s = 0; s < M::S; ++s) {
+    const int k = g * M::KS + h * M::S + s;
+    y[s] = (k < B && c < B) ? Y[k + size_t(c) * B] : T(0);
+  }
+}
+template <int B, class T = double>
+__global__ __launch_bounds__(256) void k_ilu_slab_m(T *__restrict__ bsr, const int *__restrict__ rows,
                                                     const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
                                                     const int *__restrict__ col, const int *__restrict__ upd_ptr,
                                                     const int *__restrict__ upd) {
-  using M = SlabM<B>;
-  extern __shared__ double Xs[]; // [k][16], k < 4 KS
+  using M = SlabV<B, T>;
+  using F = Mfma16<T>;
+  extern __shared__ __attribute__((aligned(16))) unsigned char slab_lds[];
+  T *Xs = reinterpret_cast<T *>(slab_lds); // [k][16], k < KP
   constexpr size_t bb = size_t(B) * B;
   const int bid = xcd_run_id(blockIdx.x, gridDim.x);
   const int i = rows[bid / M::NS], r0 = (bid % M::NS) * 16;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63, g = lane >> 4, j = lane & 15;
-  for (int e = B * 16 + t; e < 4 * M::KS * 16; e += 256) { Xs[e] = 0.; } // k >= B
+  for (int e = B * 16 + t; e < M::KP * 16; e += 256) { Xs[e] = T(0); } // k >= B
   const int ik0 = row_ptr[i], ik1 = diag_ptr[i];
-  slab_d4 acc[M::TW];
-  double pre[M::S];
-  if (ik0 < ik1) { slab_chunk_load<B>(bsr + size_t(diag_ptr[col[ik0]]) * bb, w, 0, pre); }
+  typename F::acc acc[M::TW];
+  typename Vec16<T>::type pre[M::S];
+  if (ik0 < ik1) { slabv_load<B, T>(bsr + size_t(diag_ptr[col[ik0]]) * bb, w, 0, pre); }
   for (int ik = ik0; ik < ik1; ++ik) {
-    double *aik = bsr + size_t(ik) * bb;
+    T *aik = bsr + size_t(ik) * bb;
     if (ik > ik0) { __syncthreads(); } // earlier updates of A_ik are visible; Xs is free
     for (int e = t; e < B * 16; e += 256) {
       const int r = e & 15, k = e >> 4;
-      Xs[e] = r0 + r < B ? aik[r0 + r + size_t(k) * B] : 0.;
+      Xs[e] = r0 + r < B ? aik[r0 + r + size_t(k) * B] : T(0);
     }
     lds_barrier();
     const int u0 = upd_ptr[ik], u1 = upd_ptr[ik + 1];
-    const double *dnext = ik + 1 < ik1 ? bsr + size_t(diag_ptr[col[ik + 1]]) * bb : nullptr;
-    slab_mfma<B>(bsr + size_t(diag_ptr[col[ik]]) * bb, u0 < u1 ? bsr + size_t(upd[2 * u0 + 1]) * bb : dnext, Xs,
-                 acc, pre); // L_ik = A_ik inv(A_kk)
+    const T *dnext = ik + 1 < ik1 ? bsr + size_t(diag_ptr[col[ik + 1]]) * bb : nullptr;
+    slabv_mfma<B, T>(bsr + size_t(diag_ptr[col[ik]]) * bb, u0 < u1 ? bsr + size_t(upd[2 * u0 + 1]) * bb : dnext, Xs,
+                     acc, pre); // L_ik = A_ik inv(A_kk)
     lds_barrier(); // every read of Xs is done
 #pragma unroll
     for (int tw = 0; tw < M::TW; ++tw) {
@@ -1143,7 +1222,7 @@ __global__ __launch_bounds__(256) void k_ilu_slab_m(double *__restrict__ bsr, co
       if (c < B) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int r = g + 4 * q;
+          const int r = F::row(g, q);
           Xs[c * 16 + r] = acc[tw][q];
           if (r0 + r < B) { aik[r0 + r + size_t(c) * B] = acc[tw][q]; }
         }
@@ -1151,25 +1230,25 @@ __global__ __launch_bounds__(256) void k_ilu_slab_m(double *__restrict__ bsr, co
     }
     lds_barrier();
     for (int u = u0; u < u1; ++u) {
-      double *tgt = bsr + size_t(upd[2 * u]) * bb;
-      double tv[M::TW][4];
+      T *tgt = bsr + size_t(upd[2 * u]) * bb;
+      T tv[M::TW][4];
 #pragma unroll
       for (int tw = 0; tw < M::TW; ++tw) {
         const int c = 16 * (w + 4 * tw) + j;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int r = r0 + g + 4 * q;
-          tv[tw][q] = (c < B && r < B) ? tgt[r + size_t(c) * B] : 0.;
+          const int r = r0 + F::row(g, q);
+          tv[tw][q] = (c < B && r < B) ? tgt[r + size_t(c) * B] : T(0);
         }
       }
-      slab_mfma<B>(bsr + size_t(upd[2 * u + 1]) * bb, u + 1 < u1 ? bsr + size_t(upd[2 * u + 3]) * bb : dnext, Xs,
-                   acc, pre); // L_ik A_kj
+      slabv_mfma<B, T>(bsr + size_t(upd[2 * u + 1]) * bb, u + 1 < u1 ? bsr + size_t(upd[2 * u + 3]) * bb : dnext,
+                       Xs, acc, pre); // L_ik A_kj
 #pragma unroll
       for (int tw = 0; tw < M::TW; ++tw) {
         const int c = 16 * (w + 4 * tw) + j;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int r = r0 + g + 4 * q;
+          const int r = r0 + F::row(g, q);
           if (c < B && r < B) { tgt[r + size_t(c) * B] = tv[tw][q] - acc[tw][q]; }
         }
       }
@@ -1248,35 +1327,37 @@ template <int B> struct ScaleM {
   static constexpr int LA = (PA + 1023) / 1024, LB = (PB + 1023) / 1024;
   static constexpr int KS = KP + 1;                 // B panel column stride (no bank conflicts across columns)
 };
-template <int B>
-__global__ __launch_bounds__(1024) void k_ilu_scale_upper_mfma(double *__restrict__ bsr,
-                                                               const int *__restrict__ slot,
+template <int B, class T = double>
+__global__ __launch_bounds__(1024) void k_ilu_scale_upper_mfma(T *__restrict__ bsr, const int *__restrict__ slot,
                                                                const int *__restrict__ dslot) {
   using M = ScaleM<B>;
-  __shared__ double As[M::PA];           // [k][r]
-  __shared__ double Bs[16 * M::TC * M::KS]; // [c][k], column stride KS
+  using F = Mfma16<T>;
+  __shared__ T As[M::PA];           // [k][r]
+  __shared__ T Bs[16 * M::TC * M::KS]; // [c][k], column stride KS
   constexpr size_t bb = size_t(B) * B;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63, g = lane >> 4, j = lane & 15;
-  const int blk = blockIdx.x / M::CG, cg = blockIdx.x % M::CG;
+  // the column groups of a block and the blocks of a row (consecutive
+  // slots) read the same inv(U_ii): one XCD each run
+  const int bid = xcd_run_id(blockIdx.x, gridDim.x), blk = bid / M::CG, cg = bid % M::CG;
   const int cstart = 16 * M::TC * cg, tcols = min(M::TC, M::TN - M::TC * cg);
-  const double *D = bsr + size_t(dslot[blk]) * bb;
-  double *U = bsr + size_t(slot[blk]) * bb;
-  auto load = [&](int p, double (&av)[M::LA], double (&bv)[M::LB]) {
+  const T *D = bsr + size_t(dslot[blk]) * bb;
+  T *U = bsr + size_t(slot[blk]) * bb;
+  auto load = [&](int p, T (&av)[M::LA], T (&bv)[M::LB]) {
 #pragma unroll
     for (int q = 0; q < M::LA; ++q) {
       const int e = t + 1024 * q, ka = e / M::BP, ra = e % M::BP, k = p * M::KP + ka;
-      av[q] = (e < M::PA && ra < B && k < B) ? D[ra + size_t(k) * B] : 0.;
+      av[q] = (e < M::PA && ra < B && k < B) ? D[ra + size_t(k) * B] : T(0);
     }
 #pragma unroll
     for (int q = 0; q < M::LB; ++q) {
       const int e = t + 1024 * q, cb = e / M::KP, kb = e % M::KP, k = p * M::KP + kb, c = cstart + cb;
-      bv[q] = (e < M::PB && c < B && k < B) ? U[k + size_t(c) * B] : 0.;
+      bv[q] = (e < M::PB && c < B && k < B) ? U[k + size_t(c) * B] : T(0);
     }
   };
-  slab_d4 acc[M::TW];
+  typename F::acc acc[M::TW];
 #pragma unroll
-  for (int tw = 0; tw < M::TW; ++tw) { acc[tw] = slab_d4{0., 0., 0., 0.}; }
-  double av[M::LA], bv[M::LB];
+  for (int tw = 0; tw < M::TW; ++tw) { acc[tw] = typename F::acc{T(0), T(0), T(0), T(0)}; }
+  T av[M::LA], bv[M::LB];
   load(0, av, bv);
   for (int p = 0; p < M::NP; ++p) {
     if (p > 0) { __syncthreads(); } // every read of the previous panel is done
@@ -1297,11 +1378,11 @@ __global__ __launch_bounds__(1024) void k_ilu_scale_upper_mfma(double *__restric
       const int tile = w + 16 * tw;
       if (tile < M::TN * tcols) {
         const int r0 = 16 * (tile % M::TN), c0 = 16 * (tile / M::TN);
-        slab_d4 a4 = acc[tw];
+        typename F::acc a4 = acc[tw];
 #pragma unroll
         for (int s4 = 0; s4 < M::KP / 4; ++s4) {
           const int k = 4 * s4 + g;
-          a4 = __builtin_amdgcn_mfma_f64_16x16x4f64(As[k * M::BP + r0 + j], Bs[(c0 + j) * M::KS + k], a4, 0, 0, 0);
+          a4 = F::mac(As[k * M::BP + r0 + j], Bs[(c0 + j) * M::KS + k], a4);
         }
         acc[tw] = a4;
       }
@@ -1315,7 +1396,7 @@ __global__ __launch_bounds__(1024) void k_ilu_scale_upper_mfma(double *__restric
       const int r0 = 16 * (tile % M::TN), c = cstart + 16 * (tile / M::TN) + j;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int r = r0 + g + 4 * q;
+        const int r = r0 + F::row(g, q);
         if (r < B && c < B) { U[r + size_t(c) * B] = acc[tw][q]; }
       }
     }
@@ -2349,9 +2430,40 @@ void launch_ilu_slab(double *bsr, int b, const int *rows, int nrows, const int *
           bsr, rows, row_ptr, diag_ptr, col, upd_ptr, upd);
       return;
     }
-    EG_DISPATCH_SLAB(b, (k_ilu_slab_m<B><<<nrows * SlabM<B>::NS, 256, SlabM<B>::lds(), s>>>(
+    EG_DISPATCH_SLAB(b, (k_ilu_slab_m<B><<<nrows * SlabV<B, double>::NS, 256, SlabV<B, double>::lds(), s>>>(
                             bsr, rows, row_ptr, diag_ptr, col, upd_ptr, upd)));
   }
+}
+bool ilu_factor32_ok(int b) { return b == 100 || b == 144 || b == 196; }
+void launch_ilu_slab32(float *bsr, int b, const int *rows, int nrows, const int *row_ptr, const int *diag_ptr,
+                       const int *col, const int *upd_ptr, const int *upd, hipStream_t s) {
+#define EG_SLAB32(BB)                                                                              \
+  case BB:                                                                                         \
+    k_ilu_slab_m<BB, float><<<nrows * SlabV<BB, float>::NS, 256, SlabV<BB, float>::lds(), s>>>(   \
+        bsr, rows, row_ptr, diag_ptr, col, upd_ptr, upd);                                          \
+    return;
+  switch (b) { EG_SLAB32(100) EG_SLAB32(144) EG_SLAB32(196) default: break; }
+#undef EG_SLAB32
+  throw std::runtime_error("FP32 ILU factor: unsupported block size");
+}
+void launch_block_inverse32(float *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
+  switch (b) {
+  case 100: k_block_inverse_w<100, float><<<count, gj2_threads<100>(), 0, s>>>(blocks, index, flags); return;
+  case 144: k_block_inverse_w<144, float><<<count, gj2_threads<144>(), 0, s>>>(blocks, index, flags); return;
+  case 196: k_block_inverse_w<196, float><<<count, gj2_threads<196>(), 0, s>>>(blocks, index, flags); return;
+  default: break;
+  }
+  throw std::runtime_error("FP32 block inverse: unsupported block size");
+}
+void launch_ilu_scale_upper32(float *bsr, int b, const int *slot, const int *dslot, int count, hipStream_t s) {
+  if (count <= 0) { return; }
+  switch (b) {
+  case 100: k_ilu_scale_upper_mfma<100, float><<<count * ScaleM<100>::CG, 1024, 0, s>>>(bsr, slot, dslot); return;
+  case 144: k_ilu_scale_upper_mfma<144, float><<<count * ScaleM<144>::CG, 1024, 0, s>>>(bsr, slot, dslot); return;
+  case 196: k_ilu_scale_upper_mfma<196, float><<<count * ScaleM<196>::CG, 1024, 0, s>>>(bsr, slot, dslot); return;
+  default: break;
+  }
+  throw std::runtime_error("FP32 scaled upper blocks: unsupported block size");
 }
 void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
   switch (b) {

@@ -165,6 +165,11 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
       sweep_alt.resize(N);
       if (mixed) { sweep32_alt.resize(N); }
     }
+    // EG_FACTOR32=0: the mixed form factors in FP64 and converts the factor
+    const char *f32 = std::getenv("EG_FACTOR32");
+    const char *slab = std::getenv("EG_SLAB");
+    factor32 = mixed && eg::ilu_factor32_ok(block) && !resident && !dataflow_factor &&
+               !(slab && std::string(slab) == "vector") && !(f32 && std::string(f32) == "0");
     FillMaps(*pattern, dslot, slot, contrib);
   } else if (!direct) {
     blocks.resize(size_t(nblk) * block * block);
@@ -365,47 +370,19 @@ void LinearSolver::Setup(bool check_flags) {
       eg::launch_hdg_diag_blocks(p.geo, hybrid->panels, hybrid->D, blocks, p.stream);
     }
   }
-  if (ilu) {
-    const auto &pt = *pattern;
-    bool df = dataflow_factor && eg::launch_ilu_factor_df(blocks, b, pt.nb, pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr,
-                                                   pt.d_upd, done_factor, ++epoch_factor, pt.d_lower_rows,
-                                                   done_factor.p + pt.nb, p.flags, df_grid, p.stream);
-    // row-resident programs: one launch for all levels (the rows wait on
-    // each other's completion flags) when the sweeps are single-launch too
-    if (!df && resident && dataflow && resident_df) {
-      eg::launch_arm<double>(factor_pub, factor_pub.n, p.stream);
-      df = eg::launch_ilu_factor_resident_df(blocks, factor_pub, b, d_prog_off, d_prog, pt.nb, done_factor.p + pt.nb,
-                                             resident_lds, p.flags, df_grid, p.stream);
+  if (factor32) {
+    // mixed precision: the FP32 factor of the converted blocks; the FP64
+    // factor waits until an FP64 apply asks for it
+    eg::launch_convert<double, float>(blocks, blocks32, blocks.n, p.stream);
+    ReplayLevels(factor_exec32, &LinearSolver::FactorLevels32);
+    if (scaled_upper) {
+      eg::launch_ilu_scale_upper32(blocks32, b, d_up_slot, d_up_diag, static_cast<int>(d_up_slot.n), p.stream);
     }
-    if (!df) {
-      // The per-level launches are recorded once into a graph and replayed
-      // (their arguments are fixed for the solver's lifetime), which removes
-      // most of the launch gaps between the levels.
-      if (factor_exec == nullptr && use_graph) {
-        hipGraph_t graph = nullptr;
-        bool ok = hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
-        if (ok) {
-          FactorLevels();
-          ok = hipStreamEndCapture(p.stream, &graph) == hipSuccess && graph != nullptr;
-        }
-        ok = ok && hipGraphInstantiate(&factor_exec, graph, nullptr, nullptr, 0) == hipSuccess;
-        if (graph) { (void)hipGraphDestroy(graph); }
-        if (!ok) {
-          (void)hipGetLastError();
-          factor_exec = nullptr;
-          use_graph = false;
-        }
-      }
-      if (factor_exec) { hip_check(hipGraphLaunch(factor_exec, p.stream), "hipGraphLaunch"); }
-      else { FactorLevels(); }
-    }
+    factor64_pending = true;
   } else {
-    eg::launch_block_inverse(blocks, b, N / b, nullptr, p.flags, p.stream);
+    Factor64();
+    if (mixed) { eg::launch_convert<double, float>(blocks, blocks32, blocks.n, p.stream); }
   }
-  if (scaled_upper) {
-    eg::launch_ilu_scale_upper(blocks, b, d_up_slot, d_up_diag, static_cast<int>(d_up_slot.n), p.stream);
-  }
-  if (mixed) { eg::launch_convert<double, float>(blocks, blocks32, blocks.n, p.stream); }
   if (ilu && dataflow) {
     eg::launch_arm<double>(sweep, N, p.stream);
     if (mixed) { eg::launch_arm<float>(sweep32, N, p.stream); }
@@ -418,6 +395,61 @@ void LinearSolver::Setup(bool check_flags) {
   const char *context = ilu ? "Singular BSR ILU diagonal" : "Singular block Jacobi factor";
   if (check_flags) { p.CheckFlags(context); }
   else { pending_check = context; }
+}
+void LinearSolver::Factor64() {
+  const int b = block;
+  if (ilu) {
+    const auto &pt = *pattern;
+    bool df = dataflow_factor && eg::launch_ilu_factor_df(blocks, b, pt.nb, pt.d_row, pt.d_diag, pt.d_col, pt.d_upd_ptr,
+                                                   pt.d_upd, done_factor, ++epoch_factor, pt.d_lower_rows,
+                                                   done_factor.p + pt.nb, p.flags, df_grid, p.stream);
+    // row-resident programs: one launch for all levels (the rows wait on
+    // each other's completion flags) when the sweeps are single-launch too
+    if (!df && resident && dataflow && resident_df) {
+      eg::launch_arm<double>(factor_pub, factor_pub.n, p.stream);
+      df = eg::launch_ilu_factor_resident_df(blocks, factor_pub, b, d_prog_off, d_prog, pt.nb, done_factor.p + pt.nb,
+                                             resident_lds, p.flags, df_grid, p.stream);
+    }
+    if (!df) { ReplayLevels(factor_exec, &LinearSolver::FactorLevels); }
+  } else {
+    eg::launch_block_inverse(blocks, b, N / b, nullptr, p.flags, p.stream);
+  }
+  if (scaled_upper) {
+    eg::launch_ilu_scale_upper(blocks, b, d_up_slot, d_up_diag, static_cast<int>(d_up_slot.n), p.stream);
+  }
+}
+// The per-level launches are recorded once into a graph and replayed (their
+// arguments are fixed for the solver's lifetime), which removes most of the
+// launch gaps between the levels.
+void LinearSolver::ReplayLevels(hipGraphExec_t &exec, void (LinearSolver::*levels)()) {
+  if (exec == nullptr && use_graph) {
+    hipGraph_t graph = nullptr;
+    bool ok = hipStreamBeginCapture(p.stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+      (this->*levels)();
+      ok = hipStreamEndCapture(p.stream, &graph) == hipSuccess && graph != nullptr;
+    }
+    ok = ok && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+    if (graph) { (void)hipGraphDestroy(graph); }
+    if (!ok) {
+      (void)hipGetLastError();
+      exec = nullptr;
+      use_graph = false;
+    }
+  }
+  if (exec) { hip_check(hipGraphLaunch(exec, p.stream), "hipGraphLaunch"); }
+  else { (this->*levels)(); }
+}
+// The FP32 factor of the mixed form, level by level (src/block_ilu.cpp:73-86
+// in single precision; the same products as the FP64 slab kernels).
+void LinearSolver::FactorLevels32() {
+  const auto &pt = *pattern;
+  for (size_t l = 0; l + 1 < pt.lower_level_ptr.size(); ++l) {
+    const int first = pt.lower_level_ptr[l], count = pt.lower_level_ptr[l + 1] - first;
+    eg::launch_ilu_slab32(blocks32, block, pt.d_lower_rows.p + first, count, pt.d_row, pt.d_diag, pt.d_col,
+                          pt.d_upd_ptr, pt.d_upd, p.stream);
+    eg::launch_block_inverse32(blocks32, block, count, pt.d_lower_diag.p + first, p.flags, p.stream);
+  }
 }
 
 // ILU(0) factor, one launch (or two) per dependency level (src/block_ilu.cpp:73-86).
@@ -455,6 +487,7 @@ void LinearSolver::FactorLevels() {
 }
 LinearSolver::~LinearSolver() {
   if (factor_exec) { (void)hipGraphExecDestroy(factor_exec); }
+  if (factor_exec32) { (void)hipGraphExecDestroy(factor_exec32); }
   for (hipEvent_t e : step_done) {
     if (e) { (void)hipEventDestroy(e); }
   }
@@ -470,7 +503,10 @@ bool LinearSolver::Batchable() const {
          pattern->perm.empty() && !assembled && pattern->max_deps <= 8 && block >= 4 && block <= 36 &&
          block % 4 == 0;
 }
-template <> const double *LinearSolver::Factors<double>() const { return blocks.p; }
+template <> const double *LinearSolver::Factors<double>() const {
+  require(!factor64_pending, "FP64 factor of a mixed-precision solver not computed");
+  return blocks.p;
+}
 template <> const float *LinearSolver::Factors<float>() const { return blocks32.p; }
 template <> double *LinearSolver::SweepBuffer<double>() { return sweep.p; }
 template <> float *LinearSolver::SweepBuffer<float>() { return sweep32.p; }
@@ -557,7 +593,16 @@ template <class T> static void sweeps(LinearSolver &s, const T *bsr, const T *b,
                                   s.p.opt.sweep_threads, s.scaled_upper, s.p.stream);
   }
 }
-void LinearSolver::Precondition(const double *b, double *y) { precondition<double>(*this, blocks, b, y); }
+void LinearSolver::Precondition(const double *b, double *y) {
+  if (factor64_pending) {
+    // the FP64 factor of a mixed-precision solver, on its first FP64 use
+    // after a set-up (an FP64 fallback solve or an FP64 apply)
+    factor64_pending = false;
+    Factor64();
+    p.CheckFlags("Singular BSR ILU diagonal");
+  }
+  precondition<double>(*this, blocks, b, y);
+}
 void LinearSolver::Precondition(const float *b, float *y) { precondition<float>(*this, blocks32, b, y); }
 
 double LinearSolver::Norm(const double *v) {

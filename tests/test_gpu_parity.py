@@ -385,6 +385,77 @@ def test_row_sliced_sweeps(gpu, oracle_lib, args, slices, precision, monkeypatch
         assert rel(y.double().cpu().numpy(), o.precondition(0, 1, b)) < (2e-4 if f32 else 1e-10), rep
 
 
+@pytest.mark.parametrize("args", ["--nx 6 --ny 6 --k 4", "--nx 6 --ny 5 --k 5", "--nx 5 --ny 4 --k 6"])
+def test_fp32_factor(gpu, oracle_lib, args, monkeypatch):
+    """The mixed form's FP32 ILU(0) factor of 100 / 144 / 196 cell blocks
+    (slab products on v_mfma_f32_16x16x4f32, FP32 Gauss-Jordan inverses and
+    upper scaling): its preconditioned vector against the oracle's FP64 one
+    and against the FP64 factor converted to FP32 (EG_FACTOR32=0), both to a
+    single-precision bound, on three applications in a row; and the FP64
+    apply of the same mixed-precision solver (its FP64 factor, computed on
+    first use) bitwise equal to an FP64 solver's."""
+    import torch
+    base = f"{args} --precond block-ilu0"
+    print(f"case: FP32 factor {base}", flush=True)
+    o = oracle_lib.Oracle(base)
+    u = perturbed(o)
+    du = torch.tensor(u, device="cuda")
+    o.freeze(u, 1e-3)
+    o.set_stage(u, 0.05)
+    o.dg_refresh(u)
+    ctx = {}
+    for name, extra, f32 in (("fp32", " --precision mixed", None), ("converted", " --precision mixed", "0"),
+                             ("fp64", "", None)):
+        if f32 is None:
+            monkeypatch.delenv("EG_FACTOR32", raising=False)
+        else:
+            monkeypatch.setenv("EG_FACTOR32", f32)
+        c = gpu.Context(base + extra)
+        c.freeze(du, 1e-3)
+        c.set_stage(du, 0.05)
+        c.call("eg_dg_refresh", gpu._ptr(du))
+        c.call("eg_pc_setup", 0)
+        ctx[name] = c
+    for rep in range(3):
+        b = np.sin((0.11 + 0.07 * rep) * (np.arange(o.nu) + 1))
+        ref = o.precondition(0, 1, b)
+        y = {}
+        for name in ("fp32", "converted"):
+            y[name] = torch.zeros(o.nu, dtype=torch.float32, device="cuda")
+            ctx[name].call("eg_pc_apply_f32", 0, gpu._ptr(torch.tensor(b, dtype=torch.float32, device="cuda")),
+                           gpu._ptr(y[name]))
+        e_oracle = rel(y["fp32"].double().cpu().numpy(), ref)
+        e_conv = rel(y["fp32"].double().cpu().numpy(), y["converted"].double().cpu().numpy())
+        print(f"rep {rep}: FP32 factor vs oracle {e_oracle:.2e}, vs converted FP64 factor {e_conv:.2e}")
+        assert e_oracle < 2e-4 and e_conv < 2e-4, rep
+    b = torch.tensor(np.cos(0.13 * (np.arange(o.nu) + 1)), device="cuda")
+    y64 = [torch.zeros(o.nu, dtype=torch.float64, device="cuda") for _ in range(2)]
+    ctx["fp32"].call("eg_pc_apply", 0, gpu._ptr(b), gpu._ptr(y64[0]))
+    ctx["fp64"].call("eg_pc_apply", 0, gpu._ptr(b), gpu._ptr(y64[1]))
+    assert torch.equal(y64[0], y64[1])
+
+
+@pytest.mark.parametrize("args", ["--nx 6 --ny 6 --k 4 --steps 1 --newton-steps 2",
+                                  "--nx 4 --ny 4 --k 5 --steps 1 --newton-steps 2",
+                                  "--nx 4 --ny 4 --k 6 --steps 1 --newton-steps 1"])
+def test_fp32_factor_run(gpu, oracle_lib, args):
+    """DG runs of the mixed form with the FP32 factor (k = 4, 5, 6): every
+    update within 1e-8 of the oracle's FP64 update, the final state within
+    1e-9."""
+    args = f"{args} --alpha-deg 30 --cfl .5 --restart 30 --precond block-ilu0"
+    print(f"case: mixed dg {args}", flush=True)
+    o = oracle_lib.Oracle(args)
+    ugo, so, inco, _ = o.run("dg", increments=16)
+    ug, sg, incg, _ = gpu.Context(args + " --precision mixed").run("dg", increments=16)
+    assert sg.updates == int(so[0])
+    metric, _ = o.metrics()
+    u_scale = float(np.sqrt(np.sum(metric * ugo * ugo)))
+    for i in range(sg.updates):
+        ok, e = increment_ok(o, incg[i], inco[i], u_scale, bound=1e-8)
+        assert ok, (i, e)
+    assert rel(ug, ugo) < 1e-9
+
+
 @pytest.mark.parametrize("args", ["--nx 16 --ny 16 --k 3", "--nx 5 --ny 3 --k 2", "--nx 4 --ny 4 --k 4",
                                   "--nx 3 --ny 3 --k 6", "--nx 64 --ny 64 --k 1"])
 @pytest.mark.parametrize("env", [("EG_SCALED_UPPER", "0"), ("EG_SLAB", "vector"), ("EG_FACTOR_GRAPH", "0"),
