@@ -397,20 +397,19 @@ __device__ __forceinline__ bool gj2_inverse(T *__restrict__ A, Gj2Lds<B, NCG, T>
         gj2_publish<B, NCG, T>(x, used_next, lane, L, buf ^ 1);
         __builtin_amdgcn_s_setprio(0);
       }
-      // the pivot row's values of this wave's columns (scalar registers)
-      T prow[NE];
+      // each pivot-row value of this wave's columns read (readlane) just
+      // before its column's update: two scalar registers live, not 2 NE
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
         if (ip == i) {
 #pragma unroll
-          for (int j = 0; j < NE; ++j) { prow[j] = read_lane(v[i][j], lp); }
+          for (int j = 0; j < NE; ++j) {
+            if ((ahead && j == j1) || (cg == s && j == 0)) { continue; }
+            const T pj = read_lane(v[i][j], lp);
+#pragma unroll
+            for (int r = 0; r < RW; ++r) { v[r][j] = fma(-m[r], pj, v[r][j]); }
+          }
         }
-      }
-#pragma unroll
-      for (int j = 0; j < NE; ++j) {
-        if ((ahead && j == j1) || (cg == s && j == 0)) { continue; }
-#pragma unroll
-        for (int i = 0; i < RW; ++i) { v[i][j] = fma(-m[i], prow[j], v[i][j]); }
       }
       if (cg == s) {
 #pragma unroll
@@ -1134,11 +1133,12 @@ __device__ __forceinline__ void slabv_load(const T *__restrict__ Y, int ct, int 
 template <int B, class T>
 __device__ __forceinline__ void slabv_mfma(const T *__restrict__ Y, const T *__restrict__ Ynext, const T *Xs,
                                            typename Mfma16<T>::acc (&acc)[SlabV<B, T>::TW],
-                                           typename Vec16<T>::type (&pre)[SlabV<B, T>::S]) {
+                                           typename Vec16<T>::type (&pre)[SlabV<B, T>::S], int w) {
   using M = SlabV<B, T>;
   using F = Mfma16<T>;
   using vec = typename Vec16<T>::type;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j = lane & 15;
+  // w: the wave (0..3) of the slab's workgroup
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
   vec cur[M::S];
 #pragma unroll
   for (int s = 0; s < M::S; ++s) { cur[s] = pre[s]; }
@@ -1185,19 +1185,20 @@ __device__ __forceinline__ void slab_chunk_load(const T *__restrict__ Y, int ct,
     y[s] = (k < B && c < B) ? Y[k + size_t(c) * B] : T(0);
   }
 }
-template <int B, class T = double>
-__global__ __launch_bounds__(256) void k_ilu_slab_m(T *__restrict__ bsr, const int *__restrict__ rows,
-                                                    const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
-                                                    const int *__restrict__ col, const int *__restrict__ upd_ptr,
-                                                    const int *__restrict__ upd) {
+// The products of rows [r0, r0 + 16) of block row i by the 256 threads of a
+// workgroup (t the thread index), the slab Xs in LDS. The wave index is
+// passed to the MFMA chains as a value (fewer live registers than recomputing
+// it there: 171 instead of 195 VGPRs at b = 196 in FP64, 135 instead of 153
+// in FP32).
+template <int B, class T>
+__device__ __forceinline__ void slab_m_row(T *__restrict__ bsr, int i, int r0, const int *__restrict__ row_ptr,
+                                           const int *__restrict__ diag_ptr, const int *__restrict__ col,
+                                           const int *__restrict__ upd_ptr, const int *__restrict__ upd, T *Xs,
+                                           int t) {
   using M = SlabV<B, T>;
   using F = Mfma16<T>;
-  extern __shared__ __attribute__((aligned(16))) unsigned char slab_lds[];
-  T *Xs = reinterpret_cast<T *>(slab_lds); // [k][16], k < KP
   constexpr size_t bb = size_t(B) * B;
-  const int bid = xcd_run_id(blockIdx.x, gridDim.x);
-  const int i = rows[bid / M::NS], r0 = (bid % M::NS) * 16;
-  const int t = threadIdx.x, w = t >> 6, lane = t & 63, g = lane >> 4, j = lane & 15;
+  const int w = t >> 6, lane = t & 63, g = lane >> 4, j = lane & 15;
   for (int e = B * 16 + t; e < M::KP * 16; e += 256) { Xs[e] = T(0); } // k >= B
   const int ik0 = row_ptr[i], ik1 = diag_ptr[i];
   typename F::acc acc[M::TW];
@@ -1214,7 +1215,7 @@ __global__ __launch_bounds__(256) void k_ilu_slab_m(T *__restrict__ bsr, const i
     const int u0 = upd_ptr[ik], u1 = upd_ptr[ik + 1];
     const T *dnext = ik + 1 < ik1 ? bsr + size_t(diag_ptr[col[ik + 1]]) * bb : nullptr;
     slabv_mfma<B, T>(bsr + size_t(diag_ptr[col[ik]]) * bb, u0 < u1 ? bsr + size_t(upd[2 * u0 + 1]) * bb : dnext, Xs,
-                     acc, pre); // L_ik = A_ik inv(A_kk)
+                     acc, pre, w); // L_ik = A_ik inv(A_kk)
     lds_barrier(); // every read of Xs is done
 #pragma unroll
     for (int tw = 0; tw < M::TW; ++tw) {
@@ -1242,7 +1243,7 @@ __global__ __launch_bounds__(256) void k_ilu_slab_m(T *__restrict__ bsr, const i
         }
       }
       slabv_mfma<B, T>(bsr + size_t(upd[2 * u + 1]) * bb, u + 1 < u1 ? bsr + size_t(upd[2 * u + 3]) * bb : dnext,
-                       Xs, acc, pre); // L_ik A_kj
+                       Xs, acc, pre, w); // L_ik A_kj
 #pragma unroll
       for (int tw = 0; tw < M::TW; ++tw) {
         const int c = 16 * (w + 4 * tw) + j;
@@ -1254,6 +1255,17 @@ __global__ __launch_bounds__(256) void k_ilu_slab_m(T *__restrict__ bsr, const i
       }
     }
   }
+}
+
+template <int B, class T = double>
+__global__ __launch_bounds__(256) void k_ilu_slab_m(T *__restrict__ bsr, const int *__restrict__ rows,
+                                                    const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
+                                                    const int *__restrict__ col, const int *__restrict__ upd_ptr,
+                                                    const int *__restrict__ upd) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char slab_lds[];
+  const int bid = xcd_run_id(blockIdx.x, gridDim.x);
+  slab_m_row<B, T>(bsr, rows[bid / SlabV<B, T>::NS], (bid % SlabV<B, T>::NS) * 16, row_ptr, diag_ptr, col, upd_ptr,
+                   upd, reinterpret_cast<T *>(slab_lds), threadIdx.x);
 }
 
 // M_ij = inv(U_ii) U_ij in place for every block right of the diagonal
