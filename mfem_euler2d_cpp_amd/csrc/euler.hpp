@@ -299,9 +299,9 @@ public:
   // launched directly).
   bool use_graph = true;
   hipGraphExec_t factor_exec = nullptr;
-  // Mixed precision with 100, 144 or 196 cell blocks: the factor is computed
-  // in FP32 on blocks32 (factor32, per-level launches in their own graph);
-  // the FP64 blocks keep the unfactored Jacobian blocks, factored only when
+  // Mixed precision with 100, 144 or 196 cell blocks: the blocks are filled
+  // in FP32 into blocks32 and factored there (factor32, per-level launches
+  // in their own graph); the FP64 blocks are filled and factored only when
   // an FP64 preconditioner apply asks for them (factor64_pending).
   bool factor32 = false;
   bool factor64_pending = false;

@@ -23,8 +23,11 @@ void launch_residual(const Geo &g, const Frozen *fz, const double *u, const doub
                      const double *old, double *r, double *trial, double *partial, int *flags, hipStream_t s);
 template <class T>
 void launch_dg_apply(const Geo &g, const Frozen *fz, const double *u, const T *x, T *y, hipStream_t s);
-void launch_dg_blocks(const Geo &g, const Frozen *fz, const double *u, double *diag, const int *dslot,
-                      double *bsr, const int *slot, hipStream_t s);
+// DG cell blocks, formed in FP64 and stored as T (double, or float for the
+// single-precision factor of the mixed form).
+template <class T>
+void launch_dg_blocks(const Geo &g, const Frozen *fz, const double *u, T *diag, const int *dslot, T *bsr,
+                      const int *slot, hipStream_t s);
 void launch_hdg_build(const Geo &g, const Frozen *fz, const double *u, double *A, double *C, hipStream_t s);
 void launch_hdg_solve(const Geo &g, const Frozen *fz, const double *Ainv, const double *res, double *Xy,
                       hipStream_t s);

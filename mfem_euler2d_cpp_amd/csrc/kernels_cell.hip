@@ -444,10 +444,10 @@ template <int Q> constexpr int dg_blocks_split() { return Q <= 4 ? 4 : 1; }
 // cell hide the memory latency; 1024 would spill registers), larger ones 256
 // (their register use needs it).
 template <int Q> constexpr int cell_threads() { return Q <= 4 ? 512 : 256; }
-template <int Q>
+template <int Q, class T>
 __global__ __launch_bounds__(256) void k_dg_blocks(Geo g, const Frozen *__restrict__ fz,
-                                                   const double *__restrict__ u, double *__restrict__ diag,
-                                                   const int *__restrict__ dslot, double *__restrict__ bsr,
+                                                   const double *__restrict__ u, T *__restrict__ diag,
+                                                   const int *__restrict__ dslot, T *__restrict__ bsr,
                                                    const int *__restrict__ slot) {
   constexpr int P = Q * Q, M = 4 * P, S = dg_blocks_split<Q>();
   __shared__ CellJac<Q> cj;
@@ -477,7 +477,8 @@ __global__ __launch_bounds__(256) void k_dg_blocks(Geo g, const Frozen *__restri
     }
   }
   __syncthreads();
-  double *dblk = diag + size_t(dslot ? dslot[e] : e) * M * M;
+  // entries formed in FP64, stored as T (FP32: the mixed form's factor)
+  T *dblk = diag + size_t(dslot ? dslot[e] : e) * M * M;
   for (int idx = part * blockDim.x + threadIdx.x; idx < M * M; idx += S * blockDim.x) {
     const int row = idx % M, col = idx / M;
     const int i = row / 4, c = row % 4, j = col / 4, d = col % 4;
@@ -490,7 +491,7 @@ __global__ __launch_bounds__(256) void k_dg_blocks(Geo g, const Frozen *__restri
     }
     v *= im;
     if (row == col) { v += 1.; }
-    dblk[idx] = v;
+    dblk[idx] = T(v);
     if (bsr) {
       for (int s = 0; s < 4; ++s) {
         int qq;
@@ -499,8 +500,8 @@ __global__ __launch_bounds__(256) void k_dg_blocks(Geo g, const Frozen *__restri
         const int sl = slot[e * 4 + s];
         bool first = true;
         for (int s2 = 0; s2 < s; ++s2) { first = first && slot[e * 4 + s2] != sl; }
-        double *blk = bsr + size_t(sl) * M * M;
-        blk[idx] = first ? o : blk[idx] + o;
+        T *blk = bsr + size_t(sl) * M * M;
+        blk[idx] = first ? T(o) : T(blk[idx] + o);
       }
     }
   }
@@ -839,10 +840,16 @@ template void launch_dg_apply<double>(const Geo &, const Frozen *, const double 
                                       hipStream_t);
 template void launch_dg_apply<float>(const Geo &, const Frozen *, const double *, const float *, float *,
                                      hipStream_t);
-void launch_dg_blocks(const Geo &g, const Frozen *fz, const double *u, double *diag, const int *dslot,
-                      double *bsr, const int *slot, hipStream_t s) {
-  EG_DISPATCH_Q(g.q, (k_dg_blocks<Q><<<g.n * dg_blocks_split<Q>(), 256, 0, s>>>(g, fz, u, diag, dslot, bsr, slot)));
+template <class T>
+void launch_dg_blocks(const Geo &g, const Frozen *fz, const double *u, T *diag, const int *dslot, T *bsr,
+                      const int *slot, hipStream_t s) {
+  EG_DISPATCH_Q(g.q,
+                (k_dg_blocks<Q, T><<<g.n * dg_blocks_split<Q>(), 256, 0, s>>>(g, fz, u, diag, dslot, bsr, slot)));
 }
+template void launch_dg_blocks<double>(const Geo &, const Frozen *, const double *, double *, const int *, double *,
+                                       const int *, hipStream_t);
+template void launch_dg_blocks<float>(const Geo &, const Frozen *, const double *, float *, const int *, float *,
+                                      const int *, hipStream_t);
 void launch_hdg_build(const Geo &g, const Frozen *fz, const double *u, double *A, double *C, hipStream_t s) {
   EG_DISPATCH_Q(g.q, (k_hdg_build<Q><<<g.n, cell_threads<Q>(), 0, s>>>(g, fz, u, A, C)));
 }

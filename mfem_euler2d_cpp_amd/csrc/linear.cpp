@@ -341,7 +341,7 @@ void LinearSolver::Setup(bool check_flags) {
     // src/linear.cpp:184-187: the assembled operator at the current state
     const auto &op = *op_pattern;
     if (form == 0) {
-      eg::launch_dg_blocks(p.geo, p.frozen, state, op_blocks, op_dslot, op_blocks, op_slot, p.stream);
+      eg::launch_dg_blocks<double>(p.geo, p.frozen, state, op_blocks, op_dslot, op_blocks, op_slot, p.stream);
     } else {
       eg::launch_hdg_fill(p.geo, hybrid->panels, hybrid->D, op_contrib, op.d_row, op.d_diag, op_blocks, p.stream);
     }
@@ -357,10 +357,13 @@ void LinearSolver::Setup(bool check_flags) {
     return;
   }
   if (form == 0) {
-    if (ilu) {
-      eg::launch_dg_blocks(p.geo, p.frozen, state, blocks, dslot, blocks, slot, p.stream);
+    if (factor32) {
+      // the single-precision factor's blocks, stored in FP32 directly
+      eg::launch_dg_blocks<float>(p.geo, p.frozen, state, blocks32, dslot, blocks32, slot, p.stream);
+    } else if (ilu) {
+      eg::launch_dg_blocks<double>(p.geo, p.frozen, state, blocks, dslot, blocks, slot, p.stream);
     } else {
-      eg::launch_dg_blocks(p.geo, p.frozen, state, blocks, nullptr, nullptr, nullptr, p.stream);
+      eg::launch_dg_blocks<double>(p.geo, p.frozen, state, blocks, nullptr, nullptr, nullptr, p.stream);
     }
   } else {
     if (ilu) {
@@ -371,9 +374,8 @@ void LinearSolver::Setup(bool check_flags) {
     }
   }
   if (factor32) {
-    // mixed precision: the FP32 factor of the converted blocks; the FP64
-    // factor waits until an FP64 apply asks for it
-    eg::launch_convert<double, float>(blocks, blocks32, blocks.n, p.stream);
+    // mixed precision: the FP32 factor of the FP32 blocks; the FP64 blocks
+    // and their factor wait until an FP64 apply asks for them
     ReplayLevels(factor_exec32, &LinearSolver::FactorLevels32);
     if (scaled_upper) {
       eg::launch_ilu_scale_upper32(blocks32, b, d_up_slot, d_up_diag, static_cast<int>(d_up_slot.n), p.stream);
@@ -595,9 +597,11 @@ template <class T> static void sweeps(LinearSolver &s, const T *bsr, const T *b,
 }
 void LinearSolver::Precondition(const double *b, double *y) {
   if (factor64_pending) {
-    // the FP64 factor of a mixed-precision solver, on its first FP64 use
-    // after a set-up (an FP64 fallback solve or an FP64 apply)
+    // the FP64 blocks and factor of a mixed-precision solver, on their first
+    // FP64 use after a set-up (an FP64 fallback solve or an FP64 apply); the
+    // refreshed state is still the set-up's
     factor64_pending = false;
+    eg::launch_dg_blocks<double>(p.geo, p.frozen, state, blocks, dslot, blocks, slot, p.stream);
     Factor64();
     p.CheckFlags("Singular BSR ILU diagonal");
   }

@@ -373,7 +373,7 @@ void Verify(const Options &base, const EulerGas &gas) {
     // the block fill (diagonal and face-coupling blocks) against the action
     LinearSolver dgi(pi, 0);
     dgi.Refresh(du);
-    eg::launch_dg_blocks(pi.geo, pi.frozen, du, dgi.blocks, dgi.dslot, dgi.blocks, dgi.slot, pi.stream);
+    eg::launch_dg_blocks<double>(pi.geo, pi.frozen, du, dgi.blocks, dgi.dslot, dgi.blocks, dgi.slot, pi.stream);
     const Dense J = dense_bsr(*dgi.pattern, down(pi, dgi.blocks, dgi.blocks.n));
     const double mfrel = rel(J.mult(v), z);
     // DG and HDG increments at the state
