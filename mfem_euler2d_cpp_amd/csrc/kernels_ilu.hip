@@ -819,12 +819,41 @@ __device__ __forceinline__ void resident_load(const double *bsr, const int *P, d
     }
   }
 }
-// The row program applied in LDS (ascending k: L_ik, then its update pairs)
-// and the diagonal block inverted; returns true for a singular pivot.
+// One 16 x 16 output tile (tr, tc) of the B x B block product X Y (B <= 36,
+// a multiple of 4) on the FP64 matrix cores (v_mfma_f64_16x16x4f64): lane
+// (g, j) supplies X(16 tr + j, 4 s + g) and Y(4 s + g, 16 tc + j) at step s;
+// entry q of the result is (X Y)(16 tr + g + 4 q, 16 tc + j).
+typedef double rf_d4 __attribute__((ext_vector_type(4)));
+template <int B> __device__ __forceinline__ rf_d4 block_mm_tile(const double *X, const double *Y, int tr, int tc) {
+  static_assert(B % 4 == 0, "whole k steps");
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  const int r = 16 * tr + j, c = 16 * tc + j;
+  rf_d4 acc = {0., 0., 0., 0.};
+#pragma unroll
+  for (int s = 0; s < B / 4; ++s) {
+    const int k = 4 * s + g;
+    const double a = r < B ? X[r + k * B] : 0.;
+    const double b = c < B ? Y[k + c * B] : 0.;
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+// Factor of one row program in LDS (src/block_ilu.cpp:73-86): for each lower
+// block in ascending k, L_ik = A_ik inv(U_kk) and then A_ij -= L_ik U_kj for
+// its update pairs; then the diagonal block's inverse by one wavefront.
+// Returns true for a singular pivot. Blocks from 20 x 20 form each product as
+// 16 x 16 matrix-core tiles, one tile a wavefront (the face factor at 48x48
+// Q4 / Q6: 3.96 -> 3.32 / 6.77 -> 4.85 ms); smaller ones, where a tile is a
+// latency chain on one wavefront, keep one entry a thread (block_dot; 1.63 /
+// 1.83 ms at Q2 / Q3 against 1.87 / 2.22 with tiles).
 template <int B>
 __device__ __forceinline__ bool resident_factor(const int *P, double *sm) {
-  constexpr int bb = B * B, NT = Shape<B>::NT, NQ = (bb + NT - 1) / NT;
-  const int t = threadIdx.x;
+  constexpr int bb = B * B, NT = Shape<B>::NT, T = (B + 15) / 16, NW = NT / 64, NQ = (bb + NT - 1) / NT;
+  constexpr bool tiles = B >= 20;
+  static_assert(!tiles || T * T <= NW, "one product tile a wavefront");
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63, g = lane >> 4, j = lane & 15;
+  const int tr = w % T, tc = w / T;
+  const bool tiler = w < T * T;
   const int nrow = P[2], dpos = P[3], ndep = P[4];
   double *Arow = sm, *Dep = sm + size_t(nrow) * bb;
   int pc = 5 + ndep;
@@ -834,33 +863,64 @@ __device__ __forceinline__ bool resident_factor(const int *P, double *sm) {
     pc = ops + 2 * nupd;
     double *X = Arow + size_t(l) * bb;
     const double *Y = Dep + size_t(inv) * bb;
-    double res[NQ];
+    if constexpr (!tiles) {
+      double res[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int o = t + q * NT;
-      res[q] = o < bb ? block_dot<B>(X, Y, o) : 0.;
-    }
-    __syncthreads();
+      for (int q = 0; q < NQ; ++q) {
+        const int o = t + q * NT;
+        res[q] = o < bb ? block_dot<B>(X, Y, o) : 0.;
+      }
+      __syncthreads();
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int o = t + q * NT;
-      if (o < bb) { X[o] = res[q]; }
-    }
-    __syncthreads();
-    // the update pairs of this lower block two at a time (independent
-    // accumulators, one read of L_ik for both)
+      for (int q = 0; q < NQ; ++q) {
+        const int o = t + q * NT;
+        if (o < bb) { X[o] = res[q]; }
+      }
+      __syncthreads();
+      // the update pairs of this lower block two at a time (independent
+      // accumulators, one read of L_ik for both)
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int o = t + q * NT;
-      if (o < bb) {
-        int u = 0;
-        for (; u + 1 < nupd; u += 2) {
-          double a0, a1;
-          block_dot2<B>(X, Dep + size_t(P[ops + 2 * u + 1]) * bb, Dep + size_t(P[ops + 2 * u + 3]) * bb, o, a0, a1);
-          Arow[size_t(P[ops + 2 * u]) * bb + o] -= a0;
-          Arow[size_t(P[ops + 2 * u + 2]) * bb + o] -= a1;
+      for (int q = 0; q < NQ; ++q) {
+        const int o = t + q * NT;
+        if (o < bb) {
+          int u = 0;
+          for (; u + 1 < nupd; u += 2) {
+            double a0, a1;
+            block_dot2<B>(X, Dep + size_t(P[ops + 2 * u + 1]) * bb, Dep + size_t(P[ops + 2 * u + 3]) * bb, o, a0,
+                          a1);
+            Arow[size_t(P[ops + 2 * u]) * bb + o] -= a0;
+            Arow[size_t(P[ops + 2 * u + 2]) * bb + o] -= a1;
+          }
+          if (u < nupd) {
+            Arow[size_t(P[ops + 2 * u]) * bb + o] -= block_dot<B>(X, Dep + size_t(P[ops + 2 * u + 1]) * bb, o);
+          }
         }
-        if (u < nupd) { Arow[size_t(P[ops + 2 * u]) * bb + o] -= block_dot<B>(X, Dep + size_t(P[ops + 2 * u + 1]) * bb, o); }
+      }
+      __syncthreads();
+      continue;
+    }
+    rf_d4 res = {0., 0., 0., 0.};
+    if (tiler) { res = block_mm_tile<B>(X, Y, tr, tc); }
+    __syncthreads(); // every read of A_ik is done
+    if (tiler) {
+      const int c = 16 * tc + j;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = 16 * tr + g + 4 * q;
+        if (r < B && c < B) { X[r + c * B] = res[q]; }
+      }
+    }
+    __syncthreads();
+    if (tiler) {
+      const int c = 16 * tc + j;
+      for (int u = 0; u < nupd; ++u) {
+        const rf_d4 prod = block_mm_tile<B>(X, Dep + size_t(P[ops + 2 * u + 1]) * bb, tr, tc);
+        double *tgt = Arow + size_t(P[ops + 2 * u]) * bb;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 16 * tr + g + 4 * q;
+          if (r < B && c < B) { tgt[r + c * B] -= prod[q]; }
+        }
       }
     }
     __syncthreads();
