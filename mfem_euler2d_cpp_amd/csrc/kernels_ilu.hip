@@ -1317,8 +1317,11 @@ __device__ __forceinline__ void slab_m_row(T *__restrict__ bsr, int i, int r0, c
   }
 }
 
+// Up to 196 x 196 blocks three waves a SIMD (at most 168 VGPRs): a level's
+// 13 slabs a row of 64 rows at 128x128 Q6 fit about one round of the device.
 template <int B, class T = double>
-__global__ __launch_bounds__(256) void k_ilu_slab_m(T *__restrict__ bsr, const int *__restrict__ rows,
+__global__ __launch_bounds__(256, B <= 196 ? 3 : 1) void k_ilu_slab_m(T *__restrict__ bsr,
+                                                                       const int *__restrict__ rows,
                                                     const int *__restrict__ row_ptr, const int *__restrict__ diag_ptr,
                                                     const int *__restrict__ col, const int *__restrict__ upd_ptr,
                                                     const int *__restrict__ upd) {
