@@ -645,7 +645,7 @@ __global__ __launch_bounds__(64 * B / PW) void k_block_inverse_p(double *__restr
   if (gj_panel<B, PW>(blocks + size_t(blk) * B * B, L, flags) && threadIdx.x == 0) { atomicOr(flags, kErrSingular); }
 }
 template <int B> struct Gj2Pick;
-template <> struct Gj2Pick<100> { static constexpr int NCG = 4; };
+template <> struct Gj2Pick<100> { static constexpr int NCG = 8; };
 template <> struct Gj2Pick<144> { static constexpr int NCG = 8; };
 template <> struct Gj2Pick<196> { static constexpr int NCG = 8; };
 template <int B> constexpr int gj2_threads() { return 64 * Gj2Pick<B>::NCG; }
