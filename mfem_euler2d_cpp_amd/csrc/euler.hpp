@@ -279,6 +279,12 @@ public:
   DeviceArray<double> sweep_alt;
   DeviceArray<float> sweep32_alt;
   int parity[2] = {0, 0};
+  // c = inv(U_ii) w_i of every row, formed between the two sliced sweeps by
+  // a block-diagonal product (given_c; EG_SWEEP_GIVEN_C=0: in the upper
+  // sweep, before each slice's wait).
+  bool given_c = false;
+  DeviceArray<double> sweep_c;
+  DeviceArray<float> sweep32_c;
   DeviceArray<double> factor_scratch; // large blocks: one block per row (or lower block) of the widest level
   // Large-block split factor plan, per dependency level l: lower blocks
   // [a_ptr[l], a_ptr[l+1]) and updated blocks [t_ptr[l], t_ptr[l+1]).

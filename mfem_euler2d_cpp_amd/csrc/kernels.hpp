@@ -73,7 +73,7 @@ void launch_block_permute(const T *src, T *dst, const int *perm, int b, int nb, 
 int la_init(); // nonzero if the large-LDS attribute could not be set (clears the error state)
 void launch_block_inverse(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s);
 template <class T>
-void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s);
+void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s, const int *slot = nullptr);
 // rows: the level's rows; row_diag: their diagonal block slots; scratch:
 // nrows b x b blocks, used for large blocks (big_block(b)) only.
 void launch_ilu_factor_level(double *bsr, int b, const int *rows, const int *row_diag, int nrows,
@@ -170,9 +170,11 @@ template <class T> void launch_arm(T *v, size_t n, hipStream_t s);
 // vector (narm values each). False when the kernel cannot be launched.
 bool ilu_sweep_slice_ok(int b, int max_deps);
 template <class T>
+// given_c (upper sweep): src holds c = inv(U_ii) w_i for every row already
+// (launch_block_gemv over the diagonal slots), not w.
 bool launch_ilu_sweep_slice(bool lower, const T *bsr, int b, int nb, int slices, const int *row_ptr,
                             const int *diag_ptr, const int *col, const T *src, T *dst, T *arm, long narm, int *flags,
-                            const int *order, int *ticket, int max_grid, hipStream_t s);
+                            const int *order, int *ticket, int max_grid, hipStream_t s, bool given_c = false);
 
 bool launch_ilu_factor_df(double *bsr, int b, int nb, const int *row_ptr, const int *diag_ptr, const int *col,
                           const int *upd_ptr, const int *upd, int *done, int epoch, const int *order, int *ticket,

@@ -2143,12 +2143,14 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_stream(const T *__restrict__
 // row's output from the whole dependency vectors, so the slices of a row
 // never wait on each other and the ticket order argument holds.
 // LOWER: y_i = b_i - sum L_ij y_j (src = b, dst = w; arms `arm`, the
-// upper sweep's output). UPPER (scaled): y_i = c_i - sum M_ij y_j with
-// c_i = inv(U_ii) w_i formed before the wait from the whole w_i (src = w,
-// complete: the lower sweep is an earlier launch), dst = y; it arms `arm`,
-// the next application's intermediate vector (the intermediate vectors
-// alternate: a slice cannot re-arm w_i while the others still read it).
-template <class T, bool LOWER>
+// upper sweep's output). UPPER (scaled): y_i = c_i - sum M_ij y_j, with
+// c_i = inv(U_ii) w_i given (CDIAG false: src = c, formed between the two
+// sweeps by a block-diagonal product) or formed here before the wait from
+// the whole w_i (CDIAG: src = w, complete: the lower sweep is an earlier
+// launch), dst = y; it arms `arm`, the next application's intermediate
+// vector (the intermediate vectors alternate: a slice cannot re-arm w_i
+// while the others still read it).
+template <class T, bool LOWER, bool CDIAG = !LOWER>
 __global__ __launch_bounds__(1024) void k_ilu_sweep_slice(const T *__restrict__ bsr, int b, int nb, int kd,
                                                           int slices, const int *__restrict__ row_ptr,
                                                           const int *__restrict__ diag_ptr,
@@ -2158,7 +2160,7 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_slice(const T *__restrict__ 
   // cell blocks: at most 4 neighbours a triangle. PRE: columns of each
   // dependency block loaded before the wait (5 covers a 196-row slice of 8:
   // no load after the wait; the FP64 upper sweep keeps 4, as 5 spills there)
-  constexpr int NT = 1024, PRE = (sizeof(T) == 8 && !LOWER) ? 4 : 5, KDM = 4;
+  constexpr int NT = 1024, PRE = (sizeof(T) == 8 && CDIAG) ? 4 : 5, KDM = 4;
   extern __shared__ __attribute__((aligned(16))) unsigned char slice_lds[];
   __shared__ int next_t;
   T *xs = reinterpret_cast<T *>(slice_lds); // kd x b dependency values
@@ -2189,7 +2191,7 @@ __global__ __launch_bounds__(1024) void k_ilu_sweep_slice(const T *__restrict__ 
       }
     }
     T own = T(0);
-    if (LOWER) {
+    if (!CDIAG) {
       if (part == 0 && rr < b) { own = src[size_t(i) * b + rr]; }
     } else {
       for (int c = tid; c < b; c += NT) { ws[c] = src[size_t(i) * b + c]; }
@@ -2642,11 +2644,12 @@ bool ilu_sweep_slice_ok(int b, int max_deps) { return (b == 100 || b == 144 || b
 template <class T>
 bool launch_ilu_sweep_slice(bool lower, const T *bsr, int b, int nb, int slices, const int *row_ptr,
                             const int *diag_ptr, const int *col, const T *src, T *dst, T *arm, long narm, int *flags,
-                            const int *order, int *ticket, int max_grid, hipStream_t s) {
+                            const int *order, int *ticket, int max_grid, hipStream_t s, bool given_c) {
   if (slices < 1 || slices > 8) { return false; }
   const int kd = 4, total = nb * slices;
-  const void *fn = lower ? reinterpret_cast<const void *>(k_ilu_sweep_slice<T, true>)
-                         : reinterpret_cast<const void *>(k_ilu_sweep_slice<T, false>);
+  const void *fn = lower     ? reinterpret_cast<const void *>(k_ilu_sweep_slice<T, true>)
+                   : given_c ? reinterpret_cast<const void *>(k_ilu_sweep_slice<T, false, false>)
+                             : reinterpret_cast<const void *>(k_ilu_sweep_slice<T, false, true>);
   const size_t lds = (size_t(kd) * b + 1024 + b) * sizeof(T);
   const int cap = coresident(fn, 1024, lds);
   int grid = max_grid > 0 ? std::min(total, max_grid * slices) : total;
@@ -2660,10 +2663,10 @@ bool launch_ilu_sweep_slice(bool lower, const T *bsr, int b, int nb, int slices,
 }
 template bool launch_ilu_sweep_slice<double>(bool, const double *, int, int, int, const int *, const int *,
                                              const int *, const double *, double *, double *, long, int *,
-                                             const int *, int *, int, hipStream_t);
+                                             const int *, int *, int, hipStream_t, bool);
 template bool launch_ilu_sweep_slice<float>(bool, const float *, int, int, int, const int *, const int *,
                                             const int *, const float *, float *, float *, long, int *, const int *,
-                                            int *, int, hipStream_t);
+                                            int *, int, hipStream_t, bool);
 template <class T> void launch_arm(T *v, size_t n, hipStream_t s) {
   k_arm<T><<<int(std::min<size_t>((n + 255) / 256, 4096)), 256, 0, s>>>(v, n);
 }

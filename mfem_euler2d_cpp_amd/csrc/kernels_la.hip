@@ -272,10 +272,12 @@ __global__ __launch_bounds__(kBigThreads) void k_ilu_products_big(double *__rest
   }
 }
 
-// y_blk = A_blk x_blk for count independent b x b blocks; one wave per block.
+// y_blk = A_blk x_blk for count independent b x b blocks (block blk at slot
+// slot[blk] when given, else at blk); one wave per block.
 template <class T>
 __global__ __launch_bounds__(256) void k_block_gemv(const T *__restrict__ blocks, int b, int count,
-                                                    const T *__restrict__ x, T *__restrict__ y) {
+                                                    const T *__restrict__ x, T *__restrict__ y,
+                                                    const int *__restrict__ slot) {
   extern __shared__ unsigned char raw[];
   T *xs = reinterpret_cast<T *>(raw);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -285,7 +287,7 @@ __global__ __launch_bounds__(256) void k_block_gemv(const T *__restrict__ blocks
   if (valid) { for (int i = lane; i < b; i += 64) { xw[i] = x[size_t(blk) * b + i]; } }
   __syncthreads();
   if (valid) {
-    const T *a = blocks + size_t(blk) * b * b;
+    const T *a = blocks + size_t(slot ? slot[blk] : blk) * b * b;
     for (int r = lane; r < b; r += 64) {
       T s = T(0);
       for (int c = 0; c < b; ++c) { s += a[size_t(c) * b + r] * xw[c]; }
@@ -620,11 +622,12 @@ void launch_block_inverse(double *blocks, int b, int count, const int *index, in
   k_block_inverse_big<<<count, kBigThreads, 0, s>>>(blocks, b, index, flags);
 }
 template <class T>
-void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s) {
-  k_block_gemv<T><<<(count + 3) / 4, 256, 4 * b * sizeof(T), s>>>(blocks, b, count, x, y);
+void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s, const int *slot) {
+  k_block_gemv<T><<<(count + 3) / 4, 256, 4 * b * sizeof(T), s>>>(blocks, b, count, x, y, slot);
 }
-template void launch_block_gemv<double>(const double *, int, int, const double *, double *, hipStream_t);
-template void launch_block_gemv<float>(const float *, int, int, const float *, float *, hipStream_t);
+template void launch_block_gemv<double>(const double *, int, int, const double *, double *, hipStream_t,
+                                        const int *);
+template void launch_block_gemv<float>(const float *, int, int, const float *, float *, hipStream_t, const int *);
 
 void launch_ilu_factor_level(double *bsr, int b, const int *rows, const int *row_diag, int nrows,
                              const int *row_ptr, const int *diag_ptr, const int *col, const int *upd_ptr,

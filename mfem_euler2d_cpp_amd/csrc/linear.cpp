@@ -174,6 +174,12 @@ LinearSolver::LinearSolver(Problem &prob, int f, Hybrid *hyb)
     if (slices > 0) {
       sweep_alt.resize(N);
       if (mixed) { sweep32_alt.resize(N); }
+      const char *gc = std::getenv("EG_SWEEP_GIVEN_C");
+      given_c = !(gc && std::string(gc) == "0");
+      if (given_c) {
+        sweep_c.resize(N);
+        if (mixed) { sweep32_c.resize(N); }
+      }
     }
     // EG_FACTOR32=0: the mixed form factors in FP64 and converts the factor
     const char *f32 = std::getenv("EG_FACTOR32");
@@ -571,12 +577,17 @@ template <class T> static void sweeps(LinearSolver &s, const T *bsr, const T *b,
       const int pi = sizeof(T) == 8 ? 0 : 1;
       T *alt = sizeof(T) == 8 ? reinterpret_cast<T *>(s.sweep_alt.p) : reinterpret_cast<T *>(s.sweep32_alt.p);
       T *cur = s.parity[pi] ? alt : mid, *next = s.parity[pi] ? mid : alt;
+      T *cv = sizeof(T) == 8 ? reinterpret_cast<T *>(s.sweep_c.p) : reinterpret_cast<T *>(s.sweep32_c.p);
       if (!eg::launch_ilu_sweep_slice<T>(true, bsr, s.block, pt.nb, s.slices, pt.d_row, pt.d_diag, pt.d_col, b, cur,
                                          y, s.N, s.p.flags, pt.d_lower_rows, s.sweep_ticket, s.df_grid,
-                                         s.p.stream) ||
-          !eg::launch_ilu_sweep_slice<T>(false, bsr, s.block, pt.nb, s.slices, pt.d_row, pt.d_diag, pt.d_col, cur,
-                                         y, next, s.N, s.p.flags, pt.d_upper_rows, s.sweep_ticket, s.df_grid,
                                          s.p.stream)) {
+        throw std::runtime_error("Launch of a row-sliced ILU sweep failed");
+      }
+      // c_i = inv(U_ii) w_i for every row at once (off the upper sweep's chain)
+      if (s.given_c) { eg::launch_block_gemv<T>(bsr, s.block, pt.nb, cur, cv, s.p.stream, pt.d_diag); }
+      if (!eg::launch_ilu_sweep_slice<T>(false, bsr, s.block, pt.nb, s.slices, pt.d_row, pt.d_diag, pt.d_col,
+                                         s.given_c ? cv : cur, y, next, s.N, s.p.flags, pt.d_upper_rows,
+                                         s.sweep_ticket, s.df_grid, s.p.stream, s.given_c)) {
         throw std::runtime_error("Launch of a row-sliced ILU sweep failed");
       }
       s.parity[pi] ^= 1;
