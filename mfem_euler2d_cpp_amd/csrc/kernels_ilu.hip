@@ -1331,6 +1331,314 @@ __global__ __launch_bounds__(256, B <= 196 ? 3 : 1) void k_ilu_slab_m(T *__restr
                    upd, reinterpret_cast<T *>(slab_lds), threadIdx.x);
 }
 
+// Panel-blocked Gauss-Jordan inverse on the matrix cores (B = 100, 144, 196).
+// The pivots and the stored form are those of gj2_inverse (the largest float
+// magnitude of the pivot column among the rows not used yet, unscaled rows,
+// A^{-1}(pinv[i], piv[j]) = scale_i S(i, j)), but the steps are applied four
+// at a time. One panel wave factors the four columns k0 .. k0 + 3 alone (rows
+// lane + 64 i in registers, column k replaced by -S(i,k) / S(p,k) and 1 at
+// row p as it is done) and hands over M' = M L^{-1}: M holds the four steps'
+// multipliers S(i,k) / S(p,k) (0 at the step's own pivot row), L is unit
+// lower with L(s,t) = M(p_s, t), the coupling of the steps through their
+// pivot rows. The four steps then take every other column c from S(:, c) to
+// S(:, c) - M' S(P, c), S(P, :) the four pivot rows before the panel: the
+// tile waves apply this as one v_mfma 16x16x4 a 16 x 16 tile and take the
+// panel's own columns from the panel wave. Tile wave w keeps tile rows TRW w
+// .. TRW w + TRW - 1 of the NT x NT tiles (zero padded) as MFMA accumulators
+// and rotates them by one tile column after each tile column, so that the
+// panel's tile column is register 0. The next panel's columns go to the
+// panel wave before this panel's update, with the pivot rows; the panel wave
+// applies the update to those four columns itself and factors the next panel
+// while the tile waves update every tile. Two barriers a panel of four steps.
+template <int B, class T> struct GjM {
+  static constexpr int NT = (B + 15) / 16;
+  static constexpr int TRW = 2;
+  static constexpr int NW = (NT + TRW - 1) / TRW; // tile waves; wave NW factors the panels
+  static constexpr int BP = 16 * NT;
+  static constexpr int RW = (BP + 63) / 64;
+  static constexpr int threads() { return 64 * (NW + 1); }
+  static_assert(B % 4 == 0, "panels of four columns");
+};
+template <int B, class T> struct GjMLds {
+  alignas(16) int pv[4];                    // the panel's pivot rows
+  alignas(16) T mp[2][GjM<B, T>::BP][4];    // -M', the MFMA A operand (by panel parity)
+  alignas(16) T pan[2][4][GjM<B, T>::BP];   // the panel's columns (by panel parity)
+  alignas(16) T prow[4][GjM<B, T>::BP];     // the pivot rows before the panel, the B operand
+  T scale[B];
+  int piv[B], pinv[B];
+};
+template <int B, class T> using GjMTiles = typename Mfma16<T>::acc[GjM<B, T>::TRW][GjM<B, T>::NT];
+// tile row r16 (0..15) of an accumulator: lane group and register
+template <class T> __device__ __forceinline__ int gjm_grp(int r16) { return sizeof(T) == 8 ? r16 & 3 : r16 >> 2; }
+template <class T> __device__ __forceinline__ int gjm_reg(int r16) { return sizeof(T) == 8 ? r16 >> 2 : r16 & 3; }
+// lanes j in [j0, j0 + 4) of register tile column TC hold a panel's columns
+template <int B, class T, int TC>
+__device__ __forceinline__ void gjm_gather(const GjMTiles<B, T> &v, T (&pan)[4][GjM<B, T>::BP], int w, int j0) {
+  using G = GjM<B, T>;
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  if (j >= j0 && j < j0 + 4) {
+#pragma unroll
+    for (int a = 0; a < G::TRW; ++a) {
+      const int tr = G::TRW * w + a;
+      if (tr < G::NT) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { pan[j - j0][16 * tr + Mfma16<T>::row(g, q)] = v[a][TC][q]; }
+      }
+    }
+  }
+}
+// the panel's factored columns into lanes j in [j0, j0 + 4) of register tile column 0
+template <int B, class T>
+__device__ __forceinline__ void gjm_replace(GjMTiles<B, T> &v, const T (&pan)[4][GjM<B, T>::BP], int w, int j0) {
+  using G = GjM<B, T>;
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  if (j >= j0 && j < j0 + 4) {
+#pragma unroll
+    for (int a = 0; a < G::TRW; ++a) {
+      const int tr = G::TRW * w + a;
+      if (tr < G::NT) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { v[a][0][q] = pan[j - j0][16 * tr + Mfma16<T>::row(g, q)]; }
+      }
+    }
+  }
+}
+// S -= M' S(P, :) on register tile columns [C0, C1) but SKIP (prow holds the
+// pivot rows by register tile column)
+template <int B, class T, int C0, int C1, int SKIP>
+__device__ __forceinline__ void gjm_update(GjMTiles<B, T> &v, const T (&am)[GjM<B, T>::TRW],
+                                           const T (&prow)[4][GjM<B, T>::BP], int w) {
+  using G = GjM<B, T>;
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+#pragma unroll
+  for (int tc = C0; tc < C1; ++tc) {
+    if (tc == SKIP) { continue; }
+    const T bo = prow[g][16 * tc + j];
+#pragma unroll
+    for (int a = 0; a < G::TRW; ++a) {
+      if (G::TRW * w + a < G::NT) { v[a][tc] = Mfma16<T>::mac(am[a], bo, v[a][tc]); }
+    }
+  }
+}
+// x[pi] of lane pl (pi, pl uniform): every register read from lane pl, the
+// choice made among the scalar values (no dynamically indexed array)
+template <class T, int RW> __device__ __forceinline__ T gjm_pick(const T (&x)[RW], int pi, int pl) {
+  T r = read_lane(x[0], pl);
+#pragma unroll
+  for (int i = 1; i < RW; ++i) {
+    const T c = read_lane(x[i], pl);
+    r = pi == i ? c : r;
+  }
+  return r;
+}
+template <int B, class T>
+__device__ __forceinline__ bool gjm_inverse(T *__restrict__ A, GjMLds<B, T> &L) {
+  using G = GjM<B, T>;
+  using F = Mfma16<T>;
+  constexpr int NT = G::NT, TRW = G::TRW, RW = G::RW, BP = G::BP;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  bool singular = false;
+  if (w < G::NW) {
+    GjMTiles<B, T> v;
+#pragma unroll
+    for (int a = 0; a < TRW; ++a) {
+      const int tr = TRW * w + a;
+#pragma unroll
+      for (int tc = 0; tc < NT; ++tc) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 16 * tr + F::row(g, q), c = 16 * tc + j;
+          v[a][tc][q] = (r < B && c < B) ? A[r + size_t(c) * B] : T(0);
+        }
+      }
+    }
+    gjm_gather<B, T, 0>(v, L.pan[0], w, 0);
+    lds_barrier(); // panel 0's columns are in pan[0]
+    for (int tcb = 0; tcb < NT; ++tcb) {
+#pragma unroll
+      for (int pp = 0; pp < 4; ++pp) {
+        const int k0 = 16 * tcb + 4 * pp;
+        if (k0 >= B) { break; }
+        const int buf = (k0 >> 2) & 1;
+        lds_barrier(); // the panel wave has published pv, mp[buf] and the factored panel
+        // the pivot rows before this panel's update
+        const int4 pvv = *reinterpret_cast<const int4 *>(L.pv);
+        const int ps[4] = {__builtin_amdgcn_readfirstlane(pvv.x), __builtin_amdgcn_readfirstlane(pvv.y),
+                           __builtin_amdgcn_readfirstlane(pvv.z), __builtin_amdgcn_readfirstlane(pvv.w)};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int trp = ps[s] >> 4;
+          if (trp / TRW == w) {
+            const int ap = trp % TRW, gp = gjm_grp<T>(ps[s] & 15), qp = gjm_reg<T>(ps[s] & 15);
+#pragma unroll
+            for (int a = 0; a < TRW; ++a) {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                if (a == ap && q == qp && g == gp) {
+#pragma unroll
+                  for (int tc = 0; tc < NT; ++tc) { L.prow[s][16 * tc + j] = v[a][tc][q]; }
+                }
+              }
+            }
+          }
+        }
+        // the next panel's columns before this panel's update (the panel
+        // wave applies the update to them itself)
+        if (k0 + 4 < B) {
+          if (pp < 3) { gjm_gather<B, T, 0>(v, L.pan[buf ^ 1], w, 4 * pp + 4); }
+          else { gjm_gather<B, T, 1>(v, L.pan[buf ^ 1], w, 0); }
+        }
+        lds_barrier(); // the pivot rows are in prow, the next panel in pan[buf ^ 1]
+        T am[TRW];
+#pragma unroll
+        for (int a = 0; a < TRW; ++a) { am[a] = TRW * w + a < NT ? L.mp[buf][16 * (TRW * w + a) + j][g] : T(0); }
+        gjm_update<B, T, 0, NT, -1>(v, am, L.prow, w);
+        gjm_replace<B, T>(v, L.pan[buf], w, 4 * pp);
+      }
+#pragma unroll
+      for (int a = 0; a < TRW; ++a) {
+        const typename F::acc first = v[a][0];
+#pragma unroll
+        for (int tc = 0; tc + 1 < NT; ++tc) { v[a][tc] = v[a][tc + 1]; }
+        v[a][NT - 1] = first;
+      }
+    }
+    lds_barrier(); // piv, pinv and scale are complete
+#pragma unroll
+    for (int a = 0; a < TRW; ++a) {
+      const int tr = TRW * w + a;
+#pragma unroll
+      for (int tc = 0; tc < NT; ++tc) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 16 * tr + F::row(g, q), c = 16 * tc + j;
+          if (r < B && c < B) { A[L.pinv[r] + size_t(L.piv[c]) * B] = L.scale[r] * v[a][tc][q]; }
+        }
+      }
+    }
+  } else {
+    unsigned used = 0u; // bit i: row lane + 64 i is a pivot row already
+    T mq[4][RW];        // -M' of the previous panel
+    lds_barrier();      // panel 0's columns are in pan[0]
+    for (int np = 0; np < B / 4; ++np) {
+      const int buf = np & 1, k0 = 4 * np;
+      T x[4][RW], m[4][RW], lc[4][4];
+      int ps[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+          const int r = lane + 64 * i;
+          x[c][i] = r < BP ? L.pan[buf][c][r] : T(0);
+        }
+      }
+      if (np > 0) {
+        // the previous panel's update of these columns, x -= M' S(P, cols):
+        // the pivot rows were stored by register tile column, where these
+        // columns were register tile 0, lanes 4 pq + 4 .., or tile 1 (pq = 3)
+        const int pq = (np - 1) & 3, base = pq < 3 ? 4 * pq + 4 : 16;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const T u0 = L.prow[0][base + c], u1 = L.prow[1][base + c], u2 = L.prow[2][base + c],
+                  u3 = L.prow[3][base + c];
+#pragma unroll
+          for (int i = 0; i < RW; ++i) {
+            x[c][i] = fma(mq[3][i], u3, fma(mq[2][i], u2, fma(mq[1][i], u1, fma(mq[0][i], u0, x[c][i]))));
+          }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        float key = -1.f;
+        int bi = 0;
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+          const int r = lane + 64 * i;
+          const float a = float(x[s][i]);
+          const float kk =
+              (r < B && !((used >> i) & 1u)) ? (a != a ? __int_as_float(0x7F800000) : fabsf(a)) : -1.f;
+          if (kk > key) { key = kk; bi = i; }
+        }
+        const int pl = wave_argmax(key, key < 0.f, lane);
+        const int pi = __builtin_amdgcn_readlane(bi, pl);
+        const int p = pl + 64 * pi;
+        // the pivot row's values in the four columns and in m[t] (t < s),
+        // read from lane pl under one uniform branch on its register pi
+        T pc[4], lm[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { pc[c] = lm[c] = T(0); }
+#pragma unroll
+        for (int i = 0; i < RW; ++i) {
+          if (pi == i) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) { pc[c] = read_lane(x[c][i], pl); }
+#pragma unroll
+            for (int t = 0; t < s; ++t) { lm[t] = read_lane(m[t][i], pl); }
+          }
+        }
+        const T inv = recip(pc[s]);
+        if (!(isfinite(inv) && inv != T(0))) { singular = true; }
+#pragma unroll
+        for (int t = 0; t < s; ++t) { lc[s][t] = lm[t]; }
+#pragma unroll
+        for (int i = 0; i < RW; ++i) { m[s][i] = lane + 64 * i == p ? T(0) : x[s][i] * inv; }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          if (c == s) { continue; }
+#pragma unroll
+          for (int i = 0; i < RW; ++i) { x[c][i] = fma(-m[s][i], pc[c], x[c][i]); }
+        }
+#pragma unroll
+        for (int i = 0; i < RW; ++i) { x[s][i] = lane + 64 * i == p ? T(1) : -m[s][i]; }
+        used |= lane == pl ? 1u << pi : 0u;
+        ps[s] = p;
+        if (lane == s) {
+          L.piv[k0 + s] = p;
+          L.pinv[p] = k0 + s;
+          L.scale[p] = inv;
+        }
+      }
+      if (lane == 0) { *reinterpret_cast<int4 *>(L.pv) = int4{ps[0], ps[1], ps[2], ps[3]}; }
+#pragma unroll
+      for (int i = 0; i < RW; ++i) {
+        const int r = lane + 64 * i;
+        const T m3 = m[3][i];
+        const T m2 = fma(-m3, lc[3][2], m[2][i]);
+        const T m1 = fma(-m3, lc[3][1], fma(-m2, lc[2][1], m[1][i]));
+        const T m0 = fma(-m3, lc[3][0], fma(-m2, lc[2][0], fma(-m1, lc[1][0], m[0][i])));
+        mq[0][i] = -m0;
+        mq[1][i] = -m1;
+        mq[2][i] = -m2;
+        mq[3][i] = -m3;
+        if (r < BP) {
+          using vec = typename Vec16<T>::type;
+          if constexpr (sizeof(T) == 8) {
+            *reinterpret_cast<vec *>(&L.mp[buf][r][0]) = vec{-m0, -m1};
+            *reinterpret_cast<vec *>(&L.mp[buf][r][2]) = vec{-m2, -m3};
+          } else {
+            *reinterpret_cast<vec *>(&L.mp[buf][r][0]) = vec{-m0, -m1, -m2, -m3};
+          }
+#pragma unroll
+          for (int c = 0; c < 4; ++c) { L.pan[buf][c][r] = x[c][i]; }
+        }
+      }
+      lds_barrier(); // pv, mp[buf] and the factored panel are published
+      lds_barrier(); // the pivot rows and the next panel's columns are in LDS
+    }
+    lds_barrier();
+  }
+  return singular;
+}
+template <int B, class T = double>
+__global__ __launch_bounds__((GjM<B, T>::threads())) void k_block_inverse_m(T *__restrict__ blocks,
+                                                                            const int *__restrict__ index,
+                                                                            int *__restrict__ flags) {
+  __shared__ GjMLds<B, T> L;
+  const int blk = index ? index[blockIdx.x] : blockIdx.x;
+  if (gjm_inverse<B, T>(blocks + size_t(blk) * B * B, L) && (threadIdx.x & 63) == 0) { atomicOr(flags, kErrSingular); }
+}
+
 // M_ij = inv(U_ii) U_ij in place for every block right of the diagonal
 // (blocks up to 64, after the factor), one workgroup a block: both blocks in
 // LDS, 4 x 4 output tiles a thread, the k sum in ascending order.
@@ -2523,7 +2831,21 @@ void launch_ilu_slab32(float *bsr, int b, const int *rows, int nrows, const int 
 #undef EG_SLAB32
   throw std::runtime_error("FP32 ILU factor: unsupported block size");
 }
+// The panel-blocked MFMA inverse for 100 / 144 / 196 blocks unless
+// EG_GJ_MFMA=0 (then the lane-per-row one), read per call.
+static bool gjm_on() {
+  const char *e = std::getenv("EG_GJ_MFMA");
+  return !(e && std::string(e) == "0");
+}
 void launch_block_inverse32(float *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
+  if (gjm_on()) {
+    switch (b) {
+    case 100: k_block_inverse_m<100, float><<<count, GjM<100, float>::threads(), 0, s>>>(blocks, index, flags); return;
+    case 144: k_block_inverse_m<144, float><<<count, GjM<144, float>::threads(), 0, s>>>(blocks, index, flags); return;
+    case 196: k_block_inverse_m<196, float><<<count, GjM<196, float>::threads(), 0, s>>>(blocks, index, flags); return;
+    default: break;
+    }
+  }
   switch (b) {
   case 100: k_block_inverse_w<100, float><<<count, gj2_threads<100>(), 0, s>>>(blocks, index, flags); return;
   case 144: k_block_inverse_w<144, float><<<count, gj2_threads<144>(), 0, s>>>(blocks, index, flags); return;
@@ -2543,6 +2865,14 @@ void launch_ilu_scale_upper32(float *bsr, int b, const int *slot, const int *dsl
   throw std::runtime_error("FP32 scaled upper blocks: unsupported block size");
 }
 void launch_block_inverse_fast(double *blocks, int b, int count, const int *index, int *flags, hipStream_t s) {
+  if (gjm_on()) {
+    switch (b) {
+    case 100: k_block_inverse_m<100><<<count, GjM<100, double>::threads(), 0, s>>>(blocks, index, flags); return;
+    case 144: k_block_inverse_m<144><<<count, GjM<144, double>::threads(), 0, s>>>(blocks, index, flags); return;
+    case 196: k_block_inverse_m<196><<<count, GjM<196, double>::threads(), 0, s>>>(blocks, index, flags); return;
+    default: break;
+    }
+  }
   switch (b) {
   case 64: k_block_inverse_p<64, 8><<<count, 512, 0, s>>>(blocks, index, flags); return;
   case 100: k_block_inverse_w<100><<<count, gj2_threads<100>(), 0, s>>>(blocks, index, flags); return;

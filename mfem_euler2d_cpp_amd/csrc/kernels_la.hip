@@ -295,6 +295,31 @@ __global__ __launch_bounds__(256) void k_block_gemv(const T *__restrict__ blocks
     }
   }
 }
+// The same product for large blocks (b > 64) with one 256-thread workgroup a
+// block: wave w sums the columns c = w mod 4 for the rows lane + 64 k (the
+// loads of a column coalesced), the four partial sums added in LDS in wave
+// order.
+template <class T>
+__global__ __launch_bounds__(256) void k_block_gemv_wg(const T *__restrict__ blocks, int b,
+                                                       const T *__restrict__ x, T *__restrict__ y,
+                                                       const int *__restrict__ slot) {
+  extern __shared__ unsigned char raw[];
+  T *xs = reinterpret_cast<T *>(raw); // b values of x, then 4 b partial sums
+  T *part = xs + b;
+  const int blk = blockIdx.x, wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = threadIdx.x; i < b; i += 256) { xs[i] = x[size_t(blk) * b + i]; }
+  __syncthreads();
+  const T *a = blocks + size_t(slot ? slot[blk] : blk) * b * b;
+  for (int r = lane; r < b; r += 64) {
+    T s = T(0);
+    for (int c = wid; c < b; c += 4) { s += a[size_t(c) * b + r] * xs[c]; }
+    part[wid * b + r] = s;
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < b; r += 256) {
+    y[size_t(blk) * b + r] = ((part[r] + part[b + r]) + part[2 * b + r]) + part[3 * b + r];
+  }
+}
 
 // ------------------------------------------------------------ Krylov vectors
 // Partial sums over a fixed grid of reduce_blocks(n) blocks (grid-stride),
@@ -623,6 +648,10 @@ void launch_block_inverse(double *blocks, int b, int count, const int *index, in
 }
 template <class T>
 void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s, const int *slot) {
+  if (b > 64) {
+    k_block_gemv_wg<T><<<count, 256, 5 * b * sizeof(T), s>>>(blocks, b, x, y, slot);
+    return;
+  }
   k_block_gemv<T><<<(count + 3) / 4, 256, 4 * b * sizeof(T), s>>>(blocks, b, count, x, y, slot);
 }
 template void launch_block_gemv<double>(const double *, int, int, const double *, double *, hipStream_t,
