@@ -295,29 +295,35 @@ __global__ __launch_bounds__(256) void k_block_gemv(const T *__restrict__ blocks
     }
   }
 }
-// The same product for large blocks (b > 64) with one 256-thread workgroup a
-// block: wave w sums the columns c = w mod 4 for the rows lane + 64 k (the
-// loads of a column coalesced), the four partial sums added in LDS in wave
-// order.
+// The same product for large blocks (b > 64, b a multiple of 4) with one
+// 256-thread workgroup a block and 16-byte loads: thread t takes the V = 16 /
+// sizeof(T) rows V p .. V p + V - 1 (p = t mod b / V) of the columns c = t /
+// (b / V) mod G, G = 256 / (b / V) column groups, and the G partial sums of a
+// row are added in LDS in group order.
 template <class T>
 __global__ __launch_bounds__(256) void k_block_gemv_wg(const T *__restrict__ blocks, int b,
                                                        const T *__restrict__ x, T *__restrict__ y,
                                                        const int *__restrict__ slot) {
+  constexpr int V = 16 / sizeof(T);
+  typedef T vec __attribute__((ext_vector_type(V)));
   extern __shared__ unsigned char raw[];
-  T *xs = reinterpret_cast<T *>(raw); // b values of x, then 4 b partial sums
+  T *xs = reinterpret_cast<T *>(raw); // b values of x, then G b partial sums
   T *part = xs + b;
-  const int blk = blockIdx.x, wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = threadIdx.x; i < b; i += 256) { xs[i] = x[size_t(blk) * b + i]; }
+  const int blk = blockIdx.x, t = threadIdx.x, np = b / V, ng = 256 / np, p = t % np, grp = t / np;
+  for (int i = t; i < b; i += 256) { xs[i] = x[size_t(blk) * b + i]; }
   __syncthreads();
-  const T *a = blocks + size_t(slot ? slot[blk] : blk) * b * b;
-  for (int r = lane; r < b; r += 64) {
-    T s = T(0);
-    for (int c = wid; c < b; c += 4) { s += a[size_t(c) * b + r] * xs[c]; }
-    part[wid * b + r] = s;
+  if (grp < ng) {
+    const T *a = blocks + size_t(slot ? slot[blk] : blk) * b * b + V * p;
+    vec acc = vec(T(0));
+    for (int c = grp; c < b; c += ng) { acc += *reinterpret_cast<const vec *>(a + size_t(c) * b) * xs[c]; }
+#pragma unroll
+    for (int e = 0; e < V; ++e) { part[grp * b + V * p + e] = acc[e]; }
   }
   __syncthreads();
-  for (int r = threadIdx.x; r < b; r += 256) {
-    y[size_t(blk) * b + r] = ((part[r] + part[b + r]) + part[2 * b + r]) + part[3 * b + r];
+  for (int r = t; r < b; r += 256) {
+    T s = part[r];
+    for (int k = 1; k < ng; ++k) { s += part[k * b + r]; }
+    y[size_t(blk) * b + r] = s;
   }
 }
 
@@ -648,8 +654,9 @@ void launch_block_inverse(double *blocks, int b, int count, const int *index, in
 }
 template <class T>
 void launch_block_gemv(const T *blocks, int b, int count, const T *x, T *y, hipStream_t s, const int *slot) {
-  if (b > 64) {
-    k_block_gemv_wg<T><<<count, 256, 5 * b * sizeof(T), s>>>(blocks, b, x, y, slot);
+  if (b > 64 && b % 4 == 0) {
+    const int groups = 256 / (b / (16 / int(sizeof(T))));
+    k_block_gemv_wg<T><<<count, 256, (groups + 1) * b * sizeof(T), s>>>(blocks, b, x, y, slot);
     return;
   }
   k_block_gemv<T><<<(count + 3) / 4, 256, 4 * b * sizeof(T), s>>>(blocks, b, count, x, y, slot);
