@@ -1403,21 +1403,24 @@ __device__ __forceinline__ void gjm_replace(GjMTiles<B, T> &v, const T (&pan)[4]
     }
   }
 }
-// S -= M' S(P, :) on register tile columns [C0, C1) but SKIP (prow holds the
-// pivot rows by register tile column)
-template <int B, class T, int C0, int C1, int SKIP>
+// S -= M' S(P, :) on every register tile column (prow holds the pivot rows
+// by register tile column). The B operand of the next tile column is read
+// from LDS while the MFMAs of this one issue. A tile row past the last (the
+// last wave's, when NT is odd) has a zero A operand and stays zero; it costs
+// no time, as that wave shares its SIMD with a wave of two tile rows.
+template <int B, class T>
 __device__ __forceinline__ void gjm_update(GjMTiles<B, T> &v, const T (&am)[GjM<B, T>::TRW],
                                            const T (&prow)[4][GjM<B, T>::BP], int w) {
   using G = GjM<B, T>;
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
+  const T *pr = &prow[g][j];
+  T bn = pr[0];
 #pragma unroll
-  for (int tc = C0; tc < C1; ++tc) {
-    if (tc == SKIP) { continue; }
-    const T bo = prow[g][16 * tc + j];
+  for (int tc = 0; tc < G::NT; ++tc) {
+    const T bo = bn;
+    if (tc + 1 < G::NT) { bn = pr[16 * (tc + 1)]; }
 #pragma unroll
-    for (int a = 0; a < G::TRW; ++a) {
-      if (G::TRW * w + a < G::NT) { v[a][tc] = Mfma16<T>::mac(am[a], bo, v[a][tc]); }
-    }
+    for (int a = 0; a < G::TRW; ++a) { v[a][tc] = Mfma16<T>::mac(am[a], bo, v[a][tc]); }
   }
 }
 // x[pi] of lane pl (pi, pl uniform): every register read from lane pl, the
@@ -1493,7 +1496,7 @@ __device__ __forceinline__ bool gjm_inverse(T *__restrict__ A, GjMLds<B, T> &L) 
         T am[TRW];
 #pragma unroll
         for (int a = 0; a < TRW; ++a) { am[a] = TRW * w + a < NT ? L.mp[buf][16 * (TRW * w + a) + j][g] : T(0); }
-        gjm_update<B, T, 0, NT, -1>(v, am, L.prow, w);
+        gjm_update<B, T>(v, am, L.prow, w);
         gjm_replace<B, T>(v, L.pan[buf], w, 4 * pp);
       }
 #pragma unroll
