@@ -1346,10 +1346,12 @@ __global__ __launch_bounds__(256, B <= 196 ? 3 : 1) void k_ilu_slab_m(T *__restr
 // panel's own columns from the panel wave. Tile wave w keeps tile rows TRW w
 // .. TRW w + TRW - 1 of the NT x NT tiles (zero padded) as MFMA accumulators
 // and rotates them by one tile column after each tile column, so that the
-// panel's tile column is register 0. The next panel's columns go to the
-// panel wave before this panel's update, with the pivot rows; the panel wave
-// applies the update to those four columns itself and factors the next panel
-// while the tile waves update every tile. Two barriers a panel of four steps.
+// panel's tile column is register 0. After the update of panel np the tile
+// waves hand the columns of panel np + 2 to the panel wave, which applies
+// the update of np + 1 to them itself (the pivot rows' values are in those
+// columns) and factors them while the tile waves write the pivot rows of np +
+// 1, wait for each other (an LDS count, not a barrier) and update every
+// tile. One barrier a panel of four steps.
 template <int B, class T> struct GjM {
   static constexpr int NT = (B + 15) / 16;
   static constexpr int TRW = 2;
@@ -1360,13 +1362,28 @@ template <int B, class T> struct GjM {
   static_assert(B % 4 == 0, "panels of four columns");
 };
 template <int B, class T> struct GjMLds {
-  alignas(16) int pv[4];                    // the panel's pivot rows
+  alignas(16) int pv[2][4];                 // the panel's pivot rows (by panel parity)
+  int cnt;                                  // tile waves past their pivot-row writes
   alignas(16) T mp[2][GjM<B, T>::BP][4];    // -M', the MFMA A operand (by panel parity)
-  alignas(16) T pan[2][4][GjM<B, T>::BP];   // the panel's columns (by panel parity)
+  alignas(16) T pan[3][4][GjM<B, T>::BP];   // a panel's columns (by panel index mod 3)
   alignas(16) T prow[4][GjM<B, T>::BP];     // the pivot rows before the panel, the B operand
   T scale[B];
   int piv[B], pinv[B];
 };
+// Wait of the tile waves for each other (the panel wave takes no part): each
+// counts itself once its LDS writes are done, then waits for target counts.
+// The wait is bounded (kErrSpin past 2^20 polls); every tile wave counts
+// itself once a panel, so the count always arrives.
+__device__ __forceinline__ int gjm_tile_sync(int *cnt, int target) {
+  if ((threadIdx.x & 63) == 0) { __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+  int spins = 0;
+  while (__builtin_amdgcn_readfirstlane(__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP)) <
+         target) {
+    if (++spins > (1 << 20)) { return kErrSpin; }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return 0;
+}
 template <int B, class T> using GjMTiles = typename Mfma16<T>::acc[GjM<B, T>::TRW][GjM<B, T>::NT];
 // tile row r16 (0..15) of an accumulator: lane group and register
 template <class T> __device__ __forceinline__ int gjm_grp(int r16) { return sizeof(T) == 8 ? r16 & 3 : r16 >> 2; }
@@ -1435,13 +1452,14 @@ template <class T, int RW> __device__ __forceinline__ T gjm_pick(const T (&x)[RW
   return r;
 }
 template <int B, class T>
-__device__ __forceinline__ bool gjm_inverse(T *__restrict__ A, GjMLds<B, T> &L) {
+__device__ __forceinline__ int gjm_inverse(T *__restrict__ A, GjMLds<B, T> &L) {
   using G = GjM<B, T>;
   using F = Mfma16<T>;
   constexpr int NT = G::NT, TRW = G::TRW, RW = G::RW, BP = G::BP;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63, g = lane >> 4, j = lane & 15;
-  bool singular = false;
+  int err = 0;
+  if (threadIdx.x == 0) { L.cnt = 0; }
   if (w < G::NW) {
     GjMTiles<B, T> v;
 #pragma unroll
@@ -1456,17 +1474,19 @@ __device__ __forceinline__ bool gjm_inverse(T *__restrict__ A, GjMLds<B, T> &L) 
         }
       }
     }
+    // panels 0 and 1 before any update
     gjm_gather<B, T, 0>(v, L.pan[0], w, 0);
-    lds_barrier(); // panel 0's columns are in pan[0]
+    gjm_gather<B, T, 0>(v, L.pan[1], w, 4);
+    lds_barrier(); // panels 0 and 1 are in pan[0], pan[1]; cnt is zero
     for (int tcb = 0; tcb < NT; ++tcb) {
 #pragma unroll
       for (int pp = 0; pp < 4; ++pp) {
         const int k0 = 16 * tcb + 4 * pp;
         if (k0 >= B) { break; }
-        const int buf = (k0 >> 2) & 1;
-        lds_barrier(); // the panel wave has published pv, mp[buf] and the factored panel
+        const int np = k0 >> 2, par = np & 1, b3 = np % 3;
+        lds_barrier(); // the panel wave has published pv[par], mp[par] and the factored panel in pan[b3]
         // the pivot rows before this panel's update
-        const int4 pvv = *reinterpret_cast<const int4 *>(L.pv);
+        const int4 pvv = *reinterpret_cast<const int4 *>(L.pv[par]);
         const int ps[4] = {__builtin_amdgcn_readfirstlane(pvv.x), __builtin_amdgcn_readfirstlane(pvv.y),
                            __builtin_amdgcn_readfirstlane(pvv.z), __builtin_amdgcn_readfirstlane(pvv.w)};
 #pragma unroll
@@ -1486,18 +1506,18 @@ __device__ __forceinline__ bool gjm_inverse(T *__restrict__ A, GjMLds<B, T> &L) 
             }
           }
         }
-        // the next panel's columns before this panel's update (the panel
-        // wave applies the update to them itself)
-        if (k0 + 4 < B) {
-          if (pp < 3) { gjm_gather<B, T, 0>(v, L.pan[buf ^ 1], w, 4 * pp + 4); }
-          else { gjm_gather<B, T, 1>(v, L.pan[buf ^ 1], w, 0); }
-        }
-        lds_barrier(); // the pivot rows are in prow, the next panel in pan[buf ^ 1]
+        err |= gjm_tile_sync(&L.cnt, G::NW * (np + 1)); // every tile wave's pivot rows are in prow
         T am[TRW];
 #pragma unroll
-        for (int a = 0; a < TRW; ++a) { am[a] = TRW * w + a < NT ? L.mp[buf][16 * (TRW * w + a) + j][g] : T(0); }
+        for (int a = 0; a < TRW; ++a) { am[a] = TRW * w + a < NT ? L.mp[par][16 * (TRW * w + a) + j][g] : T(0); }
         gjm_update<B, T>(v, am, L.prow, w);
-        gjm_replace<B, T>(v, L.pan[buf], w, 4 * pp);
+        gjm_replace<B, T>(v, L.pan[b3], w, 4 * pp);
+        // the columns of panel np + 2 after this update, for the panel wave
+        // (which applies the update of panel np + 1 to them itself)
+        if (k0 + 8 < B) {
+          if (pp < 2) { gjm_gather<B, T, 0>(v, L.pan[(np + 2) % 3], w, 4 * pp + 8); }
+          else { gjm_gather<B, T, 1>(v, L.pan[(np + 2) % 3], w, 4 * pp - 8); }
+        }
       }
 #pragma unroll
       for (int a = 0; a < TRW; ++a) {
@@ -1521,11 +1541,13 @@ __device__ __forceinline__ bool gjm_inverse(T *__restrict__ A, GjMLds<B, T> &L) 
       }
     }
   } else {
+    bool singular = false;
     unsigned used = 0u; // bit i: row lane + 64 i is a pivot row already
     T mq[4][RW];        // -M' of the previous panel
-    lds_barrier();      // panel 0's columns are in pan[0]
+    int pq[4] = {0, 0, 0, 0}; // its pivot rows
+    lds_barrier();      // panels 0 and 1 are in pan[0], pan[1]
     for (int np = 0; np < B / 4; ++np) {
-      const int buf = np & 1, k0 = 4 * np;
+      const int par = np & 1, b3 = np % 3, k0 = 4 * np;
       T x[4][RW], m[4][RW], lc[4][4];
       int ps[4];
 #pragma unroll
@@ -1533,18 +1555,17 @@ __device__ __forceinline__ bool gjm_inverse(T *__restrict__ A, GjMLds<B, T> &L) 
 #pragma unroll
         for (int i = 0; i < RW; ++i) {
           const int r = lane + 64 * i;
-          x[c][i] = r < BP ? L.pan[buf][c][r] : T(0);
+          x[c][i] = r < BP ? L.pan[b3][c][r] : T(0);
         }
       }
       if (np > 0) {
-        // the previous panel's update of these columns, x -= M' S(P, cols):
-        // the pivot rows were stored by register tile column, where these
-        // columns were register tile 0, lanes 4 pq + 4 .., or tile 1 (pq = 3)
-        const int pq = (np - 1) & 3, base = pq < 3 ? 4 * pq + 4 : 16;
+        // the previous panel's update of these columns, x -= M' S(P, cols),
+        // with the pivot rows' values taken from these columns themselves
+        // (they are the values before that update)
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const T u0 = L.prow[0][base + c], u1 = L.prow[1][base + c], u2 = L.prow[2][base + c],
-                  u3 = L.prow[3][base + c];
+          const T u0 = L.pan[b3][c][pq[0]], u1 = L.pan[b3][c][pq[1]], u2 = L.pan[b3][c][pq[2]],
+                  u3 = L.pan[b3][c][pq[3]];
 #pragma unroll
           for (int i = 0; i < RW; ++i) {
             x[c][i] = fma(mq[3][i], u3, fma(mq[2][i], u2, fma(mq[1][i], u1, fma(mq[0][i], u0, x[c][i]))));
@@ -1602,7 +1623,7 @@ __device__ __forceinline__ bool gjm_inverse(T *__restrict__ A, GjMLds<B, T> &L) 
           L.scale[p] = inv;
         }
       }
-      if (lane == 0) { *reinterpret_cast<int4 *>(L.pv) = int4{ps[0], ps[1], ps[2], ps[3]}; }
+      if (lane == 0) { *reinterpret_cast<int4 *>(L.pv[par]) = int4{ps[0], ps[1], ps[2], ps[3]}; }
 #pragma unroll
       for (int i = 0; i < RW; ++i) {
         const int r = lane + 64 * i;
@@ -1617,21 +1638,23 @@ __device__ __forceinline__ bool gjm_inverse(T *__restrict__ A, GjMLds<B, T> &L) 
         if (r < BP) {
           using vec = typename Vec16<T>::type;
           if constexpr (sizeof(T) == 8) {
-            *reinterpret_cast<vec *>(&L.mp[buf][r][0]) = vec{-m0, -m1};
-            *reinterpret_cast<vec *>(&L.mp[buf][r][2]) = vec{-m2, -m3};
+            *reinterpret_cast<vec *>(&L.mp[par][r][0]) = vec{-m0, -m1};
+            *reinterpret_cast<vec *>(&L.mp[par][r][2]) = vec{-m2, -m3};
           } else {
-            *reinterpret_cast<vec *>(&L.mp[buf][r][0]) = vec{-m0, -m1, -m2, -m3};
+            *reinterpret_cast<vec *>(&L.mp[par][r][0]) = vec{-m0, -m1, -m2, -m3};
           }
 #pragma unroll
-          for (int c = 0; c < 4; ++c) { L.pan[buf][c][r] = x[c][i]; }
+          for (int c = 0; c < 4; ++c) { L.pan[b3][c][r] = x[c][i]; }
         }
       }
-      lds_barrier(); // pv, mp[buf] and the factored panel are published
-      lds_barrier(); // the pivot rows and the next panel's columns are in LDS
+#pragma unroll
+      for (int s = 0; s < 4; ++s) { pq[s] = ps[s]; }
+      lds_barrier(); // pv[par], mp[par] and the factored panel in pan[b3] are published
     }
     lds_barrier();
+    err = singular ? kErrSingular : 0;
   }
-  return singular;
+  return err;
 }
 template <int B, class T = double>
 __global__ __launch_bounds__((GjM<B, T>::threads())) void k_block_inverse_m(T *__restrict__ blocks,
@@ -1639,7 +1662,8 @@ __global__ __launch_bounds__((GjM<B, T>::threads())) void k_block_inverse_m(T *_
                                                                             int *__restrict__ flags) {
   __shared__ GjMLds<B, T> L;
   const int blk = index ? index[blockIdx.x] : blockIdx.x;
-  if (gjm_inverse<B, T>(blocks + size_t(blk) * B * B, L) && (threadIdx.x & 63) == 0) { atomicOr(flags, kErrSingular); }
+  const int err = gjm_inverse<B, T>(blocks + size_t(blk) * B * B, L);
+  if (err && (threadIdx.x & 63) == 0) { atomicOr(flags, err); }
 }
 
 // M_ij = inv(U_ii) U_ij in place for every block right of the diagonal
