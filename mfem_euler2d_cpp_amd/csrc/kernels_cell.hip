@@ -553,39 +553,79 @@ __global__ __launch_bounds__(cell_threads<Q>()) void k_hdg_build(Geo g, const Fr
 // src/operators.cpp:121-142). Dense(B_e) column (side, qq, c) holds
 // phi_a M^{-1} sf at rows 4*node_a + c of the normal line through (side, qq),
 // so each X column combines q columns of A_e^{-1}. Output [X | y], m x (4t+1).
+// One workgroup a cell takes A_e^{-1} a chunk of SR rows at a time into LDS
+// (each entry read from memory once, runs of SR values of a column), forms
+// those rows of X there, and sums the rows of y over NP parts (k = p, p +
+// NP, ..., then the parts in order). The weights of Dense(B_e) are formed
+// once a cell.
+template <int Q> constexpr int solve_rows() { return Q <= 7 ? 32 : 16; }
 template <int Q>
-__global__ __launch_bounds__(cell_threads<Q>()) void k_hdg_solve(Geo g, const Frozen *__restrict__ fz,
+__global__ __launch_bounds__(256) void k_hdg_solve(Geo g, const Frozen *__restrict__ fz,
                                                    const double *__restrict__ Ainv, const double *__restrict__ res,
                                                    double *__restrict__ Xy) {
-  constexpr int P = Q * Q, M = 4 * P, T4 = 16 * Q;
+  constexpr int P = Q * Q, M = 4 * P, T4 = 16 * Q, SR = solve_rows<Q>(), NT = 256, NP = NT / SR;
   __shared__ double sf[4 * Q];
+  __shared__ double bw[4 * Q * Q]; // weight of node a on line (side, qq): bw[(side * Q + qq) * Q + a]
   __shared__ double rs[M];
-  const int e = blockIdx.x;
-  for (int fp = threadIdx.x; fp < 4 * Q; fp += blockDim.x) {
+  __shared__ double ch[M * SR]; // rows r0 .. r0 + SR - 1 of A^{-1}: ch[k * SR + rr]
+  __shared__ double yp[NP * SR];
+  const int e = blockIdx.x, t = threadIdx.x;
+  for (int fp = t; fp < 4 * Q; fp += NT) {
     const int side = fp / Q, qq = fp % Q, left = side & 1;
     const Frozen z = fz[face_of(g, e, side) * Q + qq];
     const double scale = g.dt * length_of(g, side) / g.area * g.w1[qq];
     sf[fp] = -scale * (left ? z.a : z.b);
   }
-  for (int i = threadIdx.x; i < M; i += blockDim.x) { rs[i] = res[size_t(e) * M + i]; }
-  __syncthreads();
+  for (int i = t; i < M; i += NT) { rs[i] = res[size_t(e) * M + i]; }
+  __syncthreads(); // sf is complete
+  for (int idx = t; idx < 4 * Q * Q; idx += NT) {
+    const int fp = idx / Q, a = idx % Q, side = fp / Q, qq = fp % Q;
+    const int node = line_node<Q>(side, qq, a);
+    bw[idx] = side_value<Q, double>(g, side, a) * (1. / (g.w1[node % Q] * g.w1[node / Q])) * sf[fp];
+  }
   const double *ai = Ainv + size_t(e) * M * M;
   double *xb = Xy + size_t(e) * M * (T4 + 1);
-  for (int idx = threadIdx.x; idx < M * (T4 + 1); idx += blockDim.x) {
-    const int row = idx % M, col = idx / M;
-    double v = 0.;
-    if (col == T4) {
-      for (int k = 0; k < M; ++k) { v += ai[size_t(k) * M + row] * rs[k]; }
-    } else {
+  // the next chunk is loaded into registers while this one is used
+  constexpr int LPT = (M * SR + NT - 1) / NT;
+  double pf[LPT];
+  auto load_chunk = [&](int r0) {
+#pragma unroll
+    for (int l = 0; l < LPT; ++l) {
+      const int idx = t + l * NT, k = idx / SR, rr = idx % SR;
+      pf[l] = (idx < M * SR && r0 + rr < M) ? ai[size_t(k) * M + r0 + rr] : 0.;
+    }
+  };
+  load_chunk(0);
+  for (int r0 = 0; r0 < M; r0 += SR) {
+#pragma unroll
+    for (int l = 0; l < LPT; ++l) {
+      const int idx = t + l * NT;
+      if (idx < M * SR) { ch[idx] = pf[l]; }
+    }
+    __syncthreads(); // the chunk (and sf, rs, bw) are in LDS
+    if (r0 + SR < M) { load_chunk(r0 + SR); }
+    {
+      const int rr = t % SR, part = t / SR;
+      double v = 0.;
+      for (int k = part; k < M; k += NP) { v += ch[k * SR + rr] * rs[k]; }
+      yp[part * SR + rr] = v;
+    }
+    for (int idx = t; idx < SR * T4; idx += NT) {
+      const int rr = idx % SR, col = idx / SR, row = r0 + rr;
       const int side = col / (4 * Q), qq = (col / 4) % Q, c = col % 4;
+      double v = 0.;
       for (int a = 0; a < Q; ++a) {
         const int node = line_node<Q>(side, qq, a);
-        const double bval = side_value<Q, double>(g, side, a) * (1. / (g.w1[node % Q] * g.w1[node / Q])) *
-                            sf[side * Q + qq];
-        v += ai[size_t(4 * node + c) * M + row] * bval;
+        v += ch[(4 * node + c) * SR + rr] * bw[(side * Q + qq) * Q + a];
       }
+      if (row < M) { xb[row + size_t(M) * col] = v; }
     }
-    xb[idx] = v;
+    __syncthreads(); // every read of ch is done; yp is complete
+    if (t < SR && r0 + t < M) {
+      double v = 0.;
+      for (int part = 0; part < NP; ++part) { v += yp[part * SR + t]; }
+      xb[r0 + t + size_t(M) * T4] = v;
+    }
   }
 }
 
@@ -855,7 +895,7 @@ void launch_hdg_build(const Geo &g, const Frozen *fz, const double *u, double *A
 }
 void launch_hdg_solve(const Geo &g, const Frozen *fz, const double *Ainv, const double *res, double *Xy,
                       hipStream_t s) {
-  EG_DISPATCH_Q(g.q, (k_hdg_solve<Q><<<g.n, cell_threads<Q>(), 0, s>>>(g, fz, Ainv, res, Xy)));
+  EG_DISPATCH_Q(g.q, (k_hdg_solve<Q><<<g.n, 256, 0, s>>>(g, fz, Ainv, res, Xy)));
 }
 void launch_hdg_panels(const Geo &g, const double *Xy, const double *C, double *panels, double *cy,
                        hipStream_t s) {
