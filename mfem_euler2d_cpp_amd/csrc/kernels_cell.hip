@@ -632,34 +632,62 @@ __global__ __launch_bounds__(256) void k_hdg_solve(Geo g, const Frozen *__restri
 // Condensed products (src/condensation.cpp:131-148, src/operators.cpp:7-26):
 // S_e = C_e X_e written as the per-face row panels, and C_e y_e.
 // Panel storage: [face][role(0 left cell,1 right cell)][col 4t][row t].
+// One workgroup a cell takes [X | y] CC columns at a time into LDS (each
+// column read from memory once; the next chunk in registers meanwhile) and
+// forms those columns of S_e there, with the sums in the order of the
+// per-entry form.
 template <int Q>
-__global__ __launch_bounds__(cell_threads<Q>()) void k_hdg_panels(Geo g, const double *__restrict__ Xy,
+__global__ __launch_bounds__(256) void k_hdg_panels(Geo g, const double *__restrict__ Xy,
                                                     const double *__restrict__ C, double *__restrict__ panels,
                                                     double *__restrict__ cy) {
-  constexpr int P = Q * Q, M = 4 * P, T = 4 * Q, T4 = 16 * Q;
-  const int e = blockIdx.x;
+  constexpr int P = Q * Q, M = 4 * P, T = 4 * Q, T4 = 16 * Q, CC = 16, NT = 256;
+  constexpr int LPT = (M * CC + NT - 1) / NT;
+  __shared__ double xs[CC * M];   // columns c0 .. c0 + CC - 1 of [X | y]
+  __shared__ double cl[T4 * 4];   // C_e: 16 values a face point
+  const int e = blockIdx.x, t = threadIdx.x;
   const double *xb = Xy + size_t(e) * M * (T4 + 1);
   const double *cb = C + size_t(e) * T4 * 4;
-  for (int idx = threadIdx.x; idx < T4 * (T4 + 1); idx += blockDim.x) {
-    const int row = idx % T4, col = idx / T4;
-    const int side = row / T, qq = (row / 4) % Q, c = row % 4;
-    const double *xc = xb + size_t(col) * M;
-    double val[4] = {0, 0, 0, 0};
-    for (int a = 0; a < Q; ++a) {
-      const double phi = side_value<Q, double>(g, side, a);
-      const double *xv = xc + 4 * line_node<Q>(side, qq, a);
-      for (int d = 0; d < 4; ++d) { val[d] += phi * xv[d]; }
+  for (int i = t; i < T4 * 4; i += NT) { cl[i] = cb[i]; }
+  double pf[LPT];
+  auto load_chunk = [&](int c0) {
+#pragma unroll
+    for (int l = 0; l < LPT; ++l) {
+      const int idx = t + l * NT;
+      pf[l] = (idx < M * CC && c0 + idx / M <= T4) ? xb[size_t(c0) * M + idx] : 0.;
     }
-    const double *fl = cb + (side * Q + qq) * 16;
-    double res = 0.;
-    for (int d = 0; d < 4; ++d) { res += fl[c + 4 * d] * val[d]; }
-    if (col == T4) {
-      cy[size_t(e) * T4 + row] = res;
-    } else {
-      const int f = face_of(g, e, side);
-      const int role = (side & 1) ? 0 : 1;
-      panels[((size_t(f) * 2 + role) * T4 + col) * T + (row % T)] = res;
+  };
+  load_chunk(0);
+  for (int c0 = 0; c0 <= T4; c0 += CC) {
+#pragma unroll
+    for (int l = 0; l < LPT; ++l) {
+      const int idx = t + l * NT;
+      if (idx < M * CC) { xs[idx] = pf[l]; }
     }
+    __syncthreads(); // the chunk (and C_e) are in LDS
+    if (c0 + CC <= T4) { load_chunk(c0 + CC); }
+    for (int idx = t; idx < T4 * CC; idx += NT) {
+      const int row = idx % T4, cc = idx / T4, col = c0 + cc;
+      if (col > T4) { continue; }
+      const int side = row / T, qq = (row / 4) % Q, c = row % 4;
+      const double *xc = xs + cc * M;
+      double val[4] = {0, 0, 0, 0};
+      for (int a = 0; a < Q; ++a) {
+        const double phi = side_value<Q, double>(g, side, a);
+        const double *xv = xc + 4 * line_node<Q>(side, qq, a);
+        for (int d = 0; d < 4; ++d) { val[d] += phi * xv[d]; }
+      }
+      const double *fl = cl + (side * Q + qq) * 16;
+      double res = 0.;
+      for (int d = 0; d < 4; ++d) { res += fl[c + 4 * d] * val[d]; }
+      if (col == T4) {
+        cy[size_t(e) * T4 + row] = res;
+      } else {
+        const int f = face_of(g, e, side);
+        const int role = (side & 1) ? 0 : 1;
+        panels[((size_t(f) * 2 + role) * T4 + col) * T + (row % T)] = res;
+      }
+    }
+    __syncthreads(); // every read of the chunk is done
   }
 }
 
@@ -899,7 +927,7 @@ void launch_hdg_solve(const Geo &g, const Frozen *fz, const double *Ainv, const 
 }
 void launch_hdg_panels(const Geo &g, const double *Xy, const double *C, double *panels, double *cy,
                        hipStream_t s) {
-  EG_DISPATCH_Q(g.q, (k_hdg_panels<Q><<<g.n, cell_threads<Q>(), 0, s>>>(g, Xy, C, panels, cy)));
+  EG_DISPATCH_Q(g.q, (k_hdg_panels<Q><<<g.n, 256, 0, s>>>(g, Xy, C, panels, cy)));
 }
 void launch_hdg_face(const Geo &g, const Frozen *fz, const double *cy, double *rhs, double *D, hipStream_t s) {
   const int nb = (g.nf * g.t + 255) / 256;
