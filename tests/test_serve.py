@@ -159,13 +159,18 @@ def test_batched_draws_alone_equal_in_mix(gpu):
 
 
 @pytest.mark.gpu
-def test_grouped_arnoldi_steps_same_results(gpu):
+def test_grouped_arnoldi_steps_same_results(gpu, monkeypatch):
     """The batched form (eg_serve_batched: the Arnoldi steps of concurrent
     condensed solves launched together as graph replays of the grouped
     kernels, the GMRES rotations on the device): the six instances, in mixed
     precision and in FP64, each bitwise the same alone and in the mix; the
     same updates as their own launches (eg_serve, rotations on the host) and
-    states within 1e-12 of them; steps of several solves shared launches."""
+    states within 1e-12 of them; steps of several solves shared launches.
+    A flush waits up to 2 ms for the other solves' steps (150 us by
+    default), so that steps of several solves meet however fast each solve's
+    other kernels run; the window changes when steps launch, not what they
+    compute."""
+    monkeypatch.setenv("EG_BATCH_WINDOW_US", "2000")
     cases, _ = _batched_six()
     for prec in ("mixed", "fp64"):
         reqs = [(c + f" --precision {prec}", "hdg", 0.) for c in cases]
